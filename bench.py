@@ -1,0 +1,147 @@
+#!/usr/bin/env python3
+"""Throughput of the FCFB + Welch hot path on one MI355X.
+
+A step is one pass of the path over the whole synthetic stream of the named
+workload (default ``rx888_64k``: 2^26 real s16 samples, 65536-point FFT with
+25 % overlap -> 1360 blocks, 64 channels of L = 256, Welch over 32 blocks),
+with the raw input resident in HBM before timing starts.  ``value`` is input
+MS/s = blocks * hop / step time.
+
+Prints one JSON line with the metric, a ``roofline`` object for the dominant
+kernel (algorithmic bytes / its average event-timed duration, against the
+8 TB/s HBM peak), a ``path`` object for the whole path, and a
+``cpu_baseline`` measured on this host's cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBPS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="rx888_64k")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="target CPU work for the baseline sample (0 disables it)")
+    return ap.parse_args()
+
+
+def traffic_from_profiles(workload: str):
+    """HBM bytes per launch of the forward kernel measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration), if a
+    summary for this workload has been committed under profiles/."""
+    p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        d = json.load(open(p))
+        return d.get("fwd_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(wl, x_host, seconds: float):
+    """The oracle's restatement of the reference path timed on host cores."""
+    sys.path.insert(0, ROOT)
+    from oracle import cpu_port
+    return cpu_port.time_workload(wl, x_host, seconds)
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print(f"bench.py runs on one GPU only (asked for {max(args.gpus, world)}): the FCFB + Welch "
+              "path is measured per GPU; see the sharding note in DESIGN.md.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, ROOT)
+    from spektri_amd import workloads as W
+    from spektri_amd.dsp import DspParams, DspState
+
+    wl = W.WORKLOADS[args.workload]()
+    dev = torch.device("cuda:0")
+    x = W.make_signal_torch(wl, dev)
+    torch.cuda.synchronize()
+
+    st, _ = DspState.init(DspParams(complex=False, fs_in=wl.fs, fft_size=wl.fft_size,
+                                    ffts_per_buf=wl.ffts_per_buf, spectrum_averages=wl.averages,
+                                    max_blocks=wl.nblocks))
+    for first, L, w in wl.channels:
+        st.add_channel(first, L, w)
+    stream = torch.cuda.current_stream(dev)
+    st.set_stream(stream.cuda_stream)
+
+    ptr, n = x.data_ptr(), wl.nsamples
+    for _ in range(args.warmup):
+        st.process_device(ptr, n)
+    torch.cuda.synchronize()
+    st.enable_timing(True)
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record(stream)
+    for _ in range(args.steps):
+        st.process_device(ptr, n)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms_step = t0.elapsed_time(t1) / args.steps
+    kern = st.last_timing()
+    nb = st.last_blocks()
+    assert nb == wl.nblocks
+
+    H, K, A = wl.hop, wl.bins, wl.averages
+    fwd_bytes = nb * (H * 2 + K * 8 + K * 4 / A)
+    chan_bytes = nb * sum(L * 8 + (3 * L // 4) * 8 for _, L, _ in wl.channels)
+    path_bytes = nb * wl.bytes_per_block()
+    value = nb * H / (ms_step * 1e-3) / 1e6
+    fwd_gbps = fwd_bytes / (kern["fwd_welch_ms"] * 1e-3) / 1e9
+    chan_gbps = chan_bytes / (kern["channels_ms"] * 1e-3) / 1e9
+    path_gbps = path_bytes / (ms_step * 1e-3) / 1e9
+    b_per_sample = wl.bytes_per_block() / H
+    roofline_msps = HBM_PEAK_GBPS * 1e9 / b_per_sample / 1e6
+    traffic = traffic_from_profiles(args.workload)
+    dominant = "fwd" if kern["fwd_welch_ms"] >= kern["channels_ms"] else "chan"
+    roof = {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBPS,
+            "kernel": f"fwd_real_kernel<M={wl.fft_size // 2}>" if dominant == "fwd" else "chan_kernel<L=256>",
+            "achieved": round(fwd_gbps if dominant == "fwd" else chan_gbps, 1),
+            "traffic": traffic if dominant == "fwd" else None}
+    roof["frac"] = round(roof["achieved"] / HBM_PEAK_GBPS, 4)
+
+    out = {
+        "metric": "input MS/s through FCFB+Welch, 2^16 FFT, 64 ch; % of HBM roofline",
+        "value": round(value, 1), "unit": "MS/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": wl.describe(),
+        "roofline": roof,
+        "path": {"algorithmic_bytes_per_step": int(path_bytes), "achieved_GBps": round(path_gbps, 1),
+                 "frac": round(path_gbps / HBM_PEAK_GBPS, 4), "bytes_per_input_sample": round(b_per_sample, 3),
+                 "roofline_MSps": round(roofline_msps, 1)},
+        "kernels_ms": {k: round(v, 4) for k, v in kern.items()},
+        "kernels_GBps": {"fwd_welch": round(fwd_gbps, 1), "channels": round(chan_gbps, 1)},
+    }
+    st.close()
+    if args.cpu_seconds > 0:
+        x_host = x.cpu().numpy()
+        del x
+        out["cpu_baseline"] = cpu_baseline(wl, x_host, args.cpu_seconds)
+        if out["cpu_baseline"].get("value"):
+            out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,140 @@
+/*
+ * spektri_gpu.h -- C ABI of the MI355X (gfx950) FCFB + Welch hot path.
+ *
+ * This library replaces the compute seam of spektri's DSP layer:
+ *   DspState::init(DspParams) -> (DspState, InputBufferSize)   spektri/src/dsp.rs:76-112
+ *   DspState::process_real / process_complex                   spektri/src/dsp.rs:114-160
+ *     MultiFft::process_*        (forward FFTs)                spektri/src/dsp/multifft.rs:34-96
+ *     SpectrumAccumulator::accumulate (Welch |X|^2 sums)        spektri/src/dsp/spectrum.rs:82-116
+ *     Fcfb::add_filter / Fcfb::process / FilterDsp::process     spektri/src/dsp/fcfb.rs:41-100,154-188
+ * The caller keeps framing of the raw byte stream (read_exact), the CLI,
+ * the dB quantiser, the wire format and ZeroMQ/file output; it hands raw
+ * input bytes in and reads channel samples and averaged spectra out.
+ *
+ * Conventions: every entry point returns an spk_status; nothing throws,
+ * panics or aborts across the boundary.  A context is single-threaded
+ * (like the ZeroMQ socket it feeds).  The caller owns host buffers; the
+ * library owns device memory, its HIP stream (unless one is set) and its
+ * kernels.  All arithmetic is FP32.
+ */
+#ifndef SPEKTRI_GPU_H
+#define SPEKTRI_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum spk_status {
+  SPK_OK = 0,
+  SPK_ERR_INVALID = 1,      /* bad argument (size, pointer, range) */
+  SPK_ERR_UNSUPPORTED = 2,  /* valid for the reference, not implemented here */
+  SPK_ERR_DEVICE = 3,       /* a HIP call failed */
+  SPK_ERR_STATE = 4,        /* call out of order (e.g. output before process) */
+  SPK_ERR_CAPACITY = 5      /* more blocks / records than the context was sized for */
+} spk_status;
+
+/* DataFormat codes of spektri/src/dsp/data.rs:81-95. */
+#define SPK_FMT_S16LE 0x08
+#define SPK_FMT_CF32LE 0x5C
+
+/* Mirrors DspParams (spektri/src/dsp.rs:24-36) minus fs/fc, which only
+ * the frequency<->bin helpers below need, and minus the filter list, which
+ * is added with spk_add_channel. */
+typedef struct spk_config {
+  uint32_t fft_size;          /* N: 4096..65536 (real input) */
+  uint32_t input_format;      /* SPK_FMT_S16LE; SPK_FMT_CF32LE is reserved */
+  float scaling;              /* input scaling; 0 = input_format_scaling(fmt) */
+  uint32_t ffts_per_buf;      /* F: blocks per spk_process call */
+  uint32_t spectrum_averages; /* A: blocks per Welch record */
+  uint32_t max_blocks;        /* capacity for spk_process_device; 0 = F */
+} spk_config;
+
+/* InputBufferSize (spektri/src/dsp.rs:38-42), in samples. */
+typedef struct spk_buffer_size {
+  uint64_t new_samples; /* F * 3N/4 */
+  uint64_t overlap;     /* N / 4 */
+  uint64_t total;       /* overlap + new */
+} spk_buffer_size;
+
+typedef struct spk_ctx spk_ctx;
+
+/* DspState::init (spektri/src/dsp.rs:76-112). */
+spk_status spk_create(const spk_config* cfg, spk_ctx** out);
+spk_status spk_destroy(spk_ctx* ctx);
+spk_status spk_buffer_size_of(const spk_ctx* ctx, spk_buffer_size* out);
+/* Message of the last failed call on this context (static string on NULL). */
+const char* spk_last_error(const spk_ctx* ctx);
+
+/* Fcfb::add_filter after freq_to_bins_exact (spektri/src/dsp/fcfb.rs:41-64,137-152).
+ * first_bin: number of the channel's lowest bin (BinNumbers.first, may be < 0);
+ * bins: IFFT size L, a power of two in [8, 8192] and <= N (the reference
+ * accepts any multiple of 4; other sizes return SPK_ERR_UNSUPPORTED);
+ * weights: L floats in j order (weights[j] multiplies bin first + j), or NULL
+ * for the reference's raised-cosine weights (fcfb.rs:197-203). */
+spk_status spk_add_channel(spk_ctx* ctx, int64_t first_bin, uint32_t bins, const float* weights,
+                           uint32_t* id);
+/* Replace a channel's weights (same L). */
+spk_status spk_set_channel_weights(spk_ctx* ctx, uint32_t id, const float* weights);
+
+/* DspState::process_real / process_complex (spektri/src/dsp.rs:114-160) on one
+ * buffer: exactly F * 3N/4 new raw samples from host memory (nbytes must
+ * match).  The N/4-sample history is kept on the device (zeros at start,
+ * main.rs:165).  Synchronous. */
+spk_status spk_process(spk_ctx* ctx, const void* raw_new, size_t nbytes);
+
+/* Same path on a device-resident stream of nsamples raw samples (4-byte
+ * aligned): processes floor(nsamples / (F * 3N/4)) * F blocks, like the
+ * reference's main loop over that many bytes (main.rs:171-179), in one
+ * launch sequence on the context's stream.  Asynchronous. */
+spk_status spk_process_device(spk_ctx* ctx, const void* d_raw, uint64_t nsamples);
+
+/* Blocks processed by the last process call. */
+spk_status spk_last_blocks(const spk_ctx* ctx, uint32_t* nblocks);
+
+/* FilterDsp::process output of the last call for channel id: nblocks * 3L/4
+ * complex samples as interleaved f32 (re, im), i.e. the record body of
+ * fcfb.rs:177-187.  cap and *n count floats. */
+spk_status spk_channel_output(spk_ctx* ctx, uint32_t id, float* out, size_t cap, size_t* n);
+/* Device pointer and float count of the same data (valid until the next call). */
+spk_status spk_channel_output_device(spk_ctx* ctx, uint32_t id, const float** dptr, size_t* n);
+
+/* Welch records completed by the last call (spectrum.rs:113-160): each is K
+ * floats acc / A (K = N/2 + 1).  *nrec receives the record count; cap counts
+ * floats. */
+spk_status spk_spectrum_records(spk_ctx* ctx, float* out, size_t cap, uint32_t* nrec);
+spk_status spk_spectrum_records_device(spk_ctx* ctx, const float** dptr, uint32_t* nrec);
+
+/* Forward spectra of the last call: nblocks rows of K complex f32 with a
+ * row stride of *stride complex values (the reference's fft_result_buf). */
+spk_status spk_spectra_device(spk_ctx* ctx, const float** dptr, uint64_t* stride);
+
+/* Clear history, Welch accumulators and counters (as a fresh DspState). */
+spk_status spk_reset(spk_ctx* ctx);
+/* Run on the caller's HIP stream (hipStream_t passed as void*); NULL
+ * returns to the context's own stream. */
+spk_status spk_set_stream(spk_ctx* ctx, void* stream);
+spk_status spk_synchronize(spk_ctx* ctx);
+
+/* Per-kernel timing from HIP events recorded on the context's stream around
+ * each launch: average ms per call since spk_enable_timing of ms[0] the
+ * forward FFT + Welch kernel, ms[1] all channel launches, ms[2] the Welch
+ * finaliser.  Enabling adds events only; spk_last_timing synchronizes. */
+spk_status spk_enable_timing(spk_ctx* ctx, int on);
+spk_status spk_last_timing(spk_ctx* ctx, float ms[3]);
+
+/* freq_to_bins / bins_to_freq / freq_to_bins_exact (spektri/src/dsp/fcfb.rs:225-285).
+ * Return SPK_ERR_INVALID when the reference returns None. */
+spk_status spk_freq_to_bins(double fs_in, double fc_in, uint32_t fft_size, double fs_out,
+                            double fc_out, uint32_t* bins, int64_t* first);
+spk_status spk_bins_to_freq(double fs_in, double fc_in, uint32_t fft_size, uint32_t bins,
+                            int64_t first, double* fs_out, double* fc_out);
+spk_status spk_freq_to_bins_exact(double fs_in, double fc_in, uint32_t fft_size, double fs_out,
+                                  double fc_out, uint32_t* bins, int64_t* first);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPEKTRI_GPU_H */

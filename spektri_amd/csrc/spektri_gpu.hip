@@ -1,0 +1,627 @@
+// C ABI of the FCFB + Welch hot path (declared in include/spektri_gpu.h).
+//
+// A context owns: the N/4-sample input history, a staging buffer for host
+// input, the per-block spectrum buffer, a ring of Welch accumulators, the
+// Welch record buffer, per-channel output buffers, and the twiddle tables.
+// One process call launches: the forward kernel over all blocks (fused s16
+// load, FFT, spectrum write, Welch accumulation), one channel kernel per
+// distinct IFFT size, and the Welch finaliser for groups completed.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/spektri_gpu.h"
+#include "kernels.h"
+
+using spk::cpx;
+
+namespace {
+
+const char* kNullCtxMsg = "null context";
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t n) {
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+      bytes = 0;
+    }
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, n);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  template <class T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+struct Channel {
+  int64_t first;
+  int L;
+  std::vector<float> w;
+};
+
+struct ChanGroup {  // channels sharing one IFFT size
+  int L;
+  std::vector<uint32_t> ids;
+  DevBuf desc;   // spk::ChanDesc[ids.size()]
+  DevBuf tw2, tw3;
+};
+
+struct FwdPlan {
+  int M, R, R1, R2;
+};
+const FwdPlan kFwdPlans[] = {
+    {2048, 16, 8, 16}, {4096, 16, 16, 16}, {8192, 32, 16, 16}, {16384, 32, 16, 32}, {32768, 32, 32, 32}};
+
+struct ChanPlan {
+  int L, R, R1, R2, R3;
+};
+const ChanPlan kChanPlans[] = {{8, 4, 4, 2, 1},       {16, 4, 4, 4, 1},      {32, 8, 8, 4, 1},
+                               {64, 8, 8, 8, 1},      {128, 16, 16, 8, 1},   {256, 16, 16, 16, 1},
+                               {512, 32, 32, 16, 1},  {1024, 32, 32, 32, 1}, {2048, 32, 32, 32, 2},
+                               {4096, 16, 16, 16, 16}, {8192, 32, 32, 16, 16}};
+
+const ChanPlan* find_chan_plan(int L) {
+  for (const auto& p : kChanPlans)
+    if (p.L == L) return &p;
+  return nullptr;
+}
+const FwdPlan* find_fwd_plan(int M) {
+  for (const auto& p : kFwdPlans)
+    if (p.M == M) return &p;
+  return nullptr;
+}
+
+// Pass table for radix r over sub-transforms of size p: entry [(j-1) p + k]
+// = exp(sign 2 pi i j k / (p r)), evaluated in double with the exponent
+// reduced exactly in integers, rounded to f32.
+std::vector<cpx> pass_table(int r, int p, int sign) {
+  std::vector<cpx> t;
+  if (p <= 1) return t;
+  t.resize((size_t)(r - 1) * p);
+  const long n = (long)p * r;
+  for (int j = 1; j < r; ++j)
+    for (int k = 0; k < p; ++k) {
+      const long e = ((long)j * k) % n;
+      const double a = 2.0 * M_PI * (double)e / (double)n;
+      t[(size_t)(j - 1) * p + k] = cpx{(float)std::cos(a), (float)(sign * std::sin(a))};
+    }
+  return t;
+}
+
+int bytes_per_sample(uint32_t fmt) { return fmt == SPK_FMT_S16LE ? 2 : (fmt == SPK_FMT_CF32LE ? 8 : 0); }
+
+}  // namespace
+
+struct spk_ctx {
+  spk_config cfg{};
+  int N = 0, M = 0, K = 0, H = 0, O = 0, F = 0, A = 0, bps = 0;
+  bool complex_in = false;
+  float wscale = 0.f;
+  int max_blocks = 0;
+  const FwdPlan* fplan = nullptr;
+
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+
+  DevBuf hist, head, staging, spec, welch_acc, welch_out, chan_out, weights;
+  DevBuf ftw2, ftw3, ftwu;
+  long long spec_stride = 0, welch_stride = 0;
+  int ring = 0, welch_out_cap = 0;
+
+  std::vector<Channel> chans;
+  std::vector<long long> chan_out_off;  // cpx offsets per channel (capacity layout)
+  std::map<int, ChanGroup> groups;
+  bool chans_dirty = true;
+
+  long long blocks_done = 0;  // stream blocks consumed so far
+  uint32_t last_blocks = 0;
+  uint32_t last_records = 0;
+  bool have_output = false;
+
+  // Per-kernel timing: 4 events per call (before the forward kernel, after
+  // it, after the channel kernels, after the finaliser), pooled.
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;   // events recorded since timing was enabled
+  size_t ev_done = 0;   // events already folded into ms_sum
+  double ms_sum[3] = {0, 0, 0};
+  uint32_t timed_calls = 0;
+
+  std::string err;
+};
+
+namespace {
+
+spk_status fail(spk_ctx* c, spk_status s, const std::string& msg) {
+  if (c) c->err = msg;
+  return s;
+}
+
+spk_status hip_fail(spk_ctx* c, hipError_t e, const char* what) {
+  return fail(c, SPK_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                              \
+  do {                                                  \
+    hipError_t _e = (expr);                             \
+    if (_e != hipSuccess) return hip_fail(ctx, _e, #expr); \
+  } while (0)
+
+template <class T>
+hipError_t upload(DevBuf& b, const std::vector<T>& v) {
+  hipError_t e = b.alloc(v.size() * sizeof(T));
+  if (e != hipSuccess || v.empty()) return e;
+  return hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+std::vector<float> raised_cosine_weights(int L) {
+  // fcfb.rs:197-203 in f32: 0.5 - 0.5 cos(i * (2 pi / L))
+  std::vector<float> w(L);
+  const float f = (2.0f * (float)M_PI) / (float)L;
+  for (int i = 0; i < L; ++i) w[i] = 0.5f - 0.5f * std::cos((float)i * f);
+  return w;
+}
+
+// Lay out channel outputs and descriptors, upload weights and tables.
+spk_status rebuild_channels(spk_ctx* c) {
+  std::vector<float> wall;
+  c->chan_out_off.assign(c->chans.size(), 0);
+  long long off = 0;
+  std::vector<int> w_off(c->chans.size());
+  for (size_t i = 0; i < c->chans.size(); ++i) {
+    const Channel& ch = c->chans[i];
+    w_off[i] = (int)wall.size();
+    wall.insert(wall.end(), ch.w.begin(), ch.w.end());
+    c->chan_out_off[i] = off;
+    off += (long long)c->max_blocks * (3LL * ch.L / 4);
+  }
+  HIP_TRY(c, upload(c->weights, wall));
+  HIP_TRY(c, c->chan_out.alloc((size_t)off * sizeof(cpx)));
+  c->groups.clear();
+  for (size_t i = 0; i < c->chans.size(); ++i) {
+    ChanGroup& g = c->groups[c->chans[i].L];
+    g.L = c->chans[i].L;
+    g.ids.push_back((uint32_t)i);
+  }
+  for (auto& kv : c->groups) {
+    ChanGroup& g = kv.second;
+    const ChanPlan* p = find_chan_plan(g.L);
+    std::vector<spk::ChanDesc> d;
+    for (uint32_t id : g.ids) {
+      const Channel& ch = c->chans[id];
+      long long fm = ch.first % c->N;
+      if (fm < 0) fm += c->N;
+      d.push_back(spk::ChanDesc{c->chan_out_off[id], (int)fm, w_off[id]});
+    }
+    HIP_TRY(c, upload(g.desc, d));
+    HIP_TRY(c, upload(g.tw2, pass_table(p->R2, p->R1, +1)));
+    HIP_TRY(c, upload(g.tw3, pass_table(p->R3, p->R1 * p->R2, +1)));
+  }
+  c->chans_dirty = false;
+  return SPK_OK;
+}
+
+template <int M, int R, int R1, int R2>
+void launch_fwd(const spk::FwdParams& P, hipStream_t s) {
+  hipLaunchKernelGGL((spk::fwd_real_kernel<M, R, R1, R2>), dim3(P.nblocks), dim3(M / R), 0, s, P);
+}
+
+hipError_t dispatch_fwd(int M, const spk::FwdParams& P, hipStream_t s) {
+  switch (M) {
+    case 2048: launch_fwd<2048, 16, 8, 16>(P, s); break;
+    case 4096: launch_fwd<4096, 16, 16, 16>(P, s); break;
+    case 8192: launch_fwd<8192, 32, 16, 16>(P, s); break;
+    case 16384: launch_fwd<16384, 32, 16, 32>(P, s); break;
+    case 32768: launch_fwd<32768, 32, 32, 32>(P, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int L, int R, int R1, int R2, int R3>
+void launch_chan(const spk::ChanParams& P, hipStream_t s) {
+  constexpr int T = L / R;
+  constexpr int TPW = spk::kChanThreads / T;
+  const long long tiles = (long long)P.nch * P.nblocks;
+  const unsigned grid = (unsigned)((tiles + TPW - 1) / TPW);
+  hipLaunchKernelGGL((spk::chan_kernel<L, R, R1, R2, R3>), dim3(grid), dim3(spk::kChanThreads), 0, s, P);
+}
+
+hipError_t dispatch_chan(int L, const spk::ChanParams& P, hipStream_t s) {
+  switch (L) {
+    case 8: launch_chan<8, 4, 4, 2, 1>(P, s); break;
+    case 16: launch_chan<16, 4, 4, 4, 1>(P, s); break;
+    case 32: launch_chan<32, 8, 8, 4, 1>(P, s); break;
+    case 64: launch_chan<64, 8, 8, 8, 1>(P, s); break;
+    case 128: launch_chan<128, 16, 16, 8, 1>(P, s); break;
+    case 256: launch_chan<256, 16, 16, 16, 1>(P, s); break;
+    case 512: launch_chan<512, 32, 32, 16, 1>(P, s); break;
+    case 1024: launch_chan<1024, 32, 32, 32, 1>(P, s); break;
+    case 2048: launch_chan<2048, 32, 32, 32, 2>(P, s); break;
+    case 4096: launch_chan<4096, 16, 16, 16, 16>(P, s); break;
+    case 8192: launch_chan<8192, 32, 32, 16, 16>(P, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// One launch sequence over nb blocks whose samples start at src.
+spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
+  if (c->chans_dirty) {
+    spk_status st = rebuild_channels(c);
+    if (st != SPK_OK) return st;
+  }
+  hipStream_t s = c->stream;
+  hipEvent_t* ev = nullptr;
+  if (c->timing) {
+    if (c->ev_used + 4 > c->ev_pool.size()) {
+      for (int i = 0; i < 64; ++i) {
+        hipEvent_t e;
+        HIP_TRY(c, hipEventCreate(&e));
+        c->ev_pool.push_back(e);
+      }
+    }
+    ev = &c->ev_pool[c->ev_used];
+    c->ev_used += 4;
+  }
+
+  // Block 0 reads [history | first N - O new samples] from the head copy.
+  HIP_TRY(c, hipMemcpyAsync(c->head.p, c->hist.p, (size_t)c->O * c->bps, hipMemcpyDeviceToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(c->head.as<char>() + (size_t)c->O * c->bps, src, (size_t)(c->N - c->O) * c->bps,
+                            hipMemcpyDeviceToDevice, s));
+  spk::FwdParams P{};
+  P.src = reinterpret_cast<const int16_t*>(src);
+  P.head = c->head.as<int16_t>();
+  P.spec = c->spec.as<cpx>();
+  P.spec_stride = c->spec_stride;
+  P.tw2 = c->ftw2.as<cpx>();
+  P.tw3 = c->ftw3.as<cpx>();
+  P.twu = c->ftwu.as<cpx>();
+  P.welch_acc = c->welch_acc.as<float>();
+  P.welch_stride = c->welch_stride;
+  P.welch_ring = c->ring;
+  P.welch_A = c->A;
+  P.block0 = c->blocks_done;
+  P.wscale = c->wscale;
+  P.nblocks = nb;
+  P.H = c->H;
+  P.O = c->O;
+  if (ev) HIP_TRY(c, hipEventRecord(ev[0], s));
+  HIP_TRY(c, dispatch_fwd(c->M, P, s));
+  if (ev) HIP_TRY(c, hipEventRecord(ev[1], s));
+
+  for (auto& kv : c->groups) {
+    ChanGroup& g = kv.second;
+    spk::ChanParams Q{};
+    Q.spec = c->spec.as<cpx>();
+    Q.spec_stride = c->spec_stride;
+    Q.N = c->N;
+    Q.K = c->K;
+    Q.real_input = c->complex_in ? 0 : 1;
+    Q.weights = c->weights.as<float>();
+    Q.chans = g.desc.as<spk::ChanDesc>();
+    Q.nch = (int)g.ids.size();
+    Q.nblocks = nb;
+    Q.out = c->chan_out.as<cpx>();
+    Q.out_block_stride = 3LL * g.L / 4;
+    Q.tw2 = g.tw2.as<cpx>();
+    Q.tw3 = g.tw3.as<cpx>();
+    HIP_TRY(c, dispatch_chan(g.L, Q, s));
+  }
+  if (ev) HIP_TRY(c, hipEventRecord(ev[2], s));
+
+  const long long s_first = c->blocks_done / c->A;
+  const long long s_end = (c->blocks_done + nb) / c->A;
+  const int nrec = (int)(s_end - s_first);
+  if (nrec > 0) {
+    const long long total = (long long)nrec * c->K;
+    const unsigned grid = (unsigned)std::min<long long>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(spk::welch_finalize_kernel, dim3(grid), dim3(256), 0, s, c->welch_acc.as<float>(),
+                       c->welch_stride, c->ring, s_first, nrec, c->K, c->A, c->welch_out.as<float>(),
+                       (long long)c->K);
+    HIP_TRY(c, hipGetLastError());
+  }
+  if (ev) HIP_TRY(c, hipEventRecord(ev[3], s));
+
+  // New history: the last O samples consumed.
+  const char* hsrc = reinterpret_cast<const char*>(src) + ((long long)nb * c->H - c->O) * c->bps;
+  HIP_TRY(c, hipMemcpyAsync(c->hist.p, hsrc, (size_t)c->O * c->bps, hipMemcpyDeviceToDevice, s));
+
+  c->blocks_done += nb;
+  c->last_blocks = (uint32_t)nb;
+  c->last_records = (uint32_t)nrec;
+  c->have_output = true;
+  return SPK_OK;
+}
+
+// Rust f64::round: halfway cases away from zero.
+double round_half_away(double v) { return std::copysign(std::floor(std::fabs(v) + 0.5), v); }
+
+}  // namespace
+
+extern "C" {
+
+const char* spk_last_error(const spk_ctx* ctx) { return ctx ? ctx->err.c_str() : kNullCtxMsg; }
+
+spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
+  if (!cfg || !out) return SPK_ERR_INVALID;
+  *out = nullptr;
+  spk_ctx* c = new (std::nothrow) spk_ctx();
+  if (!c) return SPK_ERR_DEVICE;
+  c->cfg = *cfg;
+  const uint32_t N = cfg->fft_size;
+  if (cfg->input_format == SPK_FMT_CF32LE) {
+    delete c;
+    return SPK_ERR_UNSUPPORTED;
+  }
+  if (cfg->input_format != SPK_FMT_S16LE || N == 0 || (N & (N - 1)) != 0 || !find_fwd_plan((int)N / 2) ||
+      cfg->ffts_per_buf == 0 || cfg->spectrum_averages == 0) {
+    delete c;
+    return SPK_ERR_INVALID;
+  }
+  c->N = (int)N;
+  c->M = c->N / 2;
+  c->complex_in = false;
+  c->K = c->N / 2 + 1;
+  c->O = c->N / 4;
+  c->H = c->N - c->O;
+  c->F = (int)cfg->ffts_per_buf;
+  c->A = (int)cfg->spectrum_averages;
+  c->bps = bytes_per_sample(cfg->input_format);
+  const float scaling = cfg->scaling != 0.f ? cfg->scaling : 1.0f / 32767.0f;
+  c->wscale = scaling / (float)c->N;  // dsp.rs:70-73
+  c->max_blocks = (int)(cfg->max_blocks ? cfg->max_blocks : cfg->ffts_per_buf);
+  if (c->max_blocks < c->F) c->max_blocks = c->F;
+  c->fplan = find_fwd_plan(c->M);
+
+  auto cleanup = [&](spk_status s) {
+    spk_destroy(c);
+    return s;
+  };
+  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) return cleanup(SPK_ERR_DEVICE);
+  c->stream = c->own_stream;
+
+  c->spec_stride = ((long long)c->K + 63) / 64 * 64;
+  c->welch_stride = c->spec_stride;
+  c->ring = (c->max_blocks + c->A - 1) / c->A + 2;
+  c->welch_out_cap = c->max_blocks / c->A + 1;
+  if (c->hist.alloc((size_t)c->O * c->bps) != hipSuccess || c->head.alloc((size_t)c->N * c->bps) != hipSuccess ||
+      c->staging.alloc((size_t)c->F * c->H * c->bps) != hipSuccess ||
+      c->spec.alloc((size_t)c->max_blocks * c->spec_stride * sizeof(cpx)) != hipSuccess ||
+      c->welch_acc.alloc((size_t)c->ring * c->welch_stride * sizeof(float)) != hipSuccess ||
+      c->welch_out.alloc((size_t)c->welch_out_cap * c->K * sizeof(float)) != hipSuccess)
+    return cleanup(SPK_ERR_DEVICE);
+  if (hipMemset(c->hist.p, 0, c->hist.bytes) != hipSuccess ||
+      hipMemset(c->welch_acc.p, 0, c->welch_acc.bytes) != hipSuccess)
+    return cleanup(SPK_ERR_DEVICE);
+
+  const FwdPlan& p = *c->fplan;
+  std::vector<cpx> twu((size_t)p.R1 * p.R2 + 1);
+  for (size_t k = 0; k < twu.size(); ++k) {
+    const double a = 2.0 * M_PI * (double)k / (double)c->N;
+    twu[k] = cpx{(float)std::cos(a), (float)(-std::sin(a))};
+  }
+  if (upload(c->ftw2, pass_table(p.R2, p.R1, -1)) != hipSuccess ||
+      upload(c->ftw3, pass_table(p.R, p.R1 * p.R2, -1)) != hipSuccess || upload(c->ftwu, twu) != hipSuccess)
+    return cleanup(SPK_ERR_DEVICE);
+  *out = c;
+  return SPK_OK;
+}
+
+spk_status spk_destroy(spk_ctx* c) {
+  if (!c) return SPK_ERR_INVALID;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+  return SPK_OK;
+}
+
+spk_status spk_buffer_size_of(const spk_ctx* c, spk_buffer_size* out) {
+  if (!c || !out) return SPK_ERR_INVALID;
+  out->overlap = (uint64_t)c->O;
+  out->new_samples = (uint64_t)c->F * c->H;
+  out->total = out->overlap + out->new_samples;
+  return SPK_OK;
+}
+
+spk_status spk_add_channel(spk_ctx* c, int64_t first_bin, uint32_t bins, const float* weights, uint32_t* id) {
+  if (!c) return SPK_ERR_INVALID;
+  if (bins == 0 || bins % 4 != 0) return fail(c, SPK_ERR_INVALID, "IFFT size must be a positive multiple of 4");
+  if ((bins & (bins - 1)) != 0 || !find_chan_plan((int)bins) || (int)bins > c->N)
+    return fail(c, SPK_ERR_UNSUPPORTED, "IFFT size must be a power of two in [8, 8192] and <= N");
+  Channel ch;
+  ch.first = first_bin;
+  ch.L = (int)bins;
+  ch.w = weights ? std::vector<float>(weights, weights + bins) : raised_cosine_weights((int)bins);
+  c->chans.push_back(std::move(ch));
+  c->chans_dirty = true;
+  if (id) *id = (uint32_t)(c->chans.size() - 1);
+  return SPK_OK;
+}
+
+spk_status spk_set_channel_weights(spk_ctx* c, uint32_t id, const float* weights) {
+  if (!c || !weights) return SPK_ERR_INVALID;
+  if (id >= c->chans.size()) return fail(c, SPK_ERR_INVALID, "no such channel");
+  Channel& ch = c->chans[id];
+  ch.w.assign(weights, weights + ch.L);
+  c->chans_dirty = true;
+  return SPK_OK;
+}
+
+spk_status spk_process(spk_ctx* c, const void* raw_new, size_t nbytes) {
+  if (!c || !raw_new) return SPK_ERR_INVALID;
+  const size_t want = (size_t)c->F * c->H * c->bps;
+  if (nbytes != want) return fail(c, SPK_ERR_INVALID, "buffer must hold exactly F * 3N/4 samples");
+  HIP_TRY(c, hipMemcpyAsync(c->staging.p, raw_new, nbytes, hipMemcpyHostToDevice, c->stream));
+  spk_status st = run_blocks(c, c->staging.p, c->F);
+  if (st != SPK_OK) return st;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return SPK_OK;
+}
+
+spk_status spk_process_device(spk_ctx* c, const void* d_raw, uint64_t nsamples) {
+  if (!c || !d_raw) return SPK_ERR_INVALID;
+  if (reinterpret_cast<uintptr_t>(d_raw) % 4 != 0) return fail(c, SPK_ERR_INVALID, "input must be 4-byte aligned");
+  const uint64_t per_buf = (uint64_t)c->F * c->H;
+  const uint64_t nb = nsamples / per_buf * c->F;
+  if (nb == 0) return fail(c, SPK_ERR_INVALID, "fewer samples than one buffer");
+  if (nb > (uint64_t)c->max_blocks) return fail(c, SPK_ERR_CAPACITY, "more blocks than max_blocks");
+  return run_blocks(c, d_raw, (int)nb);
+}
+
+spk_status spk_last_blocks(const spk_ctx* c, uint32_t* nblocks) {
+  if (!c || !nblocks) return SPK_ERR_INVALID;
+  *nblocks = c->last_blocks;
+  return SPK_OK;
+}
+
+spk_status spk_channel_output_device(spk_ctx* c, uint32_t id, const float** dptr, size_t* n) {
+  if (!c || !dptr || !n) return SPK_ERR_INVALID;
+  if (id >= c->chans.size()) return fail(c, SPK_ERR_INVALID, "no such channel");
+  if (!c->have_output || c->chans_dirty) return fail(c, SPK_ERR_STATE, "no output yet");
+  *dptr = reinterpret_cast<const float*>(c->chan_out.as<cpx>() + c->chan_out_off[id]);
+  *n = (size_t)c->last_blocks * (3 * c->chans[id].L / 4) * 2;
+  return SPK_OK;
+}
+
+spk_status spk_channel_output(spk_ctx* c, uint32_t id, float* out, size_t cap, size_t* n) {
+  if (!c || !out || !n) return SPK_ERR_INVALID;
+  const float* d = nullptr;
+  spk_status st = spk_channel_output_device(c, id, &d, n);
+  if (st != SPK_OK) return st;
+  if (*n > cap) return fail(c, SPK_ERR_CAPACITY, "output buffer too small");
+  HIP_TRY(c, hipMemcpyAsync(out, d, *n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return SPK_OK;
+}
+
+spk_status spk_spectrum_records_device(spk_ctx* c, const float** dptr, uint32_t* nrec) {
+  if (!c || !dptr || !nrec) return SPK_ERR_INVALID;
+  *dptr = c->welch_out.as<float>();
+  *nrec = c->have_output ? c->last_records : 0;
+  return SPK_OK;
+}
+
+spk_status spk_spectrum_records(spk_ctx* c, float* out, size_t cap, uint32_t* nrec) {
+  if (!c || !nrec) return SPK_ERR_INVALID;
+  const float* d = nullptr;
+  spk_status st = spk_spectrum_records_device(c, &d, nrec);
+  if (st != SPK_OK) return st;
+  const size_t n = (size_t)*nrec * c->K;
+  if (n == 0) return SPK_OK;
+  if (!out || n > cap) return fail(c, SPK_ERR_CAPACITY, "output buffer too small");
+  HIP_TRY(c, hipMemcpyAsync(out, d, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return SPK_OK;
+}
+
+spk_status spk_spectra_device(spk_ctx* c, const float** dptr, uint64_t* stride) {
+  if (!c || !dptr || !stride) return SPK_ERR_INVALID;
+  *dptr = c->spec.as<float>();
+  *stride = (uint64_t)c->spec_stride;
+  return SPK_OK;
+}
+
+spk_status spk_reset(spk_ctx* c) {
+  if (!c) return SPK_ERR_INVALID;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemset(c->hist.p, 0, c->hist.bytes));
+  HIP_TRY(c, hipMemset(c->welch_acc.p, 0, c->welch_acc.bytes));
+  c->blocks_done = 0;
+  c->last_blocks = 0;
+  c->last_records = 0;
+  c->have_output = false;
+  return SPK_OK;
+}
+
+spk_status spk_set_stream(spk_ctx* c, void* stream) {
+  if (!c) return SPK_ERR_INVALID;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->stream = stream ? reinterpret_cast<hipStream_t>(stream) : c->own_stream;
+  return SPK_OK;
+}
+
+spk_status spk_synchronize(spk_ctx* c) {
+  if (!c) return SPK_ERR_INVALID;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return SPK_OK;
+}
+
+spk_status spk_enable_timing(spk_ctx* c, int on) {
+  if (!c) return SPK_ERR_INVALID;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->timing = on != 0;
+  c->ev_used = c->ev_done = 0;
+  c->timed_calls = 0;
+  for (double& v : c->ms_sum) v = 0;
+  return SPK_OK;
+}
+
+spk_status spk_last_timing(spk_ctx* c, float ms[3]) {
+  if (!c || !ms) return SPK_ERR_INVALID;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (; c->ev_done + 4 <= c->ev_used; c->ev_done += 4) {
+    hipEvent_t* e = &c->ev_pool[c->ev_done];
+    for (int i = 0; i < 3; ++i) {
+      float v = 0;
+      HIP_TRY(c, hipEventElapsedTime(&v, e[i], e[i + 1]));
+      c->ms_sum[i] += v;
+    }
+    ++c->timed_calls;
+  }
+  if (c->ev_done == c->ev_used) c->ev_used = c->ev_done = 0;  // recycle the pool
+  for (int i = 0; i < 3; ++i) ms[i] = c->timed_calls ? (float)(c->ms_sum[i] / c->timed_calls) : 0.f;
+  return SPK_OK;
+}
+
+spk_status spk_freq_to_bins(double fs_in, double fc_in, uint32_t fft_size, double fs_out, double fc_out,
+                            uint32_t* bins, int64_t* first) {
+  if (!bins || !first || fft_size == 0) return SPK_ERR_INVALID;
+  const double spacing = fs_in / (double)fft_size;
+  const int64_t b = (int64_t)(round_half_away(fs_out / spacing / 4.0) * 4.0);
+  const int64_t center = (int64_t)(round_half_away((fc_out - fc_in) / spacing / 4.0) * 4.0);
+  if (b <= 0) return SPK_ERR_INVALID;
+  *bins = (uint32_t)b;
+  *first = center - b / 2;
+  return SPK_OK;
+}
+
+spk_status spk_bins_to_freq(double fs_in, double fc_in, uint32_t fft_size, uint32_t bins, int64_t first,
+                            double* fs_out, double* fc_out) {
+  if (!fs_out || !fc_out || fft_size == 0) return SPK_ERR_INVALID;
+  const double spacing = fs_in / (double)fft_size;
+  *fs_out = spacing * (double)bins;
+  *fc_out = fc_in + spacing * (double)(first + (int64_t)(bins / 2));
+  return SPK_OK;
+}
+
+spk_status spk_freq_to_bins_exact(double fs_in, double fc_in, uint32_t fft_size, double fs_out, double fc_out,
+                                  uint32_t* bins, int64_t* first) {
+  uint32_t b = 0;
+  int64_t f = 0;
+  spk_status st = spk_freq_to_bins(fs_in, fc_in, fft_size, fs_out, fc_out, &b, &f);
+  if (st != SPK_OK) return st;
+  double fso = 0, fco = 0;
+  spk_bins_to_freq(fs_in, fc_in, fft_size, b, f, &fso, &fco);
+  if (fso != fs_out || fco != fc_out) return SPK_ERR_INVALID;
+  *bins = b;
+  *first = f;
+  return SPK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,224 @@
+"""Host-side mirror of spektri's DSP interface over the HIP C ABI.
+
+Names and argument meaning follow the reference's Rust types:
+``DspParams`` / ``InputBufferSize`` / ``DspState::init`` (spektri/src/dsp.rs:24-112),
+``FilterParams`` / ``BinNumbers`` / ``freq_to_bins*`` / ``bins_to_freq``
+(spektri/src/dsp/fcfb.rs:206-285) and ``SpectrumFormat`` (spectrum.rs:29-32).
+``DspState.process`` replaces ``process_real`` (dsp.rs:138-160): instead of a
+converted f32 buffer with overlap it takes the raw bytes of one read_exact
+(main.rs:176), because conversion and the overlap carry happen on the GPU.
+Outputs are returned rather than written to a socket; serialisation and I/O
+stay with the caller (spektri/src/dsp/data.rs, output.rs).
+"""
+from __future__ import annotations
+
+import ctypes
+import sys
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+SpectrumFormat = ("U8", "U16")
+
+
+@dataclass
+class FilterParams:
+    """fcfb.rs:206-210 (``output`` is the caller's concern).  ``weights`` is
+    an extension: L floats in j order, None for the reference raised cosine."""
+    fs_out: float
+    fc_out: float
+    weights: np.ndarray | None = None
+
+
+@dataclass
+class BinNumbers:
+    """fcfb.rs:212-216."""
+    bins: int
+    first: int
+
+
+@dataclass
+class DspParams:
+    """dsp.rs:24-36."""
+    complex: bool
+    fs_in: float
+    fc_in: float = 0.0
+    fft_size: int = 16384
+    scaling: float = 0.0            # 0 = input_format_scaling(format)
+    ffts_per_buf: int = 8
+    spectrum_format: str = "U8"
+    spectrum_averages: int = 2000
+    filters: list = field(default_factory=list)
+    input_format: int = _lib.FMT_S16LE
+    max_blocks: int = 0             # capacity for process_device; 0 = ffts_per_buf
+
+
+@dataclass
+class InputBufferSize:
+    """dsp.rs:38-42."""
+    new: int
+    overlap: int
+    total: int
+
+
+def freq_to_bins(fs_in, fc_in, fft_size, fs_out, fc_out):
+    lib = _lib.load()
+    b, f = ctypes.c_uint32(), ctypes.c_int64()
+    st = lib.spk_freq_to_bins(fs_in, fc_in, fft_size, fs_out, fc_out, ctypes.byref(b), ctypes.byref(f))
+    return BinNumbers(b.value, f.value) if st == _lib.SPK_OK else None
+
+
+def bins_to_freq(fs_in, fc_in, fft_size, bn: BinNumbers):
+    lib = _lib.load()
+    fso, fco = ctypes.c_double(), ctypes.c_double()
+    _lib.check(lib.spk_bins_to_freq(fs_in, fc_in, fft_size, bn.bins, bn.first,
+                                    ctypes.byref(fso), ctypes.byref(fco)))
+    return fso.value, fco.value
+
+
+def freq_to_bins_exact(fs_in, fc_in, fft_size, fs_out, fc_out):
+    lib = _lib.load()
+    b, f = ctypes.c_uint32(), ctypes.c_int64()
+    st = lib.spk_freq_to_bins_exact(fs_in, fc_in, fft_size, fs_out, fc_out, ctypes.byref(b), ctypes.byref(f))
+    return BinNumbers(b.value, f.value) if st == _lib.SPK_OK else None
+
+
+class DspState:
+    """GPU-backed DspState.  Use ``DspState.init(params)``."""
+
+    def __init__(self, ctx, params: DspParams, bufsize: InputBufferSize, bins: int):
+        self._ctx = ctx
+        self.params = params
+        self.bufsize = bufsize
+        self.result_bins = bins
+        self.channels: list[tuple[int, BinNumbers]] = []  # (id, bins)
+
+    # -- construction ------------------------------------------------------
+    @classmethod
+    def init(cls, params: DspParams):
+        """dsp.rs:76-112: returns (state, InputBufferSize)."""
+        lib = _lib.load()
+        cfg = _lib.SpkConfig(params.fft_size, params.input_format, params.scaling,
+                             params.ffts_per_buf, params.spectrum_averages, params.max_blocks)
+        ctx = ctypes.c_void_p()
+        _lib.check(lib.spk_create(ctypes.byref(cfg), ctypes.byref(ctx)))
+        bs = _lib.SpkBufferSize()
+        _lib.check(lib.spk_buffer_size_of(ctx, ctypes.byref(bs)), ctx)
+        bins = params.fft_size if params.complex else params.fft_size // 2 + 1
+        st = cls(ctx, params, InputBufferSize(bs.new_samples, bs.overlap, bs.total), bins)
+        for f in params.filters:
+            st.add_filter(f)
+        return st, st.bufsize
+
+    def add_filter(self, p: FilterParams):
+        """fcfb.rs:41-64: invalid parameters are reported and skipped."""
+        bn = freq_to_bins_exact(self.params.fs_in, self.params.fc_in, self.params.fft_size,
+                                p.fs_out, p.fc_out)
+        if bn is None:
+            print("Invalid filter parameters", file=sys.stderr)
+            return None
+        return self.add_channel(bn.first, bn.bins, p.weights)
+
+    def add_channel(self, first: int, bins: int, weights=None) -> int:
+        """Add a channel by bin numbers; returns its id."""
+        lib = _lib.load()
+        cid = ctypes.c_uint32()
+        wptr = None
+        if weights is not None:
+            w = np.ascontiguousarray(weights, dtype=np.float32)
+            if w.shape != (bins,):
+                raise ValueError("weights must have length bins")
+            wptr = w.ctypes.data_as(ctypes.c_void_p)
+        _lib.check(lib.spk_add_channel(self._ctx, int(first), int(bins), wptr, ctypes.byref(cid)), self._ctx)
+        self.channels.append((cid.value, BinNumbers(bins, first)))
+        return cid.value
+
+    def set_weights(self, cid: int, weights):
+        lib = _lib.load()
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        _lib.check(lib.spk_set_channel_weights(self._ctx, cid, w.ctypes.data_as(ctypes.c_void_p)), self._ctx)
+
+    # -- processing ----------------------------------------------------------
+    def process(self, raw) -> None:
+        """One buffer of F * 3N/4 raw samples from host memory (synchronous)."""
+        buf = np.ascontiguousarray(raw).view(np.uint8)
+        lib = _lib.load()
+        _lib.check(lib.spk_process(self._ctx, buf.ctypes.data_as(ctypes.c_void_p), buf.nbytes), self._ctx)
+
+    def process_device(self, dptr: int, nsamples: int) -> None:
+        """A device-resident stream (e.g. ``tensor.data_ptr()``), asynchronous."""
+        lib = _lib.load()
+        _lib.check(lib.spk_process_device(self._ctx, ctypes.c_void_p(dptr), nsamples), self._ctx)
+
+    def synchronize(self):
+        _lib.check(_lib.load().spk_synchronize(self._ctx), self._ctx)
+
+    def reset(self):
+        _lib.check(_lib.load().spk_reset(self._ctx), self._ctx)
+
+    def set_stream(self, stream_ptr: int | None):
+        _lib.check(_lib.load().spk_set_stream(self._ctx, ctypes.c_void_p(stream_ptr or 0)), self._ctx)
+
+    def enable_timing(self, on=True):
+        _lib.check(_lib.load().spk_enable_timing(self._ctx, 1 if on else 0), self._ctx)
+
+    def last_timing(self):
+        ms = (ctypes.c_float * 3)()
+        _lib.check(_lib.load().spk_last_timing(self._ctx, ms), self._ctx)
+        return {"fwd_welch_ms": ms[0], "channels_ms": ms[1], "finalize_ms": ms[2]}
+
+    # -- results ---------------------------------------------------------------
+    def last_blocks(self) -> int:
+        n = ctypes.c_uint32()
+        _lib.check(_lib.load().spk_last_blocks(self._ctx, ctypes.byref(n)), self._ctx)
+        return n.value
+
+    def channel_output(self, cid: int) -> np.ndarray:
+        """[blocks, 3L/4] complex64 of the last call (fcfb.rs:177-187 record body)."""
+        lib = _lib.load()
+        L = dict(self.channels)[cid].bins
+        nb = self.last_blocks()
+        n = nb * (3 * L // 4) * 2
+        out = np.empty(n, dtype=np.float32)
+        got = ctypes.c_size_t()
+        _lib.check(lib.spk_channel_output(self._ctx, cid, out.ctypes.data_as(ctypes.c_void_p), n,
+                                          ctypes.byref(got)), self._ctx)
+        return out.view(np.complex64).reshape(nb, 3 * L // 4)
+
+    def channel_output_device(self, cid: int):
+        """(device pointer, float count) of the last call's output."""
+        p, n = ctypes.c_void_p(), ctypes.c_size_t()
+        _lib.check(_lib.load().spk_channel_output_device(self._ctx, cid, ctypes.byref(p), ctypes.byref(n)),
+                   self._ctx)
+        return p.value, n.value
+
+    def spectrum_records(self) -> np.ndarray:
+        """Welch records completed by the last call, [R, K] float32 (acc / A)."""
+        lib = _lib.load()
+        p, nrec = ctypes.c_void_p(), ctypes.c_uint32()
+        _lib.check(lib.spk_spectrum_records_device(self._ctx, ctypes.byref(p), ctypes.byref(nrec)), self._ctx)
+        out = np.empty((nrec.value, self.result_bins), dtype=np.float32)
+        got = ctypes.c_uint32()
+        if nrec.value:
+            _lib.check(lib.spk_spectrum_records(self._ctx, out.ctypes.data_as(ctypes.c_void_p), out.size,
+                                                ctypes.byref(got)), self._ctx)
+        return out
+
+    def spectra_device(self):
+        """(device pointer, row stride in complex values) of the last call's spectra."""
+        p, stride = ctypes.c_void_p(), ctypes.c_uint64()
+        _lib.check(_lib.load().spk_spectra_device(self._ctx, ctypes.byref(p), ctypes.byref(stride)), self._ctx)
+        return p.value, stride.value
+
+    def close(self):
+        if self._ctx:
+            _lib.load().spk_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

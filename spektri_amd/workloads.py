@@ -1,0 +1,189 @@
+"""Seeded synthetic workloads for the FCFB + Welch path.
+
+Every random parameter is drawn from ``np.random.default_rng(seed)`` with the
+seed equal to the workload index, and recorded in the returned object.
+
+* ``plain_8k``: real s16, N = 8192 (fs 8.192 MHz, bin spacing 1 kHz),
+  4096 blocks at F = 8, 4 channels of L = 256 at seeded centres in
+  [200, 3900], raised-cosine weights with 20 % transition, Welch over 16
+  blocks; 3 tones at -20 dBFS plus Gaussian noise sigma = 100 LSB.
+* ``rx888_64k``: real s16, 2^26 samples, N = 65536 (fs 131.072 MHz, 2 kHz
+  bins), 64 channels of L = 256 at round4(linspace(1000, 32000, 64)),
+  raised-cosine weights, Welch over 32 blocks; 40 tones at -60..-10 dBFS, a
+  linear sweep 0 -> fs/2 at -30 dBFS, noise sigma = 30 LSB.
+
+Signals are rounded and clipped to int16.  The deterministic part (tones,
+sweep) is computed in float64 with exact phase reduction; the noise comes from
+numpy on the host or from a seeded torch generator on a GPU, so host- and
+GPU-generated streams differ in their noise samples (tests always compare two
+paths on the same stream).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+FULL_SCALE = 32767.0
+
+
+def round4(v):
+    return (np.round(np.asarray(v, dtype=np.float64) / 4.0) * 4).astype(np.int64)
+
+
+def raised_cosine_transition(L: int, alpha: float = 0.2) -> np.ndarray:
+    """Flat-top weights over L bins (j order, j = 0 is the channel's lowest
+    bin) with raised-cosine edges: 1 for |u| <= 1 - alpha, else
+    0.5 (1 + cos(pi (|u| - 1 + alpha) / alpha)), u = (j - L/2) / (L/2)."""
+    j = np.arange(L, dtype=np.float64)
+    u = np.abs(j - L / 2) / (L / 2)
+    w = np.ones(L)
+    edge = u > 1 - alpha
+    w[edge] = 0.5 * (1 + np.cos(np.pi * (u[edge] - 1 + alpha) / alpha))
+    return w.astype(np.float32)
+
+
+def kaiser_weights(L: int, beta: float) -> np.ndarray:
+    """Kaiser window over the L bins of a channel (j order)."""
+    return np.kaiser(L, beta).astype(np.float32)
+
+
+@dataclass
+class Tone:
+    freq: float   # Hz
+    amp: float    # LSB
+    phase: float  # rad
+
+
+@dataclass
+class Workload:
+    name: str
+    seed: int
+    fft_size: int
+    fs: float
+    ffts_per_buf: int
+    averages: int
+    nsamples: int
+    channels: list = field(default_factory=list)   # (first_bin, L, weights f32[L])
+    tones: list = field(default_factory=list)
+    sweep_amp: float = 0.0
+    noise_sigma: float = 0.0
+    complex: bool = False
+
+    @property
+    def hop(self):
+        return self.fft_size - self.fft_size // 4
+
+    @property
+    def nblocks(self):
+        per = self.hop * self.ffts_per_buf
+        return (self.nsamples // per) * self.ffts_per_buf
+
+    @property
+    def bins(self):
+        return self.fft_size if self.complex else self.fft_size // 2 + 1
+
+    def bytes_per_block(self) -> float:
+        """Algorithmic HBM bytes per block: new input once, spectrum written
+        once, channel bins gathered, channel outputs written, Welch output."""
+        s_in = 8 if self.complex else 2
+        sum_l = sum(L for _, L, _ in self.channels)
+        return (self.hop * s_in + self.bins * 8 + sum_l * 8 + sum(3 * L // 4 for _, L, _ in self.channels) * 8
+                + self.bins * 4 / self.averages)
+
+    def describe(self) -> dict:
+        return {"workload": self.name, "fft_size": self.fft_size, "hop": self.hop,
+                "nsamples": self.nsamples, "blocks": self.nblocks, "ffts_per_buf": self.ffts_per_buf,
+                "channels": len(self.channels), "ifft_sizes": sorted({L for _, L, _ in self.channels}),
+                "welch_averages": self.averages, "input": "cf32le" if self.complex else "s16le"}
+
+
+def plain_8k(nsamples: int | None = None) -> Workload:
+    seed = 0
+    rng = np.random.default_rng(seed)
+    N, fs = 8192, 8.192e6
+    wl = Workload("plain_8k", seed, N, fs, 8, 16, nsamples or 4096 * 6144)
+    centres = round4(rng.uniform(200, 3900, size=4))
+    wts = raised_cosine_transition(256, 0.2)
+    wl.channels = [(int(c) - 128, 256, wts) for c in centres]
+    amp = FULL_SCALE * 10 ** (-20 / 20)
+    for f in rng.uniform(0.01, 0.49, size=3) * fs:
+        wl.tones.append(Tone(float(f), amp, float(rng.uniform(0, 2 * np.pi))))
+    wl.noise_sigma = 100.0
+    return wl
+
+
+def rx888_64k(nsamples: int | None = None) -> Workload:
+    seed = 1
+    rng = np.random.default_rng(seed)
+    N, fs = 65536, 131.072e6
+    wl = Workload("rx888_64k", seed, N, fs, 8, 32, nsamples or 2 ** 26)
+    centres = round4(np.linspace(1000, 32000, 64))
+    wts = raised_cosine_transition(256, 0.2)
+    wl.channels = [(int(c) - 128, 256, wts) for c in centres]
+    dbs = rng.uniform(-60, -10, size=40)
+    freqs = rng.uniform(0.0, 0.5, size=40) * fs
+    phases = rng.uniform(0, 2 * np.pi, size=40)
+    wl.tones = [Tone(float(f), FULL_SCALE * 10 ** (d / 20), float(p)) for f, d, p in zip(freqs, dbs, phases)]
+    wl.sweep_amp = FULL_SCALE * 10 ** (-30 / 20)
+    wl.noise_sigma = 30.0
+    return wl
+
+
+WORKLOADS = {"plain_8k": plain_8k, "rx888_64k": rx888_64k}
+
+
+def _deterministic_part_np(wl: Workload, n0: int, n1: int) -> np.ndarray:
+    n = np.arange(n0, n1, dtype=np.int64)
+    x = np.zeros(n1 - n0, dtype=np.float64)
+    for tn in wl.tones:
+        cyc = tn.freq / wl.fs
+        ph = np.mod(cyc * n.astype(np.float64), 1.0)
+        x += tn.amp * np.cos(2 * np.pi * ph + tn.phase)
+    if wl.sweep_amp:
+        # instantaneous frequency f(n) = (fs/2) n / S: phase = pi n^2 / (2 S),
+        # reduced exactly as (n^2 mod 4S) / (4S) turns.
+        S = wl.nsamples
+        frac = np.mod(n * n, 4 * S).astype(np.float64) / (4 * S)
+        x += wl.sweep_amp * np.cos(2 * np.pi * frac)
+    return x
+
+
+def make_signal_np(wl: Workload, nsamples: int | None = None) -> np.ndarray:
+    """int16 stream generated on the host (numpy), chunked."""
+    S = nsamples or wl.nsamples
+    rng = np.random.default_rng(1000 + wl.seed)
+    out = np.empty(S, dtype=np.int16)
+    step = 1 << 22
+    for a in range(0, S, step):
+        b = min(S, a + step)
+        x = _deterministic_part_np(wl, a, b)
+        if wl.noise_sigma:
+            x += rng.standard_normal(b - a) * wl.noise_sigma
+        out[a:b] = np.clip(np.round(x), -32768, 32767).astype(np.int16)
+    return out
+
+
+def make_signal_torch(wl: Workload, device, nsamples: int | None = None):
+    """int16 stream generated on a torch device (float64 arithmetic), chunked."""
+    import torch
+    S = nsamples or wl.nsamples
+    gen = torch.Generator(device=device)
+    gen.manual_seed(1000 + wl.seed)
+    out = torch.empty(S, dtype=torch.int16, device=device)
+    step = 1 << 24
+    for a in range(0, S, step):
+        b = min(S, a + step)
+        n = torch.arange(a, b, dtype=torch.int64, device=device)
+        nf = n.to(torch.float64)
+        x = torch.zeros(b - a, dtype=torch.float64, device=device)
+        for tn in wl.tones:
+            ph = torch.remainder(nf * (tn.freq / wl.fs), 1.0)
+            x += tn.amp * torch.cos(2 * np.pi * ph + tn.phase)
+        if wl.sweep_amp:
+            frac = torch.remainder(n * n, 4 * wl.nsamples).to(torch.float64) / (4 * wl.nsamples)
+            x += wl.sweep_amp * torch.cos(2 * np.pi * frac)
+        if wl.noise_sigma:
+            x += torch.randn(b - a, dtype=torch.float64, device=device, generator=gen) * wl.noise_sigma
+        out[a:b] = torch.clamp(torch.round(x), -32768, 32767).to(torch.int16)
+    return out

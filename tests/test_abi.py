@@ -1,0 +1,70 @@
+"""CPU checks of the C ABI: the library loads, exports every symbol the
+header declares, and its host-only helpers match the oracle."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import spektri_oracle as O
+from spektri_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "spektri_gpu.h")
+
+
+def header_symbols():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(spk_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    syms = header_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert sorted(_lib.declared_symbols()) == syms
+
+
+def test_null_and_invalid_arguments_return_codes():
+    lib = _lib.load()
+    assert lib.spk_destroy(None) == 1
+    assert lib.spk_last_error(None) == b"null context"
+    out = ctypes.c_void_p()
+    cfg = _lib.SpkConfig(1000, _lib.FMT_S16LE, 0.0, 8, 16, 0)  # N not a power of two
+    assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 1
+    cfg = _lib.SpkConfig(8192, _lib.FMT_CF32LE, 0.0, 8, 16, 0)
+    assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 2
+
+
+@pytest.mark.parametrize("fs,N", [(131.072e6, 65536), (128e6, 16384), (8.192e6, 8192)])
+def test_freq_to_bins_matches_oracle(fs, N):
+    lib = _lib.load()
+    d = fs / N
+    rng = np.random.default_rng(7)
+    for _ in range(300):
+        fso = float(rng.integers(1, 400)) * d * rng.choice([1.0, 1.0003])
+        fco = float(rng.integers(-3000, 3000)) * d * rng.choice([1.0, 0.9997])
+        b, f = ctypes.c_uint32(), ctypes.c_int64()
+        ref = O.freq_to_bins(fs, 0.0, N, fso, fco)
+        st = lib.spk_freq_to_bins(fs, 0.0, N, fso, fco, ctypes.byref(b), ctypes.byref(f))
+        assert (st == 0) == (ref is not None)
+        if ref is not None:
+            assert (b.value, f.value) == ref
+        ref = O.freq_to_bins_exact(fs, 0.0, N, fso, fco)
+        st = lib.spk_freq_to_bins_exact(fs, 0.0, N, fso, fco, ctypes.byref(b), ctypes.byref(f))
+        assert (st == 0) == (ref is not None)
+        if ref is not None:
+            assert (b.value, f.value) == ref
+
+
+def test_freq_to_bins_reference_known_answer():
+    lib = _lib.load()
+    b, f = ctypes.c_uint32(), ctypes.c_int64()
+    assert lib.spk_freq_to_bins(128.0e6, 0.0, 16384, 500000.0, 50.250e6, ctypes.byref(b), ctypes.byref(f)) == 0
+    assert (b.value, f.value) == (64, 6400)
+    fso, fco = ctypes.c_double(), ctypes.c_double()
+    assert lib.spk_bins_to_freq(128.0e6, 0.0, 16384, 64, 6400, ctypes.byref(fso), ctypes.byref(fco)) == 0
+    assert (fso.value, fco.value) == (500000.0, 50.250e6)

@@ -1,0 +1,198 @@
+"""GPU parity of the HIP path against the FP64 oracle (``-m gpu``).
+
+Bar: channel outputs, forward spectra and Welch records within 1e-5 RMS of
+full scale (s16 full scale maps to 1.0 through the reference's 1/32767
+scaling, so a full-scale tone at a channel centre gives |y| = 0.5).  Every
+case calls the library through its C ABI (ctypes) and compares with
+``oracle.spektri_oracle`` on the same int16 stream.
+"""
+import numpy as np
+import pytest
+
+from oracle import spektri_oracle as O
+from spektri_amd import workloads as W
+from spektri_amd.dsp import DspParams, DspState
+
+from .gpu_util import d2h, report, rms
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def geom_of(wl_or_n, F, complex_in=False):
+    return O.Geometry(wl_or_n, complex_in, F)
+
+
+def run_gpu(x_i16: np.ndarray, N, F, A, channels, max_blocks=0, use_device=True):
+    """Run the GPU path on one stream; returns (state, blocks)."""
+    st, bs = DspState.init(DspParams(complex=False, fs_in=1.0, fft_size=N, ffts_per_buf=F,
+                                     spectrum_averages=A, max_blocks=max_blocks))
+    ids = [st.add_channel(first, L, w) for first, L, w in channels]
+    if use_device:
+        dev = torch.from_numpy(x_i16).to("cuda")
+        st.process_device(dev.data_ptr(), len(x_i16))
+        st.synchronize()
+        del dev
+    return st, ids
+
+
+def oracle_path(x_i16, N, F, A, channels, blocks=None):
+    geom = O.Geometry(N, False, F)
+    s = O.input_format_scaling(O.FMT_S16LE)
+    nb = geom.blocks_for(len(x_i16)) if blocks is None else blocks
+    bl = O.frame_blocks(x_i16, geom, nb)
+    X = O.forward_spectra(bl, geom, s)
+    acc = O.WelchAccumulator(geom.bins, N, A)
+    recs = acc.accumulate(X)
+    outs = [O.channel_process(X, N, first, np.asarray(w, np.float64)) for first, L, w in channels]
+    return X, np.array(recs) if recs else np.zeros((0, geom.bins)), outs
+
+
+def gpu_spectra(st, nb, K):
+    p, stride = st.spectra_device()
+    a = d2h(p, nb * stride * 8).view(np.complex64).reshape(nb, stride)
+    return a[:, :K]
+
+
+def edge_channels(N, K, rng):
+    """Channels that exercise get_bin's modulo and conjugate branches."""
+    out = []
+    for first, L in [(-100, 256), (K - 60, 128), (-N - 40, 64), (N - 20, 32), (3 * N + 4, 16), (K - 4, 8)]:
+        out.append((first, L, rng.random(L).astype(np.float32)))
+    return out
+
+
+def compare(tag, st, ids, channels, X, recs, outs, nb, K, extra=None):
+    Xg = gpu_spectra(st, nb, K)
+    e_spec = rms(Xg, X)
+    recs_g = st.spectrum_records()
+    assert recs_g.shape == recs.shape, (recs_g.shape, recs.shape)
+    e_welch = rms(recs_g, recs) if recs.size else 0.0
+    e_ch = []
+    for cid, (first, L, w), yo in zip(ids, channels, outs):
+        yg = st.channel_output(cid)
+        assert yg.shape == yo.shape, (L, yg.shape, yo.shape)
+        e_ch.append(rms(yg, yo))
+    vals = dict(blocks=nb, spectra_rms=e_spec, welch_rms=e_welch, channel_rms_max=max(e_ch),
+                spectra_max_abs=float(np.max(np.abs(Xg - X))))
+    if extra:
+        vals.update(extra)
+    report(tag, **vals)
+    assert e_spec < TOL and e_welch < TOL and max(e_ch) < TOL, vals
+
+
+def test_plain_8k_full_size():
+    wl = W.plain_8k()
+    x = W.make_signal_np(wl)
+    rng = np.random.default_rng(11)
+    K = wl.bins
+    chans = list(wl.channels) + edge_channels(wl.fft_size, K, rng)
+    st, ids = run_gpu(x, wl.fft_size, wl.ffts_per_buf, wl.averages, chans, max_blocks=wl.nblocks)
+    nb = st.last_blocks()
+    assert nb == wl.nblocks == 4096
+    X, recs, outs = oracle_path(x, wl.fft_size, wl.ffts_per_buf, wl.averages, chans)
+    compare("plain_8k", st, ids, chans, X, recs, outs, nb, K)
+    st.close()
+
+
+@pytest.mark.parametrize("N", [4096, 8192, 16384, 32768, 65536])
+def test_every_fft_size_and_ifft_size(N):
+    rng = np.random.default_rng(N)
+    F, A = 4, 5
+    H = 3 * N // 4
+    nsamp = 3 * F * H + 123  # 3 buffers and a dropped partial buffer
+    x = np.clip(np.round(rng.standard_normal(nsamp) * 3000), -32768, 32767).astype(np.int16)
+    K = N // 2 + 1
+    chans = edge_channels(N, K, rng)
+    for L in [8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192]:
+        if L <= N:
+            first = int(rng.integers(-L, K))
+            chans.append((first, L, rng.random(L).astype(np.float32)))
+    st, ids = run_gpu(x, N, F, A, chans, max_blocks=3 * F)
+    nb = st.last_blocks()
+    assert nb == 3 * F
+    X, recs, outs = oracle_path(x, N, F, A, chans)
+    compare(f"random_N{N}", st, ids, chans, X, recs, outs, nb, K)
+    st.close()
+
+
+def test_streaming_host_buffers_carry_history_and_welch():
+    wl = W.plain_8k()
+    F, A, N = wl.ffts_per_buf, 12, wl.fft_size  # A does not divide F: records straddle calls
+    per = F * wl.hop
+    nbuf = 5
+    x = W.make_signal_np(wl, per * nbuf)
+    st, bs = DspState.init(DspParams(complex=False, fs_in=wl.fs, fft_size=N, ffts_per_buf=F,
+                                     spectrum_averages=A))
+    assert (bs.new, bs.overlap, bs.total) == (per, N // 4, per + N // 4)
+    ids = [st.add_channel(first, L, w) for first, L, w in wl.channels]
+    X, recs, outs = oracle_path(x, N, F, A, wl.channels)
+    got_recs, errs = [], []
+    for b in range(nbuf):
+        st.process(x[b * per:(b + 1) * per])
+        for cid, yo in zip(ids, outs):
+            errs.append(rms(st.channel_output(cid), yo[b * F:(b + 1) * F]))
+        r = st.spectrum_records()
+        got_recs.extend(list(r))
+    got_recs = np.array(got_recs)
+    assert got_recs.shape == recs.shape
+    e_w = rms(got_recs, recs)
+    report("plain_8k_streaming", buffers=nbuf, channel_rms_max=max(errs), welch_rms=e_w)
+    assert max(errs) < TOL and e_w < TOL
+    st.close()
+
+
+def test_reset_restores_fresh_state():
+    rng = np.random.default_rng(3)
+    N, F, A = 8192, 2, 3
+    x = np.clip(np.round(rng.standard_normal(F * 6144 * 2) * 5000), -32768, 32767).astype(np.int16)
+    chans = [(100, 64, None)]
+    st, ids = run_gpu(x, N, F, A, chans, max_blocks=4)
+    y1 = st.channel_output(ids[0])
+    dev = torch.from_numpy(x).to("cuda")
+    st.process_device(dev.data_ptr(), len(x))  # history now non-zero
+    y2 = st.channel_output(ids[0])
+    st.reset()
+    st.process_device(dev.data_ptr(), len(x))
+    y3 = st.channel_output(ids[0])
+    assert np.array_equal(y1, y3)
+    assert not np.array_equal(y1[0], y2[0])  # block 0 saw the carried history
+    np.testing.assert_array_equal(y1[1:], y2[1:])
+    st.close()
+
+
+def test_rx888_full_size_sampled_blocks():
+    wl = W.rx888_64k()
+    dev = W.make_signal_torch(wl, "cuda")
+    x = dev.cpu().numpy()
+    N, F, A = wl.fft_size, wl.ffts_per_buf, wl.averages
+    st, bs = DspState.init(DspParams(complex=False, fs_in=wl.fs, fft_size=N, ffts_per_buf=F,
+                                     spectrum_averages=A, max_blocks=wl.nblocks))
+    ids = [st.add_channel(first, L, w) for first, L, w in wl.channels]
+    st.process_device(dev.data_ptr(), wl.nsamples)
+    st.synchronize()
+    nb = st.last_blocks()
+    assert nb == wl.nblocks == 1360
+    K = wl.bins
+    geom = O.Geometry(N, False, F)
+    s = O.input_format_scaling(O.FMT_S16LE)
+    Xg_all = gpu_spectra(st, nb, K)
+    recs_g = st.spectrum_records()
+    assert recs_g.shape == (nb // A, K)
+    errs = {}
+    outs_g = [st.channel_output(cid) for cid in ids]
+    for g0 in (0, 1312):  # first and last complete Welch group
+        blocks = O.frame_blocks(x, geom, A, first=g0)
+        X = O.forward_spectra(blocks, geom, s)
+        errs[f"spectra_g{g0}"] = rms(Xg_all[g0:g0 + A], X)
+        errs[f"welch_rec{g0 // A}"] = rms(recs_g[g0 // A], O.welch_terms(X, N).mean(0))
+        e = 0.0
+        for (first, L, w), yg in zip(wl.channels, outs_g):
+            yo = O.channel_process(X, N, first, np.asarray(w, np.float64))
+            e = max(e, rms(yg[g0:g0 + A], yo))
+        errs[f"channels_g{g0}"] = e
+    report("rx888_64k", blocks=nb, **errs)
+    assert max(errs.values()) < TOL, errs
+    st.close()
