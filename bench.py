@@ -81,8 +81,11 @@ def main():
                                     max_blocks=wl.nblocks))
     for first, L, w in wl.channels:
         st.add_channel(first, L, w)
-    stream = torch.cuda.current_stream(dev)
+    # A dedicated (non-null, default-priority) stream shared by the library
+    # launches and the torch events that time them.
+    stream = torch.cuda.Stream(dev)
     st.set_stream(stream.cuda_stream)
+    torch.cuda.set_stream(stream)
 
     ptr, n = x.data_ptr(), wl.nsamples
     for _ in range(args.warmup):

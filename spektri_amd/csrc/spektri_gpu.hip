@@ -15,7 +15,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/spektri_gpu.h"
+#include "spektri_gpu.h"
 #include "kernels.h"
 
 using spk::cpx;
