@@ -50,7 +50,13 @@ typedef struct spk_config {
   uint32_t ffts_per_buf;      /* F: blocks per spk_process call */
   uint32_t spectrum_averages; /* A: blocks per Welch record */
   uint32_t max_blocks;        /* capacity for spk_process_device; 0 = F */
+  uint32_t flags;             /* SPK_FLAG_* */
 } spk_config;
+
+/* Keep every block's forward spectrum in device memory (spk_spectra_device).
+ * Without it the spectrum is only materialised when a channel kernel needs
+ * it; channels of L = 256 are then computed inside the forward kernel. */
+#define SPK_FLAG_KEEP_SPECTRA 1u
 
 /* InputBufferSize (spektri/src/dsp.rs:38-42), in samples. */
 typedef struct spk_buffer_size {
@@ -108,7 +114,8 @@ spk_status spk_spectrum_records(spk_ctx* ctx, float* out, size_t cap, uint32_t* 
 spk_status spk_spectrum_records_device(spk_ctx* ctx, const float** dptr, uint32_t* nrec);
 
 /* Forward spectra of the last call: nblocks rows of K complex f32 with a
- * row stride of *stride complex values (the reference's fft_result_buf). */
+ * row stride of *stride complex values (the reference's fft_result_buf).
+ * Valid only for contexts created with SPK_FLAG_KEEP_SPECTRA. */
 spk_status spk_spectra_device(spk_ctx* ctx, const float** dptr, uint64_t* stride);
 
 /* Clear history, Welch accumulators and counters (as a fresh DspState). */

@@ -16,12 +16,14 @@ STATUS_NAMES = {0: "SPK_OK", 1: "SPK_ERR_INVALID", 2: "SPK_ERR_UNSUPPORTED", 3: 
                 4: "SPK_ERR_STATE", 5: "SPK_ERR_CAPACITY"}
 FMT_S16LE = 0x08
 FMT_CF32LE = 0x5C
+FLAG_KEEP_SPECTRA = 1
 
 
 class SpkConfig(ctypes.Structure):
     _fields_ = [("fft_size", ctypes.c_uint32), ("input_format", ctypes.c_uint32),
                 ("scaling", ctypes.c_float), ("ffts_per_buf", ctypes.c_uint32),
-                ("spectrum_averages", ctypes.c_uint32), ("max_blocks", ctypes.c_uint32)]
+                ("spectrum_averages", ctypes.c_uint32), ("max_blocks", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32)]
 
 
 class SpkBufferSize(ctypes.Structure):

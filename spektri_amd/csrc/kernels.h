@@ -25,25 +25,55 @@ constexpr int kLdsBudgetBytes = 150 * 1024;
 // v_permlane32_swap per float gives every lane its mirror values without
 // an LDS round trip.
 // ---------------------------------------------------------------------------
+struct ChanDesc;
+
 struct FwdParams {
   const int16_t* src;   // stream samples; sample s >= 0 at src[s]
   const int16_t* head;  // block 0's N samples: O samples of history, then src[0 .. N-O)
-  cpx* spec;            // [nblocks][spec_stride]
+  cpx* spec;            // spectrum rows: per block when keep_spec, else one scratch row per workgroup
   long long spec_stride;
+  int keep_spec;
   const cpx* tw2;       // pass-2 table, forward sign
   const cpx* tw3;       // pass-3 table, forward sign
   const cpx* twu;       // exp(-2 pi i k / N), k = 0..M/R
-  float* welch_acc;     // [ring][welch_stride]
-  long long welch_stride;
-  int welch_ring;
-  int welch_A;
+  float* wpart;         // Welch partial sums [workgroup][slot][wstride]
+  long long wstride;
+  int wslots;
+  int A;
   long long block0;     // stream index of this launch's block 0
   float wscale;
   int nblocks;
   int H, O;
+  // fused channel stage (FL > 0): every channel has IFFT size FL
+  const ChanDesc* fchans;
+  int fnch;
+  const float* weights;
+  cpx* fout;
+  const cpx* ftw2;      // pass-2 table of the FL-point inverse transform
+  int N, K;
 };
 
-template <int M, int R, int R1, int R2>
+struct ChanDesc {
+  long long out_off;  // cpx offset of the channel's output (block 0)
+  int f_mod;          // first bin mod N, in [0, N)
+  int w_off;          // float offset of the channel's weights
+};
+
+// Fused channel tiles: FL points as 16 x (FL/16), 16 points per thread.
+template <int FL>
+struct FusedCfg {
+  static constexpr int RF = 16;
+  static constexpr int TF = FL / RF;
+  static constexpr int RF1 = 16;
+  static constexpr int RF2 = FL / 16;
+  static constexpr int STRIDE = FL + (FL >> 5);  // cpx per tile region (>= lds_pad(FL - 1) + 1)
+};
+template <>
+struct FusedCfg<0> {
+  static constexpr int RF = 1, TF = 1, RF1 = 1, RF2 = 1, STRIDE = 0;
+};
+
+template <int M, int R, int R1, int R2, int FL>
 struct FwdRealCfg {
   static constexpr int T = M / R;
   static constexpr int P3 = R1 * R2;
@@ -51,7 +81,9 @@ struct FwdRealCfg {
   static_assert(P3 == T, "last pass must have one butterfly per thread");
   static_assert(T >= 64 && T <= 1024, "one wave minimum, 1024 threads maximum");
   static constexpr bool SPLIT = lds_padded_size(M + 1) * 8 > kLdsBudgetBytes;
-  static constexpr int LDS_FLOATS = SPLIT ? lds_padded_size(M + 1) : 2 * lds_padded_size(M + 1);
+  static constexpr int PLANE_FLOATS = SPLIT ? lds_padded_size(M + 1) : 2 * lds_padded_size(M + 1);
+  static constexpr int TILE_FLOATS = FL > 0 ? (T / FusedCfg<FL>::TF) * FusedCfg<FL>::STRIDE * 2 : 0;
+  static constexpr int LDS_FLOATS = PLANE_FLOATS > TILE_FLOATS ? PLANE_FLOATS : TILE_FLOATS;
 };
 
 __device__ __forceinline__ float swap_hi_lo(float v, float& other) {
@@ -63,177 +95,222 @@ __device__ __forceinline__ float swap_hi_lo(float v, float& other) {
   return __uint_as_float(r[0]);
 }
 
-template <int M, int R, int R1, int R2>
+// DPP wave shifts (gfx9 wave_shr:1 / wave_shl:1): the value of lane l-1
+// (resp. l+1); lane 0 (resp. 63) keeps its own value.
+__device__ __forceinline__ float wave_shr1(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float wave_shl1(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+
+// Persistent workgroups: workgroup w takes blocks [w nb / G, (w+1) nb / G)
+// in order, so its Welch sums stay in a private partial row (slot = Welch
+// group - first group of the workgroup) that is written with plain stores;
+// welch_finalize_kernel adds the rows of every group.
+template <int M, int R, int R1, int R2, int FL>
 __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
-  using C = FwdRealCfg<M, R, R1, R2>;
+  using C = FwdRealCfg<M, R, R1, R2, FL>;
+  using FC = FusedCfg<FL>;
   constexpr int T = C::T;
   constexpr int P3 = C::P3;
   constexpr bool SPLIT = C::SPLIT;
   __shared__ float lds[C::LDS_FLOATS];
 
-  const int t = threadIdx.x;
-  const int g = blockIdx.x;
+  const int G = gridDim.x, wg = blockIdx.x;
+  const int lo = (int)((long long)wg * P.nblocks / G);
+  const int hi = (int)((long long)(wg + 1) * P.nblocks / G);
+  const long long grp0 = (P.block0 + lo) / P.A;
   const float w = P.wscale;
-  // Block g starts at stream sample g*H - O; block 0 reads the head copy.
-  const int* blk = reinterpret_cast<const int*>(
-      g == 0 ? P.head : P.src + ((long long)g * P.H - P.O));
 
-  cpx x[R];
-  // Pass 1 loads: point n = b + j M/R1 is the s16 pair at block sample 2n.
-  // Rows are addressed as (uniform row pointer) + lane offset so that the
-  // lane offsets are shared instead of kept per row.
+  for (int g = lo; g < hi; ++g) {
+    // Per-block values that do not depend on g (lane maps, table reads) are
+    // recomputed and reloaded every block: kept live across the loop they
+    // would cost more registers than the 1024-thread workgroups have.
+    int t = threadIdx.x;
+    const cpx* tw2 = P.tw2;
+    const cpx* tw3 = P.tw3;
+    const cpx* twu = P.twu;
+    asm volatile("" : "+v"(t));
+    asm volatile("" : "+s"(tw2), "+s"(tw3), "+s"(twu));
+
+    // Lane map of the last pass: lane l < 32 of wave v takes sub-transform
+    // k = pi = 32 v + l; lane l + 32 takes its mirror P3 - pi (P3/2 for pi = 0).
+    const int wave = t >> 6, lane = t & 63;
+    const bool upper = lane >= 32;
+    const int pi = 32 * wave + (lane & 31);
+    const int ksrc = upper ? (pi == 0 ? P3 / 2 : P3 - pi) : pi;
+    const int kt = (upper && pi == 0) ? P3 / 2 : pi;
+    const float sg = upper ? -1.f : 1.f;
+
+    // Block g starts at stream sample g*H - O; block 0 reads the head copy.
+    const int* blk = reinterpret_cast<const int*>(g == 0 ? P.head : P.src + ((long long)g * P.H - P.O));
+
+    cpx x[R];
+    // Pass 1 loads: point n = b + j M/R1 is the s16 pair at block sample 2n.
+    // Rows are addressed as (uniform row pointer) + lane offset so that the
+    // lane offsets are shared instead of kept per row.
 #pragma unroll
-  for (int e = 0; e < R; ++e) {
-    const int u = e / R1, j = e % R1;
-    const int* row = blk + (T * u + j * (M / R1));
-    const int v = row[t];
-    x[e].x = (float)(int16_t)v * w;
-    x[e].y = (float)(v >> 16) * w;
-  }
-  st_butterflies<M, R, T, R1, 1, -1>(x, t, nullptr);
-
-  lds_exchange<R, SPLIT>(
-      x, lds, [&](int e) { return st_out_addr<M, R, T, R1, 1>(t, e); },
-      [&](int e) { return st_in_addr<M, R, T, R2>(t, e); });
-  st_butterflies<M, R, T, R2, R1, -1>(x, t, P.tw2);
-
-  // Pass 3 with the mirror-pair lane map.  Lane l < 32 of wave w takes
-  // sub-transform k = pi = 32 w + l; lane l + 32 takes its mirror P3 - pi
-  // (P3/2 for pi = 0).  An upper lane must end with Z[ksrc + P3 (R-1-q)] in
-  // slot q, i.e. the DFT of its twiddled inputs in reversed order.  That is
-  // conj(DFT(conj(in_m) w^{m kt})) with kt = P3 - ksrc, so upper lanes read
-  // in natural order, conjugate, use the same table rows as lower lanes
-  // (column kt), and conjugate the DFT outputs.
-  const int wave = t >> 6, lane = t & 63;
-  const bool upper = lane >= 32;
-  const int pi = 32 * wave + (lane & 31);
-  const int ksrc = upper ? (pi == 0 ? P3 / 2 : P3 - pi) : pi;
-  const int kt = (upper && pi == 0) ? P3 / 2 : pi;
-  const float sg = upper ? -1.f : 1.f;
-  lds_exchange<R, SPLIT>(
-      x, lds, [&](int e) { return st_out_addr<M, R, T, R2, R1>(t, e); },
-      [&](int e) { return lds_pad(ksrc) + pad_off(e * P3); });
-#pragma unroll
-  for (int e = 0; e < R; ++e) x[e].y *= sg;
-  apply_pass_twiddles<R, P3>(x, P.tw3, kt);
-  dft<R, -1>(x);
-#pragma unroll
-  for (int e = 0; e < R; ++e) x[e].y *= sg;
-  // Lower lanes now hold Z[pi + P3 q] in slot q, upper lanes
-  // Z[ksrc + P3 (R-1-q)], so slot q of lane l and lane l^32 are mirrors
-  // (except the pair pi = 0, whose lanes are their own mirrors).
-  __syncthreads();  // every lane is done reading pass-3 inputs from LDS
-
-  const cpx wb = P.twu[upper ? ksrc : pi];
-  const float h = 0.5f * sg;
-  cpx* spec = P.spec + (long long)g * P.spec_stride;
-  // Own bin of slot q: lower pi + P3 q, upper ksrc + P3 (R-1-q).
-  const int kb = upper ? ksrc + P3 * (R - 1) : pi;
-  const int kstep = upper ? -P3 : P3;
-  const int ab = lds_pad(upper ? ksrc : pi) + (upper ? pad_off(P3 * (R - 1)) : 0);
-  const int astep = upper ? -pad_off(P3) : pad_off(P3);
-  static_for<0, R>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    cpx A, B;
-    A.x = swap_hi_lo(x[q].x, B.x);
-    A.y = swap_hi_lo(x[q].y, B.y);
-    if (wave == 0) {
-      // lane 0 (k = 0): mirror of Z[P3 q] is its own slot (R - q) mod R;
-      // lane 32 (k = P3/2): mirror of its slot q is its own slot R-1-q.
-      const cpx m0 = x[(R - q) & (R - 1)];
-      const cpx m32 = x[R - 1 - q];
-      if (lane == 0) B = m0;
-      if (lane == 32) A = m32;
+    for (int e = 0; e < R; ++e) {
+      const int u = e / R1, j = e % R1;
+      const int* row = blk + (T * u + j * (M / R1));
+      const int v = row[t];
+      x[e].x = (float)(int16_t)v * w;
+      x[e].y = (float)(v >> 16) * w;
     }
-    const cpx E = {0.5f * (A.x + B.x), h * (A.y - B.y)};
-    const cpx Od = {0.5f * (A.y + B.y), -h * (A.x - B.x)};
-    constexpr float cqx = float(ct_cos2pi(-q, 2 * R));
-    constexpr float cqy = float(ct_sin2pi(-q, 2 * R));
-    constexpr float crx = float(ct_cos2pi(-(R - 1 - q), 2 * R));
-    constexpr float cry = float(ct_sin2pi(-(R - 1 - q), 2 * R));
-    const cpx c = upper ? cpx{crx, cry} : cpx{cqx, cqy};
-    const cpx wk = cmul(wb, c);
-    const cpx X = cadd(E, cmul(wk, Od));
-    spec[kb + q * kstep] = X;
-    const int a = ab + q * astep;
-    if constexpr (SPLIT) {
-      lds[a] = X.x;
-    } else {
-      reinterpret_cast<cpx*>(lds)[a] = X;
-    }
-    if constexpr (q == 0) {
-      if (t == 0) {  // Nyquist bin M = E - O from Z[0]
-        const cpx XN = csub(E, Od);
-        spec[M] = XN;
-        if constexpr (SPLIT) lds[lds_pad(M)] = XN.x;
-        else reinterpret_cast<cpx*>(lds)[lds_pad(M)] = XN;
+    st_butterflies<M, R, T, R1, 1, -1>(x, t, nullptr);
+
+    lds_exchange<R, SPLIT>(
+        x, lds, [&](int e) { return st_out_addr<M, R, T, R1, 1>(t, e); },
+        [&](int e) { return st_in_addr<M, R, T, R2>(t, e); });
+    st_butterflies<M, R, T, R2, R1, -1>(x, t, tw2);
+
+    // Pass 3: an upper lane must end with Z[ksrc + P3 (R-1-q)] in slot q,
+    // i.e. the DFT of its twiddled inputs in reversed order.  That is
+    // conj(DFT(conj(in_m) w^{m kt})) with kt = P3 - ksrc, so upper lanes read
+    // in natural order, conjugate, use the same table rows as lower lanes
+    // (column kt), and conjugate the DFT outputs.
+    lds_exchange<R, SPLIT>(
+        x, lds, [&](int e) { return st_out_addr<M, R, T, R2, R1>(t, e); },
+        [&](int e) { return lds_pad(ksrc) + pad_off(e * P3); });
+#pragma unroll
+    for (int e = 0; e < R; ++e) x[e].y *= sg;
+    apply_pass_twiddles<R, P3>(x, tw3, kt);
+    dft<R, -1>(x);
+#pragma unroll
+    for (int e = 0; e < R; ++e) x[e].y *= sg;
+    // Lower lanes now hold Z[pi + P3 q] in slot q, upper lanes
+    // Z[ksrc + P3 (R-1-q)], so slot q of lane l and lane l^32 are mirrors
+    // (except the pair pi = 0, whose lanes are their own mirrors).
+    __syncthreads();  // every lane is done reading pass-3 inputs from LDS
+
+    const cpx wb = twu[upper ? ksrc : pi];
+    const float h = 0.5f * sg;
+    cpx* spec = P.spec + (P.keep_spec ? (long long)g : (long long)wg) * P.spec_stride;
+    // Own bin of slot q: lower pi + P3 q, upper ksrc + P3 (R-1-q).
+    const int kb = upper ? ksrc + P3 * (R - 1) : pi;
+    const int kstep = upper ? -P3 : P3;
+    static_for<0, R>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      cpx A, B;
+      A.x = swap_hi_lo(x[q].x, B.x);
+      A.y = swap_hi_lo(x[q].y, B.y);
+      if (wave == 0) {
+        // lane 0 (k = 0): mirror of Z[P3 q] is its own slot (R - q) mod R;
+        // lane 32 (k = P3/2): mirror of its slot q is its own slot R-1-q.
+        const cpx m0 = x[(R - q) & (R - 1)];
+        const cpx m32 = x[R - 1 - q];
+        if (lane == 0) B = m0;
+        if (lane == 32) A = m32;
       }
-    }
-  });
+      const cpx E = {0.5f * (A.x + B.x), h * (A.y - B.y)};
+      const cpx Od = {0.5f * (A.y + B.y), -h * (A.x - B.x)};
+      constexpr float cqx = float(ct_cos2pi(-q, 2 * R));
+      constexpr float cqy = float(ct_sin2pi(-q, 2 * R));
+      constexpr float crx = float(ct_cos2pi(-(R - 1 - q), 2 * R));
+      constexpr float cry = float(ct_sin2pi(-(R - 1 - q), 2 * R));
+      const cpx c = upper ? cpx{crx, cry} : cpx{cqx, cqy};
+      const cpx wk = cmul(wb, c);
+      const cpx X = cadd(E, cmul(wk, Od));
+      spec[kb + q * kstep] = X;
+      if constexpr (q == 0) {
+        if (t == 0) spec[M] = csub(E, Od);  // Nyquist bin M = E - O from Z[0]
+      }
+    });
 
-  // Welch: term[k] = |X[k] - (X[k-1] + X[k+1]) / 2|^2 with get_bin
-  // neighbours at the edges: X[-1] = conj X[1], X[M+1] = conj X[M-1].
-  // Thread t handles bins t + T j (and t = 0 also bin M).
-  const long long gs = P.block0 + g;
-  const int slot = (int)((gs / P.welch_A) % P.welch_ring);
-  float* acc = P.welch_acc + (long long)slot * P.welch_stride;
-  const int a0 = lds_pad(t);
-  const int am = (t == 0) ? lds_pad(1) : lds_pad(t - 1);
-  const int ap = lds_pad(t + 1);
-  __syncthreads();
-  if constexpr (SPLIT) {
-    // Real parts from the LDS plane, imaginary parts straight from this
-    // block's spectrum row (written by this workgroup before the barrier,
-    // so the reads hit L1/L2).
-    const float* spec_im = reinterpret_cast<const float*>(spec) + 1;
+    // The spectrum row is complete once every lane's stores are visible to
+    // the workgroup; Welch and the fused gather read it back (L1/L2).
+    __syncthreads();
+
+    // Welch: term[k] = |X[k] - (X[k-1] + X[k+1]) / 2|^2 with get_bin
+    // neighbours at the edges: X[-1] = conj X[1], X[M+1] = conj X[M-1].
+    // Thread t handles bins t + T j (and t = 0 also bin M); the neighbours
+    // of a bin come from the adjacent lanes by DPP wave shifts, only lanes
+    // 0 and 63 load theirs.
+    {
+      const long long gs = P.block0 + g;
+      const int slot = (int)(gs / P.A - grp0);
+      const bool fresh = (g == lo) || (gs % P.A == 0);
+      float* part = P.wpart + ((long long)wg * P.wslots + slot) * P.wstride;
 #pragma unroll
-    for (int j = 0; j < R; ++j) {
-      const int o = pad_off(T * j);
-      const int kw = t + T * j;
-      // bin -1 of row j = 0 is conj X[1]; for j > 0 it is bin T j - 1, which
-      // for t = 0 is thread T-1's bin of row j-1.
-      const float cm = (t == 0 && j > 0) ? lds[lds_pad(T * j - 1)] : lds[am + o];
-      const float cr = lds[a0 + o] - 0.5f * (cm + lds[ap + o]);
-      const float* row_im = spec_im + 2 * (T * j);
-      const float im_m = (kw == 0) ? -spec_im[2] : row_im[2 * t - 2];
-      const float ci = row_im[2 * t] - 0.5f * (im_m + row_im[2 * t + 2]);
-      unsafeAtomicAdd(acc + T * j + t, cr * cr + ci * ci);
-      // Keep the scheduler from hoisting every row's loads (register spills).
-      if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
-    }
-    if (t == 0) {
-      const float cr = lds[lds_pad(M)] - lds[lds_pad(M - 1)];
-      const float ci = spec_im[2 * M];
-      unsafeAtomicAdd(acc + M, cr * cr + ci * ci);
-    }
-  } else {
-    const cpx* l = reinterpret_cast<const cpx*>(lds);
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-      const int o = pad_off(T * j);
-      cpx cm;
-      if (t == 0) {
-        if (j > 0) {
-          cm = l[lds_pad(T * j - 1)];
-        } else {
-          cm = l[lds_pad(1)];
-          cm.y = -cm.y;
+      for (int j = 0; j < R; ++j) {
+        const cpx* row = spec + T * j;
+        const int kw = T * j + t;
+        const cpx c0 = row[t];
+        cpx cm, cp;
+        cm.x = wave_shr1(c0.x);
+        cm.y = wave_shr1(c0.y);
+        cp.x = wave_shl1(c0.x);
+        cp.y = wave_shl1(c0.y);
+        if (lane == 0) {
+          if (kw == 0) {
+            cm = spec[1];
+            cm.y = -cm.y;
+          } else {
+            cm = row[t - 1];
+          }
         }
-      } else {
-        cm = l[am + o];
+        if (lane == 63) cp = row[t + 1];
+        const float cr = c0.x - 0.5f * (cm.x + cp.x);
+        const float ci = c0.y - 0.5f * (cm.y + cp.y);
+        const float term = cr * cr + ci * ci;
+        float* pp = part + kw;
+        *pp = fresh ? term : *pp + term;
+        if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
       }
-      const cpx c0 = l[a0 + o];
-      const cpx cp = l[ap + o];
-      const float cr = c0.x - 0.5f * (cm.x + cp.x);
-      const float ci = c0.y - 0.5f * (cm.y + cp.y);
-      unsafeAtomicAdd(acc + T * j + t, cr * cr + ci * ci);
+      if (t == 0) {
+        const cpx c0 = spec[M];
+        const cpx cm = spec[M - 1];
+        const float cr = c0.x - cm.x;  // X[M+1] = conj X[M-1]
+        const float ci = c0.y;
+        const float term = cr * cr + ci * ci;
+        part[M] = fresh ? term : part[M] + term;
+      }
     }
-    if (t == 0) {
-      const cpx c0 = l[lds_pad(M)];
-      const cpx cm = l[lds_pad(M - 1)];
-      const float cr = c0.x - cm.x;
-      const float ci = c0.y;
-      unsafeAtomicAdd(acc + M, cr * cr + ci * ci);
+
+    // Fused channel stage (fcfb.rs:154-188): gather each tile's bins from
+    // the spectrum row (get_bin), weight, inverse FFT through a per-tile LDS
+    // region, keep [FL/8, 7FL/8).
+    if constexpr (FL > 0) {
+      constexpr int TF = FC::TF, RF = FC::RF;
+      const int fc = t / TF, ftt = t % TF;
+      const bool fact = fc < P.fnch;
+      const ChanDesc* fchans = P.fchans;
+      const float* fweights = P.weights;
+      asm volatile("" : "+s"(fchans), "+s"(fweights));
+      const ChanDesc fd = fchans[fact ? fc : 0];
+      cpx gx[RF];
+#pragma unroll
+      for (int e = 0; e < RF; ++e) {
+        const int n = st_in_index<FL, RF, TF, FC::RF1>(ftt, e);
+        const int jj = (n + FL / 2) & (FL - 1);
+        int m = fd.f_mod + jj;
+        if (m >= P.N) m -= P.N;
+        const bool mirror = m >= P.K;
+        cpx v = spec[mirror ? P.N - m : m];
+        if (mirror) v.y = -v.y;
+        gx[e] = cscale(v, fweights[fd.w_off + jj]);
+      }
+      float* tl = lds + fc * FC::STRIDE * 2;
+      st_butterflies<FL, RF, TF, FC::RF1, 1, +1>(gx, ftt, nullptr);
+      lds_exchange<RF, false>(
+          gx, tl, [&](int e) { return st_out_addr<FL, RF, TF, FC::RF1, 1>(ftt, e); },
+          [&](int e) { return st_in_addr<FL, RF, TF, FC::RF2>(ftt, e); });
+      const cpx* ftw2 = P.ftw2;
+      asm volatile("" : "+s"(ftw2));
+      st_butterflies<FL, RF, TF, FC::RF2, FC::RF1, +1>(gx, ftt, ftw2);
+      constexpr int PL = FL / FC::RF2;
+      if (fact) {
+        cpx* out = P.fout + fd.out_off + (long long)g * (3 * FL / 4);
+#pragma unroll
+        for (int e = 0; e < RF; ++e) {
+          const int u = e / FC::RF2, q = e % FC::RF2;
+          const int idx = ftt + TF * u + q * PL;
+          if (idx >= FL / 8 && idx < (FL / 8) * 7) out[idx - FL / 8] = gx[e];
+        }
+      }
     }
   }
 }
@@ -245,12 +322,6 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
 // A tile is one (channel, block) pair handled by T = L/R threads; a
 // 256-thread workgroup holds 256/T tiles, each with its own LDS region.
 // ---------------------------------------------------------------------------
-struct ChanDesc {
-  long long out_off;  // cpx offset of the channel's output (block 0)
-  int f_mod;          // first bin mod N, in [0, N)
-  int w_off;          // float offset of the channel's weights
-};
-
 struct ChanParams {
   const cpx* spec;
   long long spec_stride;
@@ -326,20 +397,49 @@ __global__ __launch_bounds__(kChanThreads) void chan_kernel(ChanParams P) {
 }
 
 // ---------------------------------------------------------------------------
-// Welch records (spektri/src/dsp/spectrum.rs:113-160): acc / A for every
-// group completed in this launch, then the slot is cleared for reuse.
+// Welch records (spektri/src/dsp/spectrum.rs:113-160).  For every group s
+// touched by this launch: sum the partial rows of the workgroups that
+// processed its blocks (plus the carry of a group begun in an earlier
+// launch); a complete group becomes the record sum / A, an incomplete last
+// group goes back to the carry row.
 // ---------------------------------------------------------------------------
-__global__ void welch_finalize_kernel(float* acc, long long stride, int ring, long long s_first,
-                                      int nrec, int K, int A, float* out,
-                                      long long out_stride) {
-  const long long total = (long long)nrec * K;
+struct FinParams {
+  const float* wpart;
+  long long wstride;
+  int wslots;
+  int G;               // workgroups of the forward launch
+  int nblocks;
+  long long block0;
+  int A, K;
+  long long s_first;
+  int ngroups;         // groups touched
+  float* carry;        // K floats
+  float* out;          // records [r][K]
+};
+
+__global__ void welch_finalize_kernel(FinParams F) {
+  const long long total = (long long)F.ngroups * F.K;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (long long)gridDim.x * blockDim.x) {
-    const int r = (int)(i / K), k = (int)(i % K);
-    const int slot = (int)((s_first + r) % ring);
-    float* a = acc + (long long)slot * stride + k;
-    out[(long long)r * out_stride + k] = *a / (float)A;
-    *a = 0.f;
+    const int gi = (int)(i / F.K), k = (int)(i % F.K);
+    const long long s = F.s_first + gi;
+    const long long b_lo = s * F.A > F.block0 ? s * F.A - F.block0 : 0;
+    long long b_hi = (s + 1) * F.A - F.block0;
+    if (b_hi > F.nblocks) b_hi = F.nblocks;
+    float sum = (gi == 0 && F.block0 % F.A != 0) ? F.carry[k] : 0.f;
+    // workgroup of block b: the w with floor(w nb / G) <= b < floor((w+1) nb / G)
+    const int w0 = (int)(((b_lo + 1) * F.G + F.nblocks - 1) / F.nblocks) - 1;
+    const int w1 = (int)((b_hi * F.G + F.nblocks - 1) / F.nblocks) - 1;
+    for (int wg = w0; wg <= w1; ++wg) {
+      const long long lo = (long long)wg * F.nblocks / F.G;
+      const long long slot = s - (F.block0 + lo) / F.A;
+      sum += F.wpart[((long long)wg * F.wslots + slot) * F.wstride + k];
+    }
+    if ((s + 1) * F.A <= F.block0 + F.nblocks) {
+      F.out[(long long)gi * F.K + k] = sum / (float)F.A;
+    } else {
+      F.carry[k] = sum;
+    }
   }
 }
 

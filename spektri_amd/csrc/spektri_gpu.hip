@@ -1,11 +1,12 @@
 // C ABI of the FCFB + Welch hot path (declared in include/spektri_gpu.h).
 //
 // A context owns: the N/4-sample input history, a staging buffer for host
-// input, the per-block spectrum buffer, a ring of Welch accumulators, the
-// Welch record buffer, per-channel output buffers, and the twiddle tables.
-// One process call launches: the forward kernel over all blocks (fused s16
-// load, FFT, spectrum write, Welch accumulation), one channel kernel per
-// distinct IFFT size, and the Welch finaliser for groups completed.
+// input, the spectrum rows, the Welch partial rows and carry row, the Welch
+// record buffer, per-channel output buffers, and the twiddle tables.
+// One process call launches: the persistent forward kernel (fused s16 load,
+// FFT, Welch partial sums and, when every channel has L = 256 and they fit,
+// the fused channel stage), one channel kernel per distinct IFFT size for
+// the channels not fused, and the Welch finaliser.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -74,6 +75,10 @@ const ChanPlan kChanPlans[] = {{8, 4, 4, 2, 1},       {16, 4, 4, 4, 1},      {32
                                {512, 32, 32, 16, 1},  {1024, 32, 32, 32, 1}, {2048, 32, 32, 32, 2},
                                {4096, 16, 16, 16, 16}, {8192, 32, 32, 16, 16}};
 
+// Channels that the fused forward kernel can take: IFFT size 256, at most
+// one 16-thread tile per channel.
+constexpr int kFusedL = 256;
+
 const ChanPlan* find_chan_plan(int L) {
   for (const auto& p : kChanPlans)
     if (p.L == L) return &p;
@@ -117,10 +122,13 @@ struct spk_ctx {
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
 
-  DevBuf hist, head, staging, spec, welch_acc, welch_out, chan_out, weights;
+  DevBuf hist, head, staging, spec, wpart, carry, welch_out, chan_out, weights;
   DevBuf ftw2, ftw3, ftwu;
-  long long spec_stride = 0, welch_stride = 0;
-  int ring = 0, welch_out_cap = 0;
+  long long spec_stride = 0, wstride = 0;
+  int wslots = 0, welch_out_cap = 0;
+  int gmax[2] = {0, 0};      // persistent workgroups of the unfused / fused forward kernel
+  bool keep_spec = false;
+  bool fused = false;        // channels run inside the forward kernel
 
   std::vector<Channel> chans;
   std::vector<long long> chan_out_off;  // cpx offsets per channel (capacity layout)
@@ -211,27 +219,48 @@ spk_status rebuild_channels(spk_ctx* c) {
     HIP_TRY(c, upload(g.tw2, pass_table(p->R2, p->R1, +1)));
     HIP_TRY(c, upload(g.tw3, pass_table(p->R3, p->R1 * p->R2, +1)));
   }
+  // The forward kernel runs the L = 256 channels itself when one 16-thread
+  // tile per channel fits in its workgroup.
+  const auto it = c->groups.find(kFusedL);
+  c->fused = it != c->groups.end() && (int)it->second.ids.size() * 16 <= c->M / c->fplan->R;
   c->chans_dirty = false;
   return SPK_OK;
 }
 
-template <int M, int R, int R1, int R2>
-void launch_fwd(const spk::FwdParams& P, hipStream_t s) {
-  hipLaunchKernelGGL((spk::fwd_real_kernel<M, R, R1, R2>), dim3(P.nblocks), dim3(M / R), 0, s, P);
-}
-
-hipError_t dispatch_fwd(int M, const spk::FwdParams& P, hipStream_t s) {
-  switch (M) {
-    case 2048: launch_fwd<2048, 16, 8, 16>(P, s); break;
-    case 4096: launch_fwd<4096, 16, 16, 16>(P, s); break;
-    case 8192: launch_fwd<8192, 32, 16, 16>(P, s); break;
-    case 16384: launch_fwd<16384, 32, 16, 32>(P, s); break;
-    case 32768: launch_fwd<32768, 32, 32, 32>(P, s); break;
-    default: return hipErrorInvalidValue;
-  }
+template <int M, int R, int R1, int R2, int FL>
+hipError_t launch_fwd(const spk::FwdParams& P, int G, hipStream_t s) {
+  hipLaunchKernelGGL((spk::fwd_real_kernel<M, R, R1, R2, FL>), dim3(G), dim3(M / R), 0, s, P);
   return hipGetLastError();
 }
 
+template <int M, int R, int R1, int R2, int FL>
+int fwd_wgs_per_cu() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)spk::fwd_real_kernel<M, R, R1, R2, FL>, M / R,
+                                                   0) != hipSuccess)
+    return 0;
+  return n;
+}
+
+struct FwdVariant {
+  int M, R, FL;
+  hipError_t (*launch)(const spk::FwdParams&, int, hipStream_t);
+  int (*occupancy)();
+};
+#define SPK_FWD_VARIANT(M, R, R1, R2, FL) \
+  {M, R, FL, &launch_fwd<M, R, R1, R2, FL>, &fwd_wgs_per_cu<M, R, R1, R2, FL>}
+const FwdVariant kFwdVariants[] = {
+    SPK_FWD_VARIANT(2048, 16, 8, 16, 0),    SPK_FWD_VARIANT(2048, 16, 8, 16, 256),
+    SPK_FWD_VARIANT(4096, 16, 16, 16, 0),   SPK_FWD_VARIANT(4096, 16, 16, 16, 256),
+    SPK_FWD_VARIANT(8192, 32, 16, 16, 0),   SPK_FWD_VARIANT(8192, 32, 16, 16, 256),
+    SPK_FWD_VARIANT(16384, 32, 16, 32, 0),  SPK_FWD_VARIANT(16384, 32, 16, 32, 256),
+    SPK_FWD_VARIANT(32768, 32, 32, 32, 0),  SPK_FWD_VARIANT(32768, 32, 32, 32, 256),
+};
+const FwdVariant* find_fwd_variant(int M, int FL) {
+  for (const auto& v : kFwdVariants)
+    if (v.M == M && v.FL == FL) return &v;
+  return nullptr;
+}
 template <int L, int R, int R1, int R2, int R3>
 void launch_chan(const spk::ChanParams& P, hipStream_t s) {
   constexpr int T = L / R;
@@ -288,24 +317,42 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
   P.head = c->head.as<int16_t>();
   P.spec = c->spec.as<cpx>();
   P.spec_stride = c->spec_stride;
+  // Per-block spectrum rows are needed when kept or read by a channel
+  // kernel; otherwise each workgroup reuses one scratch row.
+  const size_t unfused_groups = c->groups.size() - (c->fused ? 1 : 0);
+  P.keep_spec = (c->keep_spec || unfused_groups > 0) ? 1 : 0;
   P.tw2 = c->ftw2.as<cpx>();
   P.tw3 = c->ftw3.as<cpx>();
   P.twu = c->ftwu.as<cpx>();
-  P.welch_acc = c->welch_acc.as<float>();
-  P.welch_stride = c->welch_stride;
-  P.welch_ring = c->ring;
-  P.welch_A = c->A;
+  P.wpart = c->wpart.as<float>();
+  P.wstride = c->wstride;
+  P.wslots = c->wslots;
+  P.A = c->A;
   P.block0 = c->blocks_done;
   P.wscale = c->wscale;
   P.nblocks = nb;
   P.H = c->H;
   P.O = c->O;
+  P.N = c->N;
+  P.K = c->K;
+  const ChanGroup* fg = nullptr;
+  if (c->fused) {
+    fg = &c->groups.at(kFusedL);
+    P.fchans = fg->desc.as<spk::ChanDesc>();
+    P.fnch = (int)fg->ids.size();
+    P.weights = c->weights.as<float>();
+    P.fout = c->chan_out.as<cpx>();
+    P.ftw2 = fg->tw2.as<cpx>();
+  }
+  const FwdVariant* fv = find_fwd_variant(c->M, c->fused ? kFusedL : 0);
+  const int G = std::min(nb, c->gmax[c->fused ? 1 : 0]);
   if (ev) HIP_TRY(c, hipEventRecord(ev[0], s));
-  HIP_TRY(c, dispatch_fwd(c->M, P, s));
+  HIP_TRY(c, fv->launch(P, G, s));
   if (ev) HIP_TRY(c, hipEventRecord(ev[1], s));
 
   for (auto& kv : c->groups) {
     ChanGroup& g = kv.second;
+    if (&g == fg) continue;
     spk::ChanParams Q{};
     Q.spec = c->spec.as<cpx>();
     Q.spec_stride = c->spec_stride;
@@ -325,14 +372,25 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
   if (ev) HIP_TRY(c, hipEventRecord(ev[2], s));
 
   const long long s_first = c->blocks_done / c->A;
-  const long long s_end = (c->blocks_done + nb) / c->A;
-  const int nrec = (int)(s_end - s_first);
-  if (nrec > 0) {
-    const long long total = (long long)nrec * c->K;
-    const unsigned grid = (unsigned)std::min<long long>((total + 255) / 256, 4096);
-    hipLaunchKernelGGL(spk::welch_finalize_kernel, dim3(grid), dim3(256), 0, s, c->welch_acc.as<float>(),
-                       c->welch_stride, c->ring, s_first, nrec, c->K, c->A, c->welch_out.as<float>(),
-                       (long long)c->K);
+  const long long s_last = (c->blocks_done + nb - 1) / c->A;
+  const int nrec = (int)((c->blocks_done + nb) / c->A - s_first);
+  {
+    spk::FinParams Fp{};
+    Fp.wpart = c->wpart.as<float>();
+    Fp.wstride = c->wstride;
+    Fp.wslots = c->wslots;
+    Fp.G = G;
+    Fp.nblocks = nb;
+    Fp.block0 = c->blocks_done;
+    Fp.A = c->A;
+    Fp.K = c->K;
+    Fp.s_first = s_first;
+    Fp.ngroups = (int)(s_last - s_first + 1);
+    Fp.carry = c->carry.as<float>();
+    Fp.out = c->welch_out.as<float>();
+    const long long total = (long long)Fp.ngroups * c->K;
+    const unsigned grid = (unsigned)std::min<long long>((total + 255) / 256, 8192);
+    hipLaunchKernelGGL(spk::welch_finalize_kernel, dim3(grid), dim3(256), 0, s, Fp);
     HIP_TRY(c, hipGetLastError());
   }
   if (ev) HIP_TRY(c, hipEventRecord(ev[3], s));
@@ -395,18 +453,34 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) return cleanup(SPK_ERR_DEVICE);
   c->stream = c->own_stream;
 
+  c->keep_spec = (cfg->flags & SPK_FLAG_KEEP_SPECTRA) != 0;
+  // Persistent forward grid: resident workgroups per CU times CUs.
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+    return cleanup(SPK_ERR_DEVICE);
+  for (int f = 0; f < 2; ++f) {
+    const FwdVariant* v = find_fwd_variant(c->M, f ? kFusedL : 0);
+    const int per_cu = v ? v->occupancy() : 0;
+    if (per_cu <= 0) return cleanup(SPK_ERR_DEVICE);
+    c->gmax[f] = per_cu * ncu;
+  }
+  const int gcap = std::max(c->gmax[0], c->gmax[1]);
+  const int gmin = std::min(c->gmax[0], c->gmax[1]);
+  // A workgroup spans at most ceil(ceil(max_blocks / G) / A) + 1 Welch groups.
+  const int bpw = (c->max_blocks + gmin - 1) / gmin;
+  c->wslots = (bpw + c->A - 1) / c->A + 1;
   c->spec_stride = ((long long)c->K + 63) / 64 * 64;
-  c->welch_stride = c->spec_stride;
-  c->ring = (c->max_blocks + c->A - 1) / c->A + 2;
+  c->wstride = c->spec_stride;
   c->welch_out_cap = c->max_blocks / c->A + 1;
   if (c->hist.alloc((size_t)c->O * c->bps) != hipSuccess || c->head.alloc((size_t)c->N * c->bps) != hipSuccess ||
       c->staging.alloc((size_t)c->F * c->H * c->bps) != hipSuccess ||
-      c->spec.alloc((size_t)c->max_blocks * c->spec_stride * sizeof(cpx)) != hipSuccess ||
-      c->welch_acc.alloc((size_t)c->ring * c->welch_stride * sizeof(float)) != hipSuccess ||
+      c->spec.alloc((size_t)std::max(c->max_blocks, gcap) * c->spec_stride * sizeof(cpx)) != hipSuccess ||
+      c->wpart.alloc((size_t)gcap * c->wslots * c->wstride * sizeof(float)) != hipSuccess ||
+      c->carry.alloc((size_t)c->K * sizeof(float)) != hipSuccess ||
       c->welch_out.alloc((size_t)c->welch_out_cap * c->K * sizeof(float)) != hipSuccess)
     return cleanup(SPK_ERR_DEVICE);
-  if (hipMemset(c->hist.p, 0, c->hist.bytes) != hipSuccess ||
-      hipMemset(c->welch_acc.p, 0, c->welch_acc.bytes) != hipSuccess)
+  if (hipMemset(c->hist.p, 0, c->hist.bytes) != hipSuccess || hipMemset(c->carry.p, 0, c->carry.bytes) != hipSuccess)
     return cleanup(SPK_ERR_DEVICE);
 
   const FwdPlan& p = *c->fplan;
@@ -541,7 +615,7 @@ spk_status spk_reset(spk_ctx* c) {
   if (!c) return SPK_ERR_INVALID;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   HIP_TRY(c, hipMemset(c->hist.p, 0, c->hist.bytes));
-  HIP_TRY(c, hipMemset(c->welch_acc.p, 0, c->welch_acc.bytes));
+  HIP_TRY(c, hipMemset(c->carry.p, 0, c->carry.bytes));
   c->blocks_done = 0;
   c->last_blocks = 0;
   c->last_records = 0;

@@ -53,6 +53,7 @@ class DspParams:
     filters: list = field(default_factory=list)
     input_format: int = _lib.FMT_S16LE
     max_blocks: int = 0             # capacity for process_device; 0 = ffts_per_buf
+    keep_spectra: bool = False      # keep every block's spectrum on the device
 
 
 @dataclass
@@ -101,7 +102,8 @@ class DspState:
         """dsp.rs:76-112: returns (state, InputBufferSize)."""
         lib = _lib.load()
         cfg = _lib.SpkConfig(params.fft_size, params.input_format, params.scaling,
-                             params.ffts_per_buf, params.spectrum_averages, params.max_blocks)
+                             params.ffts_per_buf, params.spectrum_averages, params.max_blocks,
+                             _lib.FLAG_KEEP_SPECTRA if params.keep_spectra else 0)
         ctx = ctypes.c_void_p()
         _lib.check(lib.spk_create(ctypes.byref(cfg), ctypes.byref(ctx)))
         bs = _lib.SpkBufferSize()

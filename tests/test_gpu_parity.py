@@ -25,10 +25,11 @@ def geom_of(wl_or_n, F, complex_in=False):
     return O.Geometry(wl_or_n, complex_in, F)
 
 
-def run_gpu(x_i16: np.ndarray, N, F, A, channels, max_blocks=0, use_device=True):
-    """Run the GPU path on one stream; returns (state, blocks)."""
+def run_gpu(x_i16: np.ndarray, N, F, A, channels, max_blocks=0, use_device=True, keep_spectra=True):
+    """Run the GPU path on one stream; returns (state, channel ids)."""
     st, bs = DspState.init(DspParams(complex=False, fs_in=1.0, fft_size=N, ffts_per_buf=F,
-                                     spectrum_averages=A, max_blocks=max_blocks))
+                                     spectrum_averages=A, max_blocks=max_blocks,
+                                     keep_spectra=keep_spectra))
     ids = [st.add_channel(first, L, w) for first, L, w in channels]
     if use_device:
         dev = torch.from_numpy(x_i16).to("cuda")
@@ -64,8 +65,8 @@ def edge_channels(N, K, rng):
     return out
 
 
-def compare(tag, st, ids, channels, X, recs, outs, nb, K, extra=None):
-    Xg = gpu_spectra(st, nb, K)
+def compare(tag, st, ids, channels, X, recs, outs, nb, K, extra=None, spectra=True):
+    Xg = gpu_spectra(st, nb, K) if spectra else X
     e_spec = rms(Xg, X)
     recs_g = st.spectrum_records()
     assert recs_g.shape == recs.shape, (recs_g.shape, recs.shape)
@@ -81,6 +82,27 @@ def compare(tag, st, ids, channels, X, recs, outs, nb, K, extra=None):
         vals.update(extra)
     report(tag, **vals)
     assert e_spec < TOL and e_welch < TOL and max(e_ch) < TOL, vals
+
+
+@pytest.mark.parametrize("N", [8192, 65536])
+def test_fused_channels_without_kept_spectra(N):
+    """L = 256 channels run inside the forward kernel and the spectrum is
+    never written per block; mixed with other sizes it falls back to the
+    channel kernels for those, which then need the spectrum rows."""
+    rng = np.random.default_rng(N + 1)
+    F, A = 4, 3
+    H = 3 * N // 4
+    x = np.clip(np.round(rng.standard_normal(5 * F * H) * 4000), -32768, 32767).astype(np.int16)
+    K = N // 2 + 1
+    only256 = [(int(f), 256, rng.random(256).astype(np.float32)) for f in rng.integers(-300, K, size=12)]
+    only256.append((K - 100, 256, rng.random(256).astype(np.float32)))  # spans Nyquist
+    mixed = only256 + [(int(rng.integers(0, K)), L, rng.random(L).astype(np.float32)) for L in (64, 1024)]
+    for tag, chans in (("fused", only256), ("fused_mixed", mixed)):
+        st, ids = run_gpu(x, N, F, A, chans, max_blocks=5 * F, keep_spectra=False)
+        nb = st.last_blocks()
+        X, recs, outs = oracle_path(x, N, F, A, chans)
+        compare(f"{tag}_N{N}", st, ids, chans, X, recs, outs, nb, K, spectra=False)
+        st.close()
 
 
 def test_plain_8k_full_size():
@@ -169,12 +191,18 @@ def test_rx888_full_size_sampled_blocks():
     x = dev.cpu().numpy()
     N, F, A = wl.fft_size, wl.ffts_per_buf, wl.averages
     st, bs = DspState.init(DspParams(complex=False, fs_in=wl.fs, fft_size=N, ffts_per_buf=F,
-                                     spectrum_averages=A, max_blocks=wl.nblocks))
+                                     spectrum_averages=A, max_blocks=wl.nblocks, keep_spectra=True))
     ids = [st.add_channel(first, L, w) for first, L, w in wl.channels]
     st.process_device(dev.data_ptr(), wl.nsamples)
     st.synchronize()
     nb = st.last_blocks()
     assert nb == wl.nblocks == 1360
+    # the bench configuration (spectra not kept) must give the same results
+    st2, _ = DspState.init(DspParams(complex=False, fs_in=wl.fs, fft_size=N, ffts_per_buf=F,
+                                     spectrum_averages=A, max_blocks=wl.nblocks))
+    ids2 = [st2.add_channel(first, L, w) for first, L, w in wl.channels]
+    st2.process_device(dev.data_ptr(), wl.nsamples)
+    st2.synchronize()
     K = wl.bins
     geom = O.Geometry(N, False, F)
     s = O.input_format_scaling(O.FMT_S16LE)
@@ -193,6 +221,9 @@ def test_rx888_full_size_sampled_blocks():
             yo = O.channel_process(X, N, first, np.asarray(w, np.float64))
             e = max(e, rms(yg[g0:g0 + A], yo))
         errs[f"channels_g{g0}"] = e
-    report("rx888_64k", blocks=nb, **errs)
+    same = all(np.array_equal(st.channel_output(a), st2.channel_output(b)) for a, b in zip(ids, ids2))
+    errs["bench_config_vs_kept_welch"] = rms(st2.spectrum_records(), recs_g)
+    report("rx888_64k", blocks=nb, bench_config_bitwise_equal_channels=bool(same), **errs)
     assert max(errs.values()) < TOL, errs
     st.close()
+    st2.close()
