@@ -25,12 +25,37 @@ constexpr int kLdsBudgetBytes = 150 * 1024;
 // v_permlane32_swap per float gives every lane its mirror values without
 // an LDS round trip.
 // ---------------------------------------------------------------------------
-struct ChanDesc;
+struct ChanDesc {
+  long long out_off;  // cpx offset of the channel's output (block 0)
+  int f_mod;          // first bin mod N, in [0, N)
+  int w_off;          // float offset of the channel's weights
+};
+
+// Lanes of every wave whose spectrum values are published to the edge
+// table (global scratch, one region per workgroup, L2-resident): the wave
+// edges and their inner neighbours, plus wave 0's lanes 33/34 and the
+// Nyquist bin.  Index -1 = not published.
+constexpr int kEdgeLanes = 10;
+__device__ __forceinline__ int edge_slot(int lane) {
+  switch (lane) {
+    case 0: return 0;
+    case 1: return 1;
+    case 30: return 2;
+    case 31: return 3;
+    case 32: return 4;
+    case 33: return 5;
+    case 34: return 6;
+    case 62: return 7;
+    case 63: return 8;
+    default: return -1;
+  }
+}
+constexpr int kEdgeNyq = 9;  // wave 0, slot 0: X[M]
 
 struct FwdParams {
   const int16_t* src;   // stream samples; sample s >= 0 at src[s]
   const int16_t* head;  // block 0's N samples: O samples of history, then src[0 .. N-O)
-  cpx* spec;            // spectrum rows: per block when keep_spec, else one scratch row per workgroup
+  cpx* spec;            // per-block spectrum rows (written when keep_spec)
   long long spec_stride;
   int keep_spec;
   const cpx* tw2;       // pass-2 table, forward sign
@@ -39,6 +64,7 @@ struct FwdParams {
   float* wpart;         // Welch partial sums [workgroup][slot][wstride]
   long long wstride;
   int wslots;
+  cpx* edge;            // edge tables [workgroup][wave][kEdgeLanes][R]
   int A;
   long long block0;     // stream index of this launch's block 0
   float wscale;
@@ -47,16 +73,10 @@ struct FwdParams {
   // fused channel stage (FL > 0): every channel has IFFT size FL
   const ChanDesc* fchans;
   int fnch;
-  const float* weights;
+  const int* fmap;      // per bin: 2 * (tile * FL + n) + conj, or -1
+  const float* fw;      // per tile position: the channel weight
   cpx* fout;
   const cpx* ftw2;      // pass-2 table of the FL-point inverse transform
-  int N, K;
-};
-
-struct ChanDesc {
-  long long out_off;  // cpx offset of the channel's output (block 0)
-  int f_mod;          // first bin mod N, in [0, N)
-  int w_off;          // float offset of the channel's weights
 };
 
 // Fused channel tiles: FL points as 16 x (FL/16), 16 points per thread.
@@ -76,14 +96,18 @@ struct FusedCfg<0> {
 template <int M, int R, int R1, int R2, int FL>
 struct FwdRealCfg {
   static constexpr int T = M / R;
+  static constexpr int W = T / 64;  // waves
   static constexpr int P3 = R1 * R2;
   static_assert(R1 * R2 * R == M, "radix plan must cover M");
   static_assert(P3 == T, "last pass must have one butterfly per thread");
   static_assert(T >= 64 && T <= 1024, "one wave minimum, 1024 threads maximum");
-  static constexpr bool SPLIT = lds_padded_size(M + 1) * 8 > kLdsBudgetBytes;
-  static constexpr int PLANE_FLOATS = SPLIT ? lds_padded_size(M + 1) : 2 * lds_padded_size(M + 1);
+  static constexpr bool SPLIT = lds_padded_size(M) * 8 > kLdsBudgetBytes;
+  static constexpr int PLANE_FLOATS = SPLIT ? lds_padded_size(M) : 2 * lds_padded_size(M);
   static constexpr int TILE_FLOATS = FL > 0 ? (T / FusedCfg<FL>::TF) * FusedCfg<FL>::STRIDE * 2 : 0;
   static constexpr int LDS_FLOATS = PLANE_FLOATS > TILE_FLOATS ? PLANE_FLOATS : TILE_FLOATS;
+  // After the main area: the pass-3 outputs of lanes 0 and 32 of wave 0
+  // (2 R complex), which are their own mirrors.
+  static constexpr int STASH_FLOATS = 4 * R;
 };
 
 __device__ __forceinline__ float swap_hi_lo(float v, float& other) {
@@ -106,18 +130,36 @@ __device__ __forceinline__ float wave_shl1(float v) {
       __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130, 0xF, 0xF, false));
 }
 
+__device__ __forceinline__ float welch_term(cpx c0, cpx cm, cpx cp) {
+  const float cr = c0.x - 0.5f * (cm.x + cp.x);
+  const float ci = c0.y - 0.5f * (cm.y + cp.y);
+  return cr * cr + ci * ci;
+}
+
 // Persistent workgroups: workgroup w takes blocks [w nb / G, (w+1) nb / G)
 // in order, so its Welch sums stay in a private partial row (slot = Welch
 // group - first group of the workgroup) that is written with plain stores;
 // welch_finalize_kernel adds the rows of every group.
+//
+// After the last pass every lane owns R bins of the real spectrum: lower
+// lanes pi + P3 q, upper lanes ksrc + P3 (R-1-q) (slot q).  A bin's
+// neighbours k +- 1 sit in the adjacent lanes of the same slot, so Welch
+// takes them by DPP wave shifts (lower lanes: left = lane-1, right =
+// lane+1; upper lanes the other way round).  Lanes where that is wrong (the
+// wave edges, and the wrap-around lanes of waves 0 and W-1) are finished
+// after a barrier from the edge table.  Channels of the fused stage are
+// gathered by scattering each owned bin into an LDS tile buffer through the
+// bin -> position map.
 template <int M, int R, int R1, int R2, int FL>
 __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   using C = FwdRealCfg<M, R, R1, R2, FL>;
   using FC = FusedCfg<FL>;
   constexpr int T = C::T;
+  constexpr int W = C::W;
   constexpr int P3 = C::P3;
   constexpr bool SPLIT = C::SPLIT;
-  __shared__ float lds[C::LDS_FLOATS];
+  __shared__ float lds[C::LDS_FLOATS + C::STASH_FLOATS];
+  cpx* stash = reinterpret_cast<cpx*>(lds + C::LDS_FLOATS);
 
   const int G = gridDim.x, wg = blockIdx.x;
   const int lo = (int)((long long)wg * P.nblocks / G);
@@ -125,7 +167,25 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   const long long grp0 = (P.block0 + lo) / P.A;
   const float w = P.wscale;
 
+  // Raw s16 pairs of the block about to be transformed: point n = b + j M/R1
+  // of pass 1 is the pair at block sample 2n.  Rows are addressed as (uniform
+  // row pointer) + lane offset.  The next block's pairs are loaded while the
+  // current block's channel stage runs.
+  auto load_block = [&](int g, int (&vin)[R]) {
+    const int* blk = reinterpret_cast<const int*>(g == 0 ? P.head : P.src + ((long long)g * P.H - P.O));
+#pragma unroll
+    for (int e = 0; e < R; ++e) {
+      const int u = e / R1, j = e % R1;
+      vin[e] = blk[T * u + j * (M / R1) + (int)threadIdx.x];
+    }
+  };
+  // 1024-thread workgroups have no registers to spare for the prefetch.
+  constexpr bool PREFETCH = T < 1024;
+  int vin[R];
+  if (PREFETCH && lo < hi) load_block(lo, vin);
+
   for (int g = lo; g < hi; ++g) {
+    if constexpr (!PREFETCH) load_block(g, vin);
     // Per-block values that do not depend on g (lane maps, table reads) are
     // recomputed and reloaded every block: kept live across the loop they
     // would cost more registers than the 1024-thread workgroups have.
@@ -145,20 +205,11 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     const int kt = (upper && pi == 0) ? P3 / 2 : pi;
     const float sg = upper ? -1.f : 1.f;
 
-    // Block g starts at stream sample g*H - O; block 0 reads the head copy.
-    const int* blk = reinterpret_cast<const int*>(g == 0 ? P.head : P.src + ((long long)g * P.H - P.O));
-
     cpx x[R];
-    // Pass 1 loads: point n = b + j M/R1 is the s16 pair at block sample 2n.
-    // Rows are addressed as (uniform row pointer) + lane offset so that the
-    // lane offsets are shared instead of kept per row.
 #pragma unroll
     for (int e = 0; e < R; ++e) {
-      const int u = e / R1, j = e % R1;
-      const int* row = blk + (T * u + j * (M / R1));
-      const int v = row[t];
-      x[e].x = (float)(int16_t)v * w;
-      x[e].y = (float)(v >> 16) * w;
+      x[e].x = (float)(int16_t)vin[e] * w;
+      x[e].y = (float)(vin[e] >> 16) * w;
     }
     st_butterflies<M, R, T, R1, 1, -1>(x, t, nullptr);
 
@@ -183,13 +234,31 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     for (int e = 0; e < R; ++e) x[e].y *= sg;
     // Lower lanes now hold Z[pi + P3 q] in slot q, upper lanes
     // Z[ksrc + P3 (R-1-q)], so slot q of lane l and lane l^32 are mirrors
-    // (except the pair pi = 0, whose lanes are their own mirrors).
+    // (except the pair pi = 0, whose lanes are their own mirrors).  Their
+    // slots go to a private LDS stash so every slot register dies once it is
+    // untangled.
+    if (wave == 0 && (lane & 31) == 0) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) stash[(upper ? R : 0) + q] = x[q];
+    }
     __syncthreads();  // every lane is done reading pass-3 inputs from LDS
 
+    const long long gs = P.block0 + g;
+    const int slot = (int)(gs / P.A - grp0);
+    const bool fresh = (g == lo) || (gs % P.A == 0);
+    float* part = P.wpart + ((long long)wg * P.wslots + slot) * P.wstride;
+    cpx* etab = P.edge + (long long)wg * (W * kEdgeLanes * R);
+    const int es = edge_slot(lane);
+    // Lanes whose Welch neighbours are not both reachable by DPP.
+    const bool edge_lane = lane == 0 || lane == 31 || lane == 32 || lane == 63 || (wave == 0 && lane == 33);
+    const int* fmap = P.fmap;
+    const float* fw = P.fw;
+    asm volatile("" : "+s"(fmap), "+s"(fw));
+
+    // Untangle, Welch for interior lanes, edge publication, channel scatter.
     const cpx wb = twu[upper ? ksrc : pi];
     const float h = 0.5f * sg;
-    cpx* spec = P.spec + (P.keep_spec ? (long long)g : (long long)wg) * P.spec_stride;
-    // Own bin of slot q: lower pi + P3 q, upper ksrc + P3 (R-1-q).
+    cpx* spec = P.spec + (long long)g * P.spec_stride;
     const int kb = upper ? ksrc + P3 * (R - 1) : pi;
     const int kstep = upper ? -P3 : P3;
     static_for<0, R>([&](auto qc) {
@@ -200,10 +269,8 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
       if (wave == 0) {
         // lane 0 (k = 0): mirror of Z[P3 q] is its own slot (R - q) mod R;
         // lane 32 (k = P3/2): mirror of its slot q is its own slot R-1-q.
-        const cpx m0 = x[(R - q) & (R - 1)];
-        const cpx m32 = x[R - 1 - q];
-        if (lane == 0) B = m0;
-        if (lane == 32) A = m32;
+        if (lane == 0) B = stash[(R - q) & (R - 1)];
+        if (lane == 32) A = stash[R + R - 1 - q];
       }
       const cpx E = {0.5f * (A.x + B.x), h * (A.y - B.y)};
       const cpx Od = {0.5f * (A.y + B.y), -h * (A.x - B.x)};
@@ -214,86 +281,111 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
       const cpx c = upper ? cpx{crx, cry} : cpx{cqx, cqy};
       const cpx wk = cmul(wb, c);
       const cpx X = cadd(E, cmul(wk, Od));
-      spec[kb + q * kstep] = X;
+      const int own = kb + q * kstep;
+      if (P.keep_spec) spec[own] = X;
+      if (es >= 0) etab[(wave * kEdgeLanes + es) * R + q] = X;
       if constexpr (q == 0) {
-        if (t == 0) spec[M] = csub(E, Od);  // Nyquist bin M = E - O from Z[0]
-      }
-    });
-
-    // The spectrum row is complete once every lane's stores are visible to
-    // the workgroup; Welch and the fused gather read it back (L1/L2).
-    __syncthreads();
-
-    // Welch: term[k] = |X[k] - (X[k-1] + X[k+1]) / 2|^2 with get_bin
-    // neighbours at the edges: X[-1] = conj X[1], X[M+1] = conj X[M-1].
-    // Thread t handles bins t + T j (and t = 0 also bin M); the neighbours
-    // of a bin come from the adjacent lanes by DPP wave shifts, only lanes
-    // 0 and 63 load theirs.
-    {
-      const long long gs = P.block0 + g;
-      const int slot = (int)(gs / P.A - grp0);
-      const bool fresh = (g == lo) || (gs % P.A == 0);
-      float* part = P.wpart + ((long long)wg * P.wslots + slot) * P.wstride;
-#pragma unroll
-      for (int j = 0; j < R; ++j) {
-        const cpx* row = spec + T * j;
-        const int kw = T * j + t;
-        const cpx c0 = row[t];
-        cpx cm, cp;
-        cm.x = wave_shr1(c0.x);
-        cm.y = wave_shr1(c0.y);
-        cp.x = wave_shl1(c0.x);
-        cp.y = wave_shl1(c0.y);
-        if (lane == 0) {
-          if (kw == 0) {
-            cm = spec[1];
-            cm.y = -cm.y;
-          } else {
-            cm = row[t - 1];
-          }
+        if (t == 0) {  // Nyquist bin M = E - O from Z[0]
+          const cpx XN = csub(E, Od);
+          if (P.keep_spec) spec[M] = XN;
+          etab[kEdgeNyq * R] = XN;
         }
-        if (lane == 63) cp = row[t + 1];
-        const float cr = c0.x - 0.5f * (cm.x + cp.x);
-        const float ci = c0.y - 0.5f * (cm.y + cp.y);
-        const float term = cr * cr + ci * ci;
-        float* pp = part + kw;
-        *pp = fresh ? term : *pp + term;
-        if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
       }
-      if (t == 0) {
-        const cpx c0 = spec[M];
-        const cpx cm = spec[M - 1];
-        const float cr = c0.x - cm.x;  // X[M+1] = conj X[M-1]
-        const float ci = c0.y;
-        const float term = cr * cr + ci * ci;
-        part[M] = fresh ? term : part[M] + term;
+      // Welch neighbours by DPP: lower lanes left = lane-1, right = lane+1.
+      const cpx sr = {wave_shr1(X.x), wave_shr1(X.y)};
+      const cpx sl = {wave_shl1(X.x), wave_shl1(X.y)};
+      const float term = welch_term(X, upper ? sl : sr, upper ? sr : sl);
+      if (!edge_lane) {
+        float* pp = part + own;
+        *pp = fresh ? term : *pp + term;
+      }
+      if constexpr (FL > 0) {
+        // get_bin scatter (fcfb.rs:164-172): bin own -> tile position.
+        const int e = fmap[own];
+        if (e >= 0) {
+          cpx v = X;
+          if (e & 1) v.y = -v.y;
+          const int pos = e >> 1;
+          const int tile = pos / FL, n = pos % FL;
+          reinterpret_cast<cpx*>(lds)[tile * FC::STRIDE + lds_pad(n)] = cscale(v, fw[pos]);
+        }
+      }
+      if constexpr ((q & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    });
+    if constexpr (FL > 0) {
+      if (t == 0) {  // the Nyquist bin can feed a channel too
+        const int e = fmap[M];
+        if (e >= 0) {
+          cpx v = etab[kEdgeNyq * R];
+          if (e & 1) v.y = -v.y;
+          const int pos = e >> 1;
+          reinterpret_cast<cpx*>(lds)[(pos / FL) * FC::STRIDE + lds_pad(pos % FL)] = cscale(v, fw[pos]);
+        }
       }
     }
+    __syncthreads();  // edge table (global, workgroup scope) and tile buffer complete
 
-    // Fused channel stage (fcfb.rs:154-188): gather each tile's bins from
-    // the spectrum row (get_bin), weight, inverse FFT through a per-tile LDS
-    // region, keep [FL/8, 7FL/8).
+    // Welch for the edge lanes: every neighbour from the edge table.
+    // Sources (wave, lane, slot) follow from the lane map: e.g. lower lane 0
+    // of wave v > 0 has left neighbour lane 31 of wave v-1; in wave 0 the
+    // k = 0 lane wraps to lane 33 (bin P3 - 1) in slot R - q, and so on.
+    if (edge_lane) {
+      auto E_ = [&](int v, int ln, int s) { return etab[(v * kEdgeLanes + edge_slot(ln)) * R + s]; };
+#pragma unroll 4
+      for (int q = 0; q < R; ++q) {
+        const cpx X = E_(wave, lane, q);
+        cpx L, Rn;
+        if (lane == 0) {
+          Rn = E_(wave, 1, q);
+          if (wave > 0) L = E_(wave - 1, 31, q);
+          else if (q > 0) L = E_(0, 33, R - q);
+          else L = cconj(E_(0, 1, 0));  // X[-1] = conj X[1]
+        } else if (lane == 31) {
+          L = E_(wave, 30, q);
+          Rn = (wave < W - 1) ? E_(wave + 1, 0, q) : E_(0, 32, R - 1 - q);
+        } else if (lane == 32) {
+          if (wave > 0) {
+            L = E_(wave, 33, q);
+            Rn = E_(wave - 1, 63, q);
+          } else {
+            L = E_(W - 1, 31, R - 1 - q);
+            Rn = E_(W - 1, 63, q);
+          }
+        } else if (lane == 63) {
+          L = (wave < W - 1) ? E_(wave + 1, 32, q) : E_(0, 32, q);
+          Rn = E_(wave, 62, q);
+        } else {  // wave 0, lane 33: bin P3 - 1 + P3 (R-1-q)
+          L = E_(0, 34, q);
+          Rn = (q > 0) ? E_(0, 0, R - q) : etab[kEdgeNyq * R];
+        }
+        const float term = welch_term(X, L, Rn);
+        const int own = kb + q * kstep;
+        float* pp = part + own;
+        *pp = fresh ? term : *pp + term;
+      }
+    }
+    if (t == 0) {  // bin M: left X[M-1] (wave 0 lane 33, slot 0), right conj X[M-1]
+      const cpx X = etab[kEdgeNyq * R];
+      const cpx L = etab[(0 * kEdgeLanes + edge_slot(33)) * R + 0];
+      const float term = welch_term(X, L, cconj(L));
+      part[M] = fresh ? term : part[M] + term;
+    }
+
+    // The next block's raw input is loaded while this block's channels run.
+    __builtin_amdgcn_sched_barrier(0);
+    if (PREFETCH && g + 1 < hi) load_block(g + 1, vin);
+    __builtin_amdgcn_sched_barrier(0);
+
+    // Fused channel stage (fcfb.rs:154-188): inverse FFT of each tile's
+    // gathered bins in its LDS region, keep [FL/8, 7FL/8).
     if constexpr (FL > 0) {
       constexpr int TF = FC::TF, RF = FC::RF;
       const int fc = t / TF, ftt = t % TF;
       const bool fact = fc < P.fnch;
-      const ChanDesc* fchans = P.fchans;
-      const float* fweights = P.weights;
-      asm volatile("" : "+s"(fchans), "+s"(fweights));
-      const ChanDesc fd = fchans[fact ? fc : 0];
+      float* tl = lds + fc * FC::STRIDE * 2;
       cpx gx[RF];
 #pragma unroll
-      for (int e = 0; e < RF; ++e) {
-        const int n = st_in_index<FL, RF, TF, FC::RF1>(ftt, e);
-        const int jj = (n + FL / 2) & (FL - 1);
-        int m = fd.f_mod + jj;
-        if (m >= P.N) m -= P.N;
-        const bool mirror = m >= P.K;
-        cpx v = spec[mirror ? P.N - m : m];
-        if (mirror) v.y = -v.y;
-        gx[e] = cscale(v, fweights[fd.w_off + jj]);
-      }
-      float* tl = lds + fc * FC::STRIDE * 2;
+      for (int e = 0; e < RF; ++e) gx[e] = reinterpret_cast<const cpx*>(tl)[st_in_addr<FL, RF, TF, FC::RF1>(ftt, e)];
       st_butterflies<FL, RF, TF, FC::RF1, 1, +1>(gx, ftt, nullptr);
       lds_exchange<RF, false>(
           gx, tl, [&](int e) { return st_out_addr<FL, RF, TF, FC::RF1, 1>(ftt, e); },
@@ -303,6 +395,7 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
       st_butterflies<FL, RF, TF, FC::RF2, FC::RF1, +1>(gx, ftt, ftw2);
       constexpr int PL = FL / FC::RF2;
       if (fact) {
+        const ChanDesc fd = P.fchans[fc];
         cpx* out = P.fout + fd.out_off + (long long)g * (3 * FL / 4);
 #pragma unroll
         for (int e = 0; e < RF; ++e) {

@@ -122,7 +122,8 @@ struct spk_ctx {
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
 
-  DevBuf hist, head, staging, spec, wpart, carry, welch_out, chan_out, weights;
+  DevBuf hist, head, staging, spec, wpart, carry, edge, welch_out, chan_out, weights;
+  DevBuf fmap, fw;           // fused channel stage: bin -> tile position map, tile weights
   DevBuf ftw2, ftw3, ftwu;
   long long spec_stride = 0, wstride = 0;
   int wslots = 0, welch_out_cap = 0;
@@ -223,6 +224,36 @@ spk_status rebuild_channels(spk_ctx* c) {
   // tile per channel fits in its workgroup.
   const auto it = c->groups.find(kFusedL);
   c->fused = it != c->groups.end() && (int)it->second.ids.size() * 16 <= c->M / c->fplan->R;
+  if (c->fused) {
+    // Each bin may feed at most one tile position (get_bin mirrors and
+    // overlapping channels that reuse a bin are left to the channel kernel).
+    const ChanGroup& g = it->second;
+    std::vector<int> map((size_t)c->K, -1);
+    std::vector<float> fw((size_t)g.ids.size() * kFusedL);
+    for (size_t tile = 0; tile < g.ids.size() && c->fused; ++tile) {
+      const Channel& ch = c->chans[g.ids[tile]];
+      long long fm = ch.first % c->N;
+      if (fm < 0) fm += c->N;
+      for (int n = 0; n < kFusedL; ++n) {
+        const int jj = (n + kFusedL / 2) % kFusedL;
+        long long m = fm + jj;
+        if (m >= c->N) m -= c->N;
+        const bool mirror = m >= c->K;
+        const long long bin = mirror ? c->N - m : m;
+        const int pos = (int)tile * kFusedL + n;
+        if (map[(size_t)bin] != -1) {
+          c->fused = false;
+          break;
+        }
+        map[(size_t)bin] = 2 * pos + (mirror ? 1 : 0);
+        fw[(size_t)pos] = ch.w[(size_t)jj];
+      }
+    }
+    if (c->fused) {
+      HIP_TRY(c, upload(c->fmap, map));
+      HIP_TRY(c, upload(c->fw, fw));
+    }
+  }
   c->chans_dirty = false;
   return SPK_OK;
 }
@@ -333,14 +364,14 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
   P.nblocks = nb;
   P.H = c->H;
   P.O = c->O;
-  P.N = c->N;
-  P.K = c->K;
+  P.edge = c->edge.as<cpx>();
   const ChanGroup* fg = nullptr;
   if (c->fused) {
     fg = &c->groups.at(kFusedL);
     P.fchans = fg->desc.as<spk::ChanDesc>();
     P.fnch = (int)fg->ids.size();
-    P.weights = c->weights.as<float>();
+    P.fmap = c->fmap.as<int>();
+    P.fw = c->fw.as<float>();
     P.fout = c->chan_out.as<cpx>();
     P.ftw2 = fg->tw2.as<cpx>();
   }
@@ -477,6 +508,8 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
       c->staging.alloc((size_t)c->F * c->H * c->bps) != hipSuccess ||
       c->spec.alloc((size_t)std::max(c->max_blocks, gcap) * c->spec_stride * sizeof(cpx)) != hipSuccess ||
       c->wpart.alloc((size_t)gcap * c->wslots * c->wstride * sizeof(float)) != hipSuccess ||
+      c->edge.alloc((size_t)gcap * (c->M / c->fplan->R / 64) * spk::kEdgeLanes * c->fplan->R * sizeof(cpx)) !=
+          hipSuccess ||
       c->carry.alloc((size_t)c->K * sizeof(float)) != hipSuccess ||
       c->welch_out.alloc((size_t)c->welch_out_cap * c->K * sizeof(float)) != hipSuccess)
     return cleanup(SPK_ERR_DEVICE);
