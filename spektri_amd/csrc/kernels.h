@@ -64,17 +64,20 @@ struct FwdParams {
   float* wpart;         // Welch partial sums [workgroup][slot][wstride]
   long long wstride;
   int wslots;
-  cpx* edge;            // edge tables [workgroup][wave][kEdgeLanes][R]
+  cpx* edump;           // per block: spectrum values of the edge lanes [block][wave][kEdgeLanes][R]
   int A;
   long long block0;     // stream index of this launch's block 0
   float wscale;
   int nblocks;
   int H, O;
-  // fused channel stage (FL > 0): every channel has IFFT size FL
+  // fused channel stage (FL > 0): every channel has IFFT size FL, lies in
+  // bins [0, K) and no two channels share a bin
   const ChanDesc* fchans;
   int fnch;
-  const int* fmap;      // per bin: 2 * (tile * FL + n) + conj, or -1
-  const float* fw;      // per tile position: the channel weight
+  const int* ffirst;    // first bin of each fused channel
+  const int* fcells;    // per 256-bin cell: up to two channel indices (lo | hi << 16, 0xFFFF = none)
+  int ncells;
+  const float* weights;
   cpx* fout;
   const cpx* ftw2;      // pass-2 table of the FL-point inverse transform
 };
@@ -108,6 +111,10 @@ struct FwdRealCfg {
   // After the main area: the pass-3 outputs of lanes 0 and 32 of wave 0
   // (2 R complex), which are their own mirrors.
   static constexpr int STASH_FLOATS = 4 * R;
+  // Fused channel lookup: cell table and first bins (ints).
+  static constexpr int MAX_CELLS = M / 256 + 2;
+  static constexpr int MAX_FCH = FL > 0 ? T / FusedCfg<FL>::TF : 0;
+  static constexpr int LOOKUP_INTS = FL > 0 ? MAX_CELLS + MAX_FCH : 0;
 };
 
 __device__ __forceinline__ float swap_hi_lo(float v, float& other) {
@@ -158,8 +165,15 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   constexpr int W = C::W;
   constexpr int P3 = C::P3;
   constexpr bool SPLIT = C::SPLIT;
-  __shared__ float lds[C::LDS_FLOATS + C::STASH_FLOATS];
+  __shared__ float lds[C::LDS_FLOATS + C::STASH_FLOATS + C::LOOKUP_INTS];
   cpx* stash = reinterpret_cast<cpx*>(lds + C::LDS_FLOATS);
+  int* lcells = reinterpret_cast<int*>(lds + C::LDS_FLOATS + C::STASH_FLOATS);
+  int* lfirst = lcells + C::MAX_CELLS;
+  if constexpr (FL > 0) {
+    for (int i = threadIdx.x; i < P.ncells; i += T) lcells[i] = P.fcells[i];
+    for (int i = threadIdx.x; i < P.fnch; i += T) lfirst[i] = P.ffirst[i];
+    // made visible by the first barrier of the block loop
+  }
 
   const int G = gridDim.x, wg = blockIdx.x;
   const int lo = (int)((long long)wg * P.nblocks / G);
@@ -247,20 +261,24 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     const int slot = (int)(gs / P.A - grp0);
     const bool fresh = (g == lo) || (gs % P.A == 0);
     float* part = P.wpart + ((long long)wg * P.wslots + slot) * P.wstride;
-    cpx* etab = P.edge + (long long)wg * (W * kEdgeLanes * R);
     const int es = edge_slot(lane);
-    // Lanes whose Welch neighbours are not both reachable by DPP.
+    // Lanes whose Welch neighbours are not both reachable by DPP; their terms
+    // are computed by welch_edge_kernel from the per-block edge dump.
     const bool edge_lane = lane == 0 || lane == 31 || lane == 32 || lane == 63 || (wave == 0 && lane == 33);
-    const int* fmap = P.fmap;
-    const float* fw = P.fw;
-    asm volatile("" : "+s"(fmap), "+s"(fw));
+    cpx* edump = P.edump + (long long)g * (W * kEdgeLanes * R);
 
-    // Untangle, Welch for interior lanes, edge publication, channel scatter.
     const cpx wb = twu[upper ? ksrc : pi];
     const float h = 0.5f * sg;
     cpx* spec = P.spec + (long long)g * P.spec_stride;
     const int kb = upper ? ksrc + P3 * (R - 1) : pi;
     const int kstep = upper ? -P3 : P3;
+    // Partial sums of this lane's bins, loaded before the untangle so the
+    // loads overlap it.
+    float pv[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) pv[q] = (fresh || edge_lane) ? 0.f : part[kb + q * kstep];
+
+    // Untangle, Welch for interior lanes, edge dump, channel scatter.
     static_for<0, R>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       cpx A, B;
@@ -283,109 +301,74 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
       const cpx X = cadd(E, cmul(wk, Od));
       const int own = kb + q * kstep;
       if (P.keep_spec) spec[own] = X;
-      if (es >= 0) etab[(wave * kEdgeLanes + es) * R + q] = X;
+      if (es >= 0) edump[(wave * kEdgeLanes + es) * R + q] = X;
       if constexpr (q == 0) {
         if (t == 0) {  // Nyquist bin M = E - O from Z[0]
           const cpx XN = csub(E, Od);
           if (P.keep_spec) spec[M] = XN;
-          etab[kEdgeNyq * R] = XN;
+          edump[kEdgeNyq * R] = XN;
+          if constexpr (FL > 0) {
+            // the Nyquist bin can feed a channel too
+            const int e = lcells[M >> 8];
+#pragma unroll
+            for (int h2 = 0; h2 < 2; ++h2) {
+              const int ch = (e >> (16 * h2)) & 0xFFFF;
+              if (ch != 0xFFFF) {
+                const int d = M - lfirst[ch];
+                if (d >= 0 && d < FL)
+                  reinterpret_cast<cpx*>(lds)[ch * FC::STRIDE + lds_pad((d + FL / 2) & (FL - 1))] = XN;
+              }
+            }
+          }
         }
       }
       // Welch neighbours by DPP: lower lanes left = lane-1, right = lane+1.
       const cpx sr = {wave_shr1(X.x), wave_shr1(X.y)};
       const cpx sl = {wave_shl1(X.x), wave_shl1(X.y)};
       const float term = welch_term(X, upper ? sl : sr, upper ? sr : sl);
-      if (!edge_lane) {
-        float* pp = part + own;
-        *pp = fresh ? term : *pp + term;
-      }
+      if (!edge_lane) part[own] = pv[q] + term;
       if constexpr (FL > 0) {
-        // get_bin scatter (fcfb.rs:164-172): bin own -> tile position.
-        const int e = fmap[own];
-        if (e >= 0) {
-          cpx v = X;
-          if (e & 1) v.y = -v.y;
-          const int pos = e >> 1;
-          const int tile = pos / FL, n = pos % FL;
-          reinterpret_cast<cpx*>(lds)[tile * FC::STRIDE + lds_pad(n)] = cscale(v, fw[pos]);
+        // Channel scatter (fcfb.rs:164-172): bin own = first + j feeds input
+        // position n = (j + FL/2) mod FL of its channel's tile.
+        const int e = lcells[own >> 8];
+#pragma unroll
+        for (int h2 = 0; h2 < 2; ++h2) {
+          const int ch = (e >> (16 * h2)) & 0xFFFF;
+          if (ch != 0xFFFF) {
+            const int d = own - lfirst[ch];
+            if (d >= 0 && d < FL)
+              reinterpret_cast<cpx*>(lds)[ch * FC::STRIDE + lds_pad((d + FL / 2) & (FL - 1))] = X;
+          }
         }
       }
       if constexpr ((q & 7) == 7) __builtin_amdgcn_sched_barrier(0);
     });
-    if constexpr (FL > 0) {
-      if (t == 0) {  // the Nyquist bin can feed a channel too
-        const int e = fmap[M];
-        if (e >= 0) {
-          cpx v = etab[kEdgeNyq * R];
-          if (e & 1) v.y = -v.y;
-          const int pos = e >> 1;
-          reinterpret_cast<cpx*>(lds)[(pos / FL) * FC::STRIDE + lds_pad(pos % FL)] = cscale(v, fw[pos]);
-        }
-      }
-    }
-    __syncthreads();  // edge table (global, workgroup scope) and tile buffer complete
-
-    // Welch for the edge lanes: every neighbour from the edge table.
-    // Sources (wave, lane, slot) follow from the lane map: e.g. lower lane 0
-    // of wave v > 0 has left neighbour lane 31 of wave v-1; in wave 0 the
-    // k = 0 lane wraps to lane 33 (bin P3 - 1) in slot R - q, and so on.
-    if (edge_lane) {
-      auto E_ = [&](int v, int ln, int s) { return etab[(v * kEdgeLanes + edge_slot(ln)) * R + s]; };
-#pragma unroll 4
-      for (int q = 0; q < R; ++q) {
-        const cpx X = E_(wave, lane, q);
-        cpx L, Rn;
-        if (lane == 0) {
-          Rn = E_(wave, 1, q);
-          if (wave > 0) L = E_(wave - 1, 31, q);
-          else if (q > 0) L = E_(0, 33, R - q);
-          else L = cconj(E_(0, 1, 0));  // X[-1] = conj X[1]
-        } else if (lane == 31) {
-          L = E_(wave, 30, q);
-          Rn = (wave < W - 1) ? E_(wave + 1, 0, q) : E_(0, 32, R - 1 - q);
-        } else if (lane == 32) {
-          if (wave > 0) {
-            L = E_(wave, 33, q);
-            Rn = E_(wave - 1, 63, q);
-          } else {
-            L = E_(W - 1, 31, R - 1 - q);
-            Rn = E_(W - 1, 63, q);
-          }
-        } else if (lane == 63) {
-          L = (wave < W - 1) ? E_(wave + 1, 32, q) : E_(0, 32, q);
-          Rn = E_(wave, 62, q);
-        } else {  // wave 0, lane 33: bin P3 - 1 + P3 (R-1-q)
-          L = E_(0, 34, q);
-          Rn = (q > 0) ? E_(0, 0, R - q) : etab[kEdgeNyq * R];
-        }
-        const float term = welch_term(X, L, Rn);
-        const int own = kb + q * kstep;
-        float* pp = part + own;
-        *pp = fresh ? term : *pp + term;
-      }
-    }
-    if (t == 0) {  // bin M: left X[M-1] (wave 0 lane 33, slot 0), right conj X[M-1]
-      const cpx X = etab[kEdgeNyq * R];
-      const cpx L = etab[(0 * kEdgeLanes + edge_slot(33)) * R + 0];
-      const float term = welch_term(X, L, cconj(L));
-      part[M] = fresh ? term : part[M] + term;
-    }
 
     // The next block's raw input is loaded while this block's channels run.
     __builtin_amdgcn_sched_barrier(0);
     if (PREFETCH && g + 1 < hi) load_block(g + 1, vin);
     __builtin_amdgcn_sched_barrier(0);
 
-    // Fused channel stage (fcfb.rs:154-188): inverse FFT of each tile's
-    // gathered bins in its LDS region, keep [FL/8, 7FL/8).
+    // Fused channel stage (fcfb.rs:154-188): weight the gathered bins,
+    // inverse FFT in the tile's LDS region, keep [FL/8, 7FL/8).
     if constexpr (FL > 0) {
       constexpr int TF = FC::TF, RF = FC::RF;
       const int fc = t / TF, ftt = t % TF;
       const bool fact = fc < P.fnch;
+      const ChanDesc fd = P.fchans[fact ? fc : 0];
+      const float* wts = P.weights + fd.w_off;
+      float wv[RF];
+#pragma unroll
+      for (int e = 0; e < RF; ++e) {
+        const int n = st_in_index<FL, RF, TF, FC::RF1>(ftt, e);
+        wv[e] = wts[(n + FL / 2) & (FL - 1)];
+      }
+      __syncthreads();  // tile buffers complete
       float* tl = lds + fc * FC::STRIDE * 2;
       cpx gx[RF];
 #pragma unroll
-      for (int e = 0; e < RF; ++e) gx[e] = reinterpret_cast<const cpx*>(tl)[st_in_addr<FL, RF, TF, FC::RF1>(ftt, e)];
+      for (int e = 0; e < RF; ++e)
+        gx[e] = cscale(reinterpret_cast<const cpx*>(tl)[st_in_addr<FL, RF, TF, FC::RF1>(ftt, e)], wv[e]);
       st_butterflies<FL, RF, TF, FC::RF1, 1, +1>(gx, ftt, nullptr);
       lds_exchange<RF, false>(
           gx, tl, [&](int e) { return st_out_addr<FL, RF, TF, FC::RF1, 1>(ftt, e); },
@@ -395,7 +378,6 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
       st_butterflies<FL, RF, TF, FC::RF2, FC::RF1, +1>(gx, ftt, ftw2);
       constexpr int PL = FL / FC::RF2;
       if (fact) {
-        const ChanDesc fd = P.fchans[fc];
         cpx* out = P.fout + fd.out_off + (long long)g * (3 * FL / 4);
 #pragma unroll
         for (int e = 0; e < RF; ++e) {
@@ -405,6 +387,114 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
         }
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Welch terms of the edge lanes (spektri/src/dsp/spectrum.rs:95-111): the
+// bins whose neighbours live in another wave or wrap across slots, read from
+// the per-block edge dump.  One thread per (workgroup, slot, edge bin) sums
+// the terms of the workgroup's blocks in that Welch group and writes the
+// partial row entry the forward kernel left out.  Items: lanes 0, 31, 32, 63
+// of every wave and slot, wave 0's lane 33, and the Nyquist bin.
+// ---------------------------------------------------------------------------
+struct EdgeParams {
+  const cpx* edump;
+  float* wpart;
+  long long wstride;
+  int wslots;
+  int G, nblocks, A;
+  long long block0;
+  int W, R, P3, M;
+};
+
+__global__ void welch_edge_kernel(EdgeParams E) {
+  const int items = E.W * 4 * E.R + E.R + 1;
+  const long long total = (long long)E.G * E.wslots * items;
+  const int stride = E.W * kEdgeLanes * E.R;  // cpx per block
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int item = (int)(i % items);
+    const int sl = (int)((i / items) % E.wslots);
+    const int wg = (int)(i / ((long long)items * E.wslots));
+    const int lo = (int)((long long)wg * E.nblocks / E.G);
+    const int hi = (int)((long long)(wg + 1) * E.nblocks / E.G);
+    const long long grp = (E.block0 + lo) / E.A + sl;
+    long long b0 = grp * E.A - E.block0, b1 = (grp + 1) * E.A - E.block0;
+    if (b0 < lo) b0 = lo;
+    if (b1 > hi) b1 = hi;
+    if (b0 >= b1) continue;
+    // (wave, lane, slot) of the item, its own bin and neighbour sources
+    int v, ln, q;
+    if (item < E.W * 4 * E.R) {
+      v = item / (4 * E.R);
+      const int which = (item / E.R) % 4;
+      q = item % E.R;
+      ln = which == 0 ? 0 : (which == 1 ? 31 : (which == 2 ? 32 : 63));
+    } else if (item < E.W * 4 * E.R + E.R) {
+      v = 0;
+      ln = 33;
+      q = item - E.W * 4 * E.R;
+    } else {
+      v = 0;
+      ln = -1;  // Nyquist
+      q = 0;
+    }
+    auto src = [&](int sv, int sln, int sq) { return (sv * kEdgeLanes + edge_slot(sln)) * E.R + sq; };
+    int io, il, ir, own;
+    bool conj_l = false, conj_r = false, nyq_r = false;
+    const int W = E.W, R = E.R, P3 = E.P3;
+    if (ln < 0) {
+      io = kEdgeNyq * R;
+      il = src(0, 33, 0);
+      ir = il;
+      conj_r = true;
+      own = E.M;
+    } else {
+      io = src(v, ln, q);
+      const int pi = 32 * v + (ln & 31);
+      const bool upper = ln >= 32;
+      const int ksrc = upper ? (pi == 0 ? P3 / 2 : P3 - pi) : pi;
+      own = upper ? ksrc + P3 * (R - 1 - q) : pi + P3 * q;
+      if (ln == 0) {
+        ir = src(v, 1, q);
+        if (v > 0) il = src(v - 1, 31, q);
+        else if (q > 0) il = src(0, 33, R - q);
+        else {
+          il = src(0, 1, 0);
+          conj_l = true;  // X[-1] = conj X[1]
+        }
+      } else if (ln == 31) {
+        il = src(v, 30, q);
+        ir = (v < W - 1) ? src(v + 1, 0, q) : src(0, 32, R - 1 - q);
+      } else if (ln == 32) {
+        if (v > 0) {
+          il = src(v, 33, q);
+          ir = src(v - 1, 63, q);
+        } else {
+          il = src(W - 1, 31, R - 1 - q);
+          ir = src(W - 1, 63, q);
+        }
+      } else if (ln == 63) {
+        il = (v < W - 1) ? src(v + 1, 32, q) : src(0, 32, q);
+        ir = src(v, 62, q);
+      } else {  // wave 0, lane 33: bin P3 - 1 + P3 (R-1-q)
+        il = src(0, 34, q);
+        if (q > 0) ir = src(0, 0, R - q);
+        else nyq_r = true;
+      }
+    }
+    float sum = 0.f;
+    for (long long b = b0; b < b1; ++b) {
+      const cpx* d = E.edump + b * stride;
+      const cpx X = d[io];
+      cpx L = d[il];
+      cpx Rn = nyq_r ? d[kEdgeNyq * R] : d[ir];
+      if (conj_l) L.y = -L.y;
+      if (conj_r) Rn.y = -Rn.y;
+      sum += welch_term(X, L, Rn);
+    }
+    E.wpart[((long long)wg * E.wslots + sl) * E.wstride + own] = sum;
   }
 }
 

@@ -122,8 +122,9 @@ struct spk_ctx {
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
 
-  DevBuf hist, head, staging, spec, wpart, carry, edge, welch_out, chan_out, weights;
-  DevBuf fmap, fw;           // fused channel stage: bin -> tile position map, tile weights
+  DevBuf hist, head, staging, spec, wpart, carry, edump, welch_out, chan_out, weights;
+  DevBuf fcells, ffirst;     // fused channel stage: 256-bin cell table, first bins
+  int ncells = 0;
   DevBuf ftw2, ftw3, ftwu;
   long long spec_stride = 0, wstride = 0;
   int wslots = 0, welch_out_cap = 0;
@@ -225,33 +226,41 @@ spk_status rebuild_channels(spk_ctx* c) {
   const auto it = c->groups.find(kFusedL);
   c->fused = it != c->groups.end() && (int)it->second.ids.size() * 16 <= c->M / c->fplan->R;
   if (c->fused) {
-    // Each bin may feed at most one tile position (get_bin mirrors and
-    // overlapping channels that reuse a bin are left to the channel kernel).
+    // The fused stage takes channels inside bins [0, K) with no bin shared
+    // (get_bin mirrors and overlapping channels go to the channel kernel).
+    // A 256-bin cell then meets at most two channels.
     const ChanGroup& g = it->second;
-    std::vector<int> map((size_t)c->K, -1);
-    std::vector<float> fw((size_t)g.ids.size() * kFusedL);
+    const int ncells = c->K / 256 + 1;
+    std::vector<int> cells((size_t)ncells, -1);  // packed lo | hi << 16, 0xFFFF = none
+    std::vector<int> firsts;
+    std::vector<int> owner((size_t)c->K, -1);
     for (size_t tile = 0; tile < g.ids.size() && c->fused; ++tile) {
       const Channel& ch = c->chans[g.ids[tile]];
-      long long fm = ch.first % c->N;
-      if (fm < 0) fm += c->N;
-      for (int n = 0; n < kFusedL; ++n) {
-        const int jj = (n + kFusedL / 2) % kFusedL;
-        long long m = fm + jj;
-        if (m >= c->N) m -= c->N;
-        const bool mirror = m >= c->K;
-        const long long bin = mirror ? c->N - m : m;
-        const int pos = (int)tile * kFusedL + n;
-        if (map[(size_t)bin] != -1) {
+      if (ch.first < 0 || ch.first + kFusedL > c->K) {
+        c->fused = false;
+        break;
+      }
+      for (int j = 0; j < kFusedL; ++j) {
+        int& o = owner[(size_t)(ch.first + j)];
+        if (o != -1) {
           c->fused = false;
           break;
         }
-        map[(size_t)bin] = 2 * pos + (mirror ? 1 : 0);
-        fw[(size_t)pos] = ch.w[(size_t)jj];
+        o = (int)tile;
+      }
+      firsts.push_back((int)ch.first);
+      for (long long cell = ch.first >> 8; cell <= (ch.first + kFusedL - 1) >> 8; ++cell) {
+        int& e = cells[(size_t)cell];
+        const int lo = e & 0xFFFF;
+        if (e == -1) e = (int)(tile | (0xFFFFu << 16));
+        else if ((e >> 16 & 0xFFFF) == 0xFFFF && lo != 0xFFFF) e = (int)(lo | (tile << 16));
+        else c->fused = false;
       }
     }
     if (c->fused) {
-      HIP_TRY(c, upload(c->fmap, map));
-      HIP_TRY(c, upload(c->fw, fw));
+      HIP_TRY(c, upload(c->fcells, cells));
+      HIP_TRY(c, upload(c->ffirst, firsts));
+      c->ncells = ncells;
     }
   }
   c->chans_dirty = false;
@@ -364,14 +373,16 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
   P.nblocks = nb;
   P.H = c->H;
   P.O = c->O;
-  P.edge = c->edge.as<cpx>();
+  P.edump = c->edump.as<cpx>();
   const ChanGroup* fg = nullptr;
   if (c->fused) {
     fg = &c->groups.at(kFusedL);
     P.fchans = fg->desc.as<spk::ChanDesc>();
     P.fnch = (int)fg->ids.size();
-    P.fmap = c->fmap.as<int>();
-    P.fw = c->fw.as<float>();
+    P.ffirst = c->ffirst.as<int>();
+    P.fcells = c->fcells.as<int>();
+    P.ncells = c->ncells;
+    P.weights = c->weights.as<float>();
     P.fout = c->chan_out.as<cpx>();
     P.ftw2 = fg->tw2.as<cpx>();
   }
@@ -379,6 +390,25 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
   const int G = std::min(nb, c->gmax[c->fused ? 1 : 0]);
   if (ev) HIP_TRY(c, hipEventRecord(ev[0], s));
   HIP_TRY(c, fv->launch(P, G, s));
+  {
+    spk::EdgeParams E{};
+    E.edump = c->edump.as<cpx>();
+    E.wpart = c->wpart.as<float>();
+    E.wstride = c->wstride;
+    E.wslots = c->wslots;
+    E.G = G;
+    E.nblocks = nb;
+    E.A = c->A;
+    E.block0 = c->blocks_done;
+    E.R = c->fplan->R;
+    E.P3 = c->fplan->R1 * c->fplan->R2;
+    E.W = E.P3 / 64;
+    E.M = c->M;
+    const long long total = (long long)G * c->wslots * (E.W * 4 * E.R + E.R + 1);
+    const unsigned grid = (unsigned)std::min<long long>((total + 255) / 256, 16384);
+    hipLaunchKernelGGL(spk::welch_edge_kernel, dim3(grid), dim3(256), 0, s, E);
+    HIP_TRY(c, hipGetLastError());
+  }
   if (ev) HIP_TRY(c, hipEventRecord(ev[1], s));
 
   for (auto& kv : c->groups) {
@@ -508,8 +538,8 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
       c->staging.alloc((size_t)c->F * c->H * c->bps) != hipSuccess ||
       c->spec.alloc((size_t)std::max(c->max_blocks, gcap) * c->spec_stride * sizeof(cpx)) != hipSuccess ||
       c->wpart.alloc((size_t)gcap * c->wslots * c->wstride * sizeof(float)) != hipSuccess ||
-      c->edge.alloc((size_t)gcap * (c->M / c->fplan->R / 64) * spk::kEdgeLanes * c->fplan->R * sizeof(cpx)) !=
-          hipSuccess ||
+      c->edump.alloc((size_t)c->max_blocks * (c->M / c->fplan->R / 64) * spk::kEdgeLanes * c->fplan->R *
+                     sizeof(cpx)) != hipSuccess ||
       c->carry.alloc((size_t)c->K * sizeof(float)) != hipSuccess ||
       c->welch_out.alloc((size_t)c->welch_out_cap * c->K * sizeof(float)) != hipSuccess)
     return cleanup(SPK_ERR_DEVICE);
