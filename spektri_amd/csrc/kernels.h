@@ -274,9 +274,14 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     const int kstep = upper ? -P3 : P3;
     // Partial sums of this lane's bins, loaded before the untangle so the
     // loads overlap it.
+    // The row was written by this workgroup's previous block: read it at
+    // agent scope so the load bypasses the CU's L1 and sees those stores.
     float pv[R];
 #pragma unroll
-    for (int q = 0; q < R; ++q) pv[q] = (fresh || edge_lane) ? 0.f : part[kb + q * kstep];
+    for (int q = 0; q < R; ++q)
+      pv[q] = (fresh || edge_lane)
+                  ? 0.f
+                  : __hip_atomic_load(part + kb + q * kstep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
     // Untangle, Welch for interior lanes, edge dump, channel scatter.
     static_for<0, R>([&](auto qc) {

@@ -223,7 +223,18 @@ def test_rx888_full_size_sampled_blocks():
         errs[f"channels_g{g0}"] = e
     same = all(np.array_equal(st.channel_output(a), st2.channel_output(b)) for a, b in zip(ids, ids2))
     errs["bench_config_vs_kept_welch"] = rms(st2.spectrum_records(), recs_g)
-    report("rx888_64k", blocks=nb, bench_config_bitwise_equal_channels=bool(same), **errs)
+    # the same launch twice must give bitwise the same Welch records
+    st2.reset()
+    st2.process_device(dev.data_ptr(), wl.nsamples)
+    st2.synchronize()
+    rec_a = st2.spectrum_records()
+    st2.reset()
+    st2.process_device(dev.data_ptr(), wl.nsamples)
+    st2.synchronize()
+    deterministic = bool(np.array_equal(rec_a, st2.spectrum_records()))
+    report("rx888_64k", blocks=nb, bench_config_bitwise_equal_channels=bool(same),
+           welch_deterministic=deterministic, **errs)
     assert max(errs.values()) < TOL, errs
+    assert deterministic
     st.close()
     st2.close()
