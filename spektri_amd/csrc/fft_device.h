@@ -164,7 +164,7 @@ __device__ __forceinline__ void dft(cpx (&x)[R]) {
 
 // Padded LDS index: one pad slot per 32 elements breaks the power-of-two
 // strides of the Stockham write pattern across LDS banks.
-__device__ __forceinline__ int lds_pad(int n) { return n + (n >> 5); }
+__host__ __device__ __forceinline__ int lds_pad(int n) { return n + (n >> 5); }
 __host__ __device__ constexpr int lds_padded_size(int n) { return n + (n >> 5) + 1; }
 
 }  // namespace spk

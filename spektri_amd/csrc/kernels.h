@@ -35,6 +35,14 @@ struct ChanDesc {
 // table (global scratch, one region per workgroup, L2-resident): the wave
 // edges and their inner neighbours, plus wave 0's lanes 33/34 and the
 // Nyquist bin.  Index -1 = not published.
+// Word 3 of a raw (stride 0) buffer resource on gfx9-family GPUs: 32-bit
+// data format, accesses past num_records are dropped (stores) or read 0.
+constexpr int kBufferRsrcWord3 = 0x00020000;
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+// Cache-policy bit of a buffer access that makes a load miss the CU's L1
+// (sc1, the bit agent-scope atomic loads use).
+constexpr int kCpolSc1 = 16;
+
 constexpr int kEdgeLanes = 10;
 __device__ __forceinline__ int edge_slot(int lane) {
   switch (lane) {
@@ -61,7 +69,7 @@ struct FwdParams {
   const cpx* tw2;       // pass-2 table, forward sign
   const cpx* tw3;       // pass-3 table, forward sign
   const cpx* twu;       // exp(-2 pi i k / N), k = 0..M/R
-  float* wpart;         // Welch partial sums [workgroup][slot][wstride]
+  float* wpart;         // Welch partial sums [workgroup][slot][wstride], lane-major rows (welch_row_index)
   long long wstride;
   int wslots;
   cpx* edump;           // per block: spectrum values of the edge lanes [block][wave][kEdgeLanes][R]
@@ -74,9 +82,8 @@ struct FwdParams {
   // bins [0, K) and no two channels share a bin
   const ChanDesc* fchans;
   int fnch;
-  const int* ffirst;    // first bin of each fused channel
-  const int* fcells;    // per 256-bin cell: up to two channel indices (lo | hi << 16, 0xFFFF = none)
-  int ncells;
+  const uint4* fscat;   // per (slot group of 8, lane): LDS cpx index of each slot's bin (u16 x 8)
+  int fscat_nyq;        // LDS cpx index of the Nyquist bin
   const float* weights;
   cpx* fout;
   const cpx* ftw2;      // pass-2 table of the FL-point inverse transform
@@ -103,6 +110,7 @@ struct FwdRealCfg {
   static constexpr int P3 = R1 * R2;
   static_assert(R1 * R2 * R == M, "radix plan must cover M");
   static_assert(P3 == T, "last pass must have one butterfly per thread");
+  static_assert(R % 8 == 0, "slots are handled in groups of 8");
   static_assert(T >= 64 && T <= 1024, "one wave minimum, 1024 threads maximum");
   static constexpr bool SPLIT = lds_padded_size(M) * 8 > kLdsBudgetBytes;
   static constexpr int PLANE_FLOATS = SPLIT ? lds_padded_size(M) : 2 * lds_padded_size(M);
@@ -111,11 +119,32 @@ struct FwdRealCfg {
   // After the main area: the pass-3 outputs of lanes 0 and 32 of wave 0
   // (2 R complex), which are their own mirrors.
   static constexpr int STASH_FLOATS = 4 * R;
-  // Fused channel lookup: cell table and first bins (ints).
-  static constexpr int MAX_CELLS = M / 256 + 2;
-  static constexpr int MAX_FCH = FL > 0 ? T / FusedCfg<FL>::TF : 0;
-  static constexpr int LOOKUP_INTS = FL > 0 ? MAX_CELLS + MAX_FCH : 0;
+  // One complex slot that bins outside every fused channel are scattered to.
+  static constexpr int DUMMY_FLOATS = 2;
+  static constexpr int LDS_DUMMY = (LDS_FLOATS + STASH_FLOATS) / 2;  // cpx index
+  static_assert(LDS_DUMMY < 65536, "scatter indices are 16-bit");
 };
+
+// Lane map of the last forward pass: lane t = 64 v + l of a T = P3 thread
+// workgroup holds, in slot q, bin pi + P3 q (l < 32, pi = 32 v + l) or bin
+// ksrc + P3 (R-1-q) (l >= 32, ksrc = P3 - pi, P3/2 for pi = 0).  Welch
+// partial rows are stored lane-major: bin k at q T + t, the Nyquist bin at M.
+__host__ __device__ inline int welch_row_index(int k, int M, int R) {
+  const int P3 = M / R;
+  if (k >= M) return M;
+  const int r = k & (P3 - 1), kq = k / P3;
+  int pi, q, hi;
+  if (r < P3 / 2) {
+    pi = r;
+    q = kq;
+    hi = 0;
+  } else {
+    pi = r == P3 / 2 ? 0 : P3 - r;
+    q = R - 1 - kq;
+    hi = 32;
+  }
+  return q * P3 + 64 * (pi >> 5) + hi + (pi & 31);
+}
 
 __device__ __forceinline__ float swap_hi_lo(float v, float& other) {
   // v_permlane32_swap: returns (A, B) with A = {v.lo | v.lo}, B = {v.hi | v.hi}
@@ -136,7 +165,22 @@ __device__ __forceinline__ float wave_shl1(float v) {
   return __int_as_float(
       __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130, 0xF, 0xF, false));
 }
+// Same shifts where lane 0 (resp. 63) may read anything: one v_mov_dpp.
+__device__ __forceinline__ float wave_shr1_any(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float wave_shl1_any(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xF, 0xF, true));
+}
 
+// base[idx] with a 32-bit byte offset, so the access can use a scalar base
+// register plus one VGPR offset instead of 64-bit address arithmetic.
+template <class T>
+__device__ __forceinline__ T& at32(T* base, unsigned idx) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * (unsigned)sizeof(T));
+}
+
+// Symmetric in the two neighbours.
 __device__ __forceinline__ float welch_term(cpx c0, cpx cm, cpx cp) {
   const float cr = c0.x - 0.5f * (cm.x + cp.x);
   const float ci = c0.y - 0.5f * (cm.y + cp.y);
@@ -165,15 +209,8 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   constexpr int W = C::W;
   constexpr int P3 = C::P3;
   constexpr bool SPLIT = C::SPLIT;
-  __shared__ float lds[C::LDS_FLOATS + C::STASH_FLOATS + C::LOOKUP_INTS];
+  __shared__ __align__(16) float lds[C::LDS_FLOATS + C::STASH_FLOATS + C::DUMMY_FLOATS];
   cpx* stash = reinterpret_cast<cpx*>(lds + C::LDS_FLOATS);
-  int* lcells = reinterpret_cast<int*>(lds + C::LDS_FLOATS + C::STASH_FLOATS);
-  int* lfirst = lcells + C::MAX_CELLS;
-  if constexpr (FL > 0) {
-    for (int i = threadIdx.x; i < P.ncells; i += T) lcells[i] = P.fcells[i];
-    for (int i = threadIdx.x; i < P.fnch; i += T) lfirst[i] = P.ffirst[i];
-    // made visible by the first barrier of the block loop
-  }
 
   const int G = gridDim.x, wg = blockIdx.x;
   const int lo = (int)((long long)wg * P.nblocks / G);
@@ -193,7 +230,8 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
       vin[e] = blk[T * u + j * (M / R1) + (int)threadIdx.x];
     }
   };
-  // 1024-thread workgroups have no registers to spare for the prefetch.
+  // 1024-thread workgroups have no registers to spare for the prefetch
+  // (the prefetched words would be spilled, which waits for the loads).
   constexpr bool PREFETCH = T < 1024;
   int vin[R];
   if (PREFETCH && lo < hi) load_block(lo, vin);
@@ -212,7 +250,7 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
 
     // Lane map of the last pass: lane l < 32 of wave v takes sub-transform
     // k = pi = 32 v + l; lane l + 32 takes its mirror P3 - pi (P3/2 for pi = 0).
-    const int wave = t >> 6, lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
     const bool upper = lane >= 32;
     const int pi = 32 * wave + (lane & 31);
     const int ksrc = upper ? (pi == 0 ? P3 / 2 : P3 - pi) : pi;
@@ -240,6 +278,33 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     lds_exchange<R, SPLIT>(
         x, lds, [&](int e) { return st_out_addr<M, R, T, R2, R1>(t, e); },
         [&](int e) { return lds_pad(ksrc) + pad_off(e * P3); });
+    // Welch partial row of this block's group (lane-major: slot q of lane t
+    // at q T + t), and the first slot group's partial sums and scatter
+    // indices, loaded ahead of the pass-3 DFT.  The row was written by this
+    // workgroup's previous block: it is read at agent scope, which bypasses
+    // the CU's L1 and so sees those stores.
+    const long long gs = P.block0 + g;
+    const int slot = (int)(gs / P.A - grp0);
+    const bool fresh = (g == lo) || (gs % P.A == 0);
+    const __amdgpu_buffer_rsrc_t prow = __builtin_amdgcn_make_buffer_rsrc(
+        P.wpart + ((long long)wg * P.wslots + slot) * P.wstride, (short)0, M * (int)sizeof(float), kBufferRsrcWord3);
+    const unsigned tvoff = (unsigned)t * (unsigned)sizeof(float);
+    float pv[R];
+    uint4 sw[2];
+    auto load_group = [&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      if (fresh) {
+#pragma unroll
+        for (int q = 8 * j; q < 8 * j + 8; ++q) pv[q] = 0.f;
+      } else {
+#pragma unroll
+        for (int q = 8 * j; q < 8 * j + 8; ++q)
+          pv[q] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(prow, tvoff, q * T * (int)sizeof(float), kCpolSc1));
+      }
+      if constexpr (FL > 0) sw[j & 1] = P.fscat[j * T + t];
+    };
+    load_group(std::integral_constant<int, 0>{});
+
 #pragma unroll
     for (int e = 0; e < R; ++e) x[e].y *= sg;
     apply_pass_twiddles<R, P3>(x, tw3, kt);
@@ -257,94 +322,80 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     }
     __syncthreads();  // every lane is done reading pass-3 inputs from LDS
 
-    const long long gs = P.block0 + g;
-    const int slot = (int)(gs / P.A - grp0);
-    const bool fresh = (g == lo) || (gs % P.A == 0);
-    float* part = P.wpart + ((long long)wg * P.wslots + slot) * P.wstride;
     const int es = edge_slot(lane);
-    // Lanes whose Welch neighbours are not both reachable by DPP; their terms
-    // are computed by welch_edge_kernel from the per-block edge dump.
-    const bool edge_lane = lane == 0 || lane == 31 || lane == 32 || lane == 63 || (wave == 0 && lane == 33);
-    cpx* edump = P.edump + (long long)g * (W * kEdgeLanes * R);
-
-    const cpx wb = twu[upper ? ksrc : pi];
-    const float h = 0.5f * sg;
+    // Edge-lane dump through a buffer resource: the other lanes store out of
+    // range, which the hardware drops.
+    const int ecount = W * kEdgeLanes * R;
+    const __amdgpu_buffer_rsrc_t edr = __builtin_amdgcn_make_buffer_rsrc(
+        P.edump + (long long)g * ecount, (short)0, ecount * (int)sizeof(cpx), kBufferRsrcWord3);
+    const unsigned evoff = es >= 0 ? (unsigned)((wave * kEdgeLanes + es) * R * (int)sizeof(cpx)) : 0x7ffff000u;
     cpx* spec = P.spec + (long long)g * P.spec_stride;
     const int kb = upper ? ksrc + P3 * (R - 1) : pi;
     const int kstep = upper ? -P3 : P3;
-    // Partial sums of this lane's bins, loaded before the untangle so the
-    // loads overlap it.
-    // The row was written by this workgroup's previous block: read it at
-    // agent scope so the load bypasses the CU's L1 and sees those stores.
-    float pv[R];
-#pragma unroll
-    for (int q = 0; q < R; ++q)
-      pv[q] = (fresh || edge_lane)
-                  ? 0.f
-                  : __hip_atomic_load(part + kb + q * kstep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // Both lanes of a mirror pair form E and O of the lower bin k from
+    // A = Z[k] (lower lane's slot) and B = Z[M-k] (upper lane's slot):
+    //   X[k] = E + W^k O,  X[M-k] = conj(E - W^k O),
+    // so with u = (s/2) W^k (s = +1 lower, -1 upper) every lane computes
+    //   X.x = (A.x+B.x)/2 + u.x (A.y+B.y) + u.y (A.x-B.x)
+    //   X.y = s ((A.y-B.y)/2 + u.y (A.y+B.y) - u.x (A.x-B.x)).
+    // W^k = W^kt (table) * W^(P3 q) (constant per slot).
+    const cpx wb = twu[kt];
+    const cpx wbs = {0.5f * sg * wb.x, 0.5f * sg * wb.y};
+    const bool lane0 = lane == 0, lane32 = lane == 32;
 
-    // Untangle, Welch for interior lanes, edge dump, channel scatter.
+    // Untangle; Welch terms into the partial row (the edge lanes' terms are
+    // wrong here and are overwritten by welch_edge_kernel); edge dump;
+    // channel scatter into the tile buffers.
     static_for<0, R>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
+      if constexpr (q % 8 == 0 && q + 8 < R) load_group(std::integral_constant<int, q / 8 + 1>{});
       cpx A, B;
       A.x = swap_hi_lo(x[q].x, B.x);
       A.y = swap_hi_lo(x[q].y, B.y);
       if (wave == 0) {
         // lane 0 (k = 0): mirror of Z[P3 q] is its own slot (R - q) mod R;
         // lane 32 (k = P3/2): mirror of its slot q is its own slot R-1-q.
-        if (lane == 0) B = stash[(R - q) & (R - 1)];
-        if (lane == 32) A = stash[R + R - 1 - q];
+        const cpx s0 = stash[(R - q) & (R - 1)];
+        const cpx s32 = stash[R + R - 1 - q];
+        B = lane0 ? s0 : B;
+        A = lane32 ? s32 : A;
       }
-      const cpx E = {0.5f * (A.x + B.x), h * (A.y - B.y)};
-      const cpx Od = {0.5f * (A.y + B.y), -h * (A.x - B.x)};
-      constexpr float cqx = float(ct_cos2pi(-q, 2 * R));
-      constexpr float cqy = float(ct_sin2pi(-q, 2 * R));
-      constexpr float crx = float(ct_cos2pi(-(R - 1 - q), 2 * R));
-      constexpr float cry = float(ct_sin2pi(-(R - 1 - q), 2 * R));
-      const cpx c = upper ? cpx{crx, cry} : cpx{cqx, cqy};
-      const cpx wk = cmul(wb, c);
-      const cpx X = cadd(E, cmul(wk, Od));
-      const int own = kb + q * kstep;
-      if (P.keep_spec) spec[own] = X;
-      if (es >= 0) edump[(wave * kEdgeLanes + es) * R + q] = X;
+      const float S1 = A.x + B.x, S2 = A.y + B.y, D1 = A.x - B.x, D2 = A.y - B.y;
+      constexpr float cx = float(ct_cos2pi(-q, 2 * R));
+      constexpr float cy = float(ct_sin2pi(-q, 2 * R));
+      const float ux = wbs.x * cx - wbs.y * cy;
+      const float uy = wbs.x * cy + wbs.y * cx;
+      cpx X;
+      X.x = 0.5f * S1 + (ux * S2 + uy * D1);
+      X.y = sg * (0.5f * D2 + (uy * S2 - ux * D1));
+      if (P.keep_spec) at32(spec, (unsigned)(kb + q * kstep)) = X;
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, X), edr, evoff + q * (int)sizeof(cpx), 0, 0);
       if constexpr (q == 0) {
-        if (t == 0) {  // Nyquist bin M = E - O from Z[0]
-          const cpx XN = csub(E, Od);
+        if (t == 0) {  // Nyquist bin M = E - O of Z[0]
+          const cpx XN = {0.5f * (S1 - S2), 0.5f * (D2 + D1)};
           if (P.keep_spec) spec[M] = XN;
-          edump[kEdgeNyq * R] = XN;
-          if constexpr (FL > 0) {
-            // the Nyquist bin can feed a channel too
-            const int e = lcells[M >> 8];
-#pragma unroll
-            for (int h2 = 0; h2 < 2; ++h2) {
-              const int ch = (e >> (16 * h2)) & 0xFFFF;
-              if (ch != 0xFFFF) {
-                const int d = M - lfirst[ch];
-                if (d >= 0 && d < FL)
-                  reinterpret_cast<cpx*>(lds)[ch * FC::STRIDE + lds_pad((d + FL / 2) & (FL - 1))] = XN;
-              }
-            }
-          }
+          P.edump[(long long)g * ecount + kEdgeNyq * R] = XN;
+          if constexpr (FL > 0) reinterpret_cast<cpx*>(lds)[P.fscat_nyq] = XN;
         }
       }
-      // Welch neighbours by DPP: lower lanes left = lane-1, right = lane+1.
-      const cpx sr = {wave_shr1(X.x), wave_shr1(X.y)};
-      const cpx sl = {wave_shl1(X.x), wave_shl1(X.y)};
-      const float term = welch_term(X, upper ? sl : sr, upper ? sr : sl);
-      if (!edge_lane) part[own] = pv[q] + term;
+      // Welch neighbours by DPP: lanes l-1 and l+1 (left/right for lower
+      // lanes, right/left for upper lanes; the term is symmetric).
+      // The term |X - (l + r)/2|^2 is formed as |l + r - 2 X|^2 / 4, which
+      // rounds identically (power-of-two scalings).
+      float dx = wave_shr1_any(X.x) + wave_shl1_any(X.x);
+      float dy = wave_shr1_any(X.y) + wave_shl1_any(X.y);
+      dx = __builtin_fmaf(-2.f, X.x, dx);
+      dy = __builtin_fmaf(-2.f, X.y, dy);
+      const float acc = __builtin_fmaf(0.25f, __builtin_fmaf(dy, dy, dx * dx), pv[q]);
+      __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(acc), prow, tvoff, q * T * (int)sizeof(float), 0);
       if constexpr (FL > 0) {
-        // Channel scatter (fcfb.rs:164-172): bin own = first + j feeds input
-        // position n = (j + FL/2) mod FL of its channel's tile.
-        const int e = lcells[own >> 8];
-#pragma unroll
-        for (int h2 = 0; h2 < 2; ++h2) {
-          const int ch = (e >> (16 * h2)) & 0xFFFF;
-          if (ch != 0xFFFF) {
-            const int d = own - lfirst[ch];
-            if (d >= 0 && d < FL)
-              reinterpret_cast<cpx*>(lds)[ch * FC::STRIDE + lds_pad((d + FL / 2) & (FL - 1))] = X;
-          }
-        }
+        // Channel scatter (fcfb.rs:164-172): bin first + j feeds input
+        // position (j + FL/2) mod FL of its channel's tile; the index table
+        // sends other bins to a dummy slot.
+        const uint4 w4 = sw[(q / 8) & 1];
+        const unsigned wd = (q & 7) < 2 ? w4.x : (q & 7) < 4 ? w4.y : (q & 7) < 6 ? w4.z : w4.w;
+        const unsigned li = (q & 1) ? (wd >> 16) : (wd & 0xFFFFu);
+        reinterpret_cast<cpx*>(lds)[li] = X;
       }
       if constexpr ((q & 7) == 7) __builtin_amdgcn_sched_barrier(0);
     });
@@ -446,7 +497,7 @@ __global__ void welch_edge_kernel(EdgeParams E) {
       q = 0;
     }
     auto src = [&](int sv, int sln, int sq) { return (sv * kEdgeLanes + edge_slot(sln)) * E.R + sq; };
-    int io, il, ir, own;
+    int io, il, ir;
     bool conj_l = false, conj_r = false, nyq_r = false;
     const int W = E.W, R = E.R, P3 = E.P3;
     if (ln < 0) {
@@ -454,13 +505,8 @@ __global__ void welch_edge_kernel(EdgeParams E) {
       il = src(0, 33, 0);
       ir = il;
       conj_r = true;
-      own = E.M;
     } else {
       io = src(v, ln, q);
-      const int pi = 32 * v + (ln & 31);
-      const bool upper = ln >= 32;
-      const int ksrc = upper ? (pi == 0 ? P3 / 2 : P3 - pi) : pi;
-      own = upper ? ksrc + P3 * (R - 1 - q) : pi + P3 * q;
       if (ln == 0) {
         ir = src(v, 1, q);
         if (v > 0) il = src(v - 1, 31, q);
@@ -499,7 +545,7 @@ __global__ void welch_edge_kernel(EdgeParams E) {
       if (conj_r) Rn.y = -Rn.y;
       sum += welch_term(X, L, Rn);
     }
-    E.wpart[((long long)wg * E.wslots + sl) * E.wstride + own] = sum;
+    E.wpart[((long long)wg * E.wslots + sl) * E.wstride + (ln < 0 ? E.M : q * P3 + 64 * v + ln)] = sum;
   }
 }
 
@@ -598,7 +644,7 @@ struct FinParams {
   int G;               // workgroups of the forward launch
   int nblocks;
   long long block0;
-  int A, K;
+  int A, K, M, R;
   long long s_first;
   int ngroups;         // groups touched
   float* carry;        // K floats
@@ -621,7 +667,7 @@ __global__ void welch_finalize_kernel(FinParams F) {
     for (int wg = w0; wg <= w1; ++wg) {
       const long long lo = (long long)wg * F.nblocks / F.G;
       const long long slot = s - (F.block0 + lo) / F.A;
-      sum += F.wpart[((long long)wg * F.wslots + slot) * F.wstride + k];
+      sum += F.wpart[((long long)wg * F.wslots + slot) * F.wstride + welch_row_index(k, F.M, F.R)];
     }
     if ((s + 1) * F.A <= F.block0 + F.nblocks) {
       F.out[(long long)gi * F.K + k] = sum / (float)F.A;

@@ -123,8 +123,8 @@ struct spk_ctx {
   hipStream_t stream = nullptr;
 
   DevBuf hist, head, staging, spec, wpart, carry, edump, welch_out, chan_out, weights;
-  DevBuf fcells, ffirst;     // fused channel stage: 256-bin cell table, first bins
-  int ncells = 0;
+  DevBuf fscat;              // fused channel stage: per-lane LDS scatter indices
+  int fscat_nyq = 0;
   DevBuf ftw2, ftw3, ftwu;
   long long spec_stride = 0, wstride = 0;
   int wslots = 0, welch_out_cap = 0;
@@ -186,6 +186,8 @@ std::vector<float> raised_cosine_weights(int L) {
   return w;
 }
 
+int fused_lds_dummy(int M);
+
 // Lay out channel outputs and descriptors, upload weights and tables.
 spk_status rebuild_channels(spk_ctx* c) {
   std::vector<float> wall;
@@ -228,12 +230,14 @@ spk_status rebuild_channels(spk_ctx* c) {
   if (c->fused) {
     // The fused stage takes channels inside bins [0, K) with no bin shared
     // (get_bin mirrors and overlapping channels go to the channel kernel).
-    // A 256-bin cell then meets at most two channels.
+    // Each lane's slot q bin is scattered to LDS index tab[q / 8][lane][q % 8]
+    // (its tile position, or the dummy slot).
     const ChanGroup& g = it->second;
-    const int ncells = c->K / 256 + 1;
-    std::vector<int> cells((size_t)ncells, -1);  // packed lo | hi << 16, 0xFFFF = none
-    std::vector<int> firsts;
+    const int M = c->M, R = c->fplan->R, T = M / R;
+    const int dummy = fused_lds_dummy(M);
+    std::vector<uint16_t> tab((size_t)M, (uint16_t)dummy);
     std::vector<int> owner((size_t)c->K, -1);
+    int nyq = dummy;
     for (size_t tile = 0; tile < g.ids.size() && c->fused; ++tile) {
       const Channel& ch = c->chans[g.ids[tile]];
       if (ch.first < 0 || ch.first + kFusedL > c->K) {
@@ -241,26 +245,25 @@ spk_status rebuild_channels(spk_ctx* c) {
         break;
       }
       for (int j = 0; j < kFusedL; ++j) {
-        int& o = owner[(size_t)(ch.first + j)];
-        if (o != -1) {
+        const int k = (int)ch.first + j;
+        if (owner[(size_t)k] != -1) {
           c->fused = false;
           break;
         }
-        o = (int)tile;
-      }
-      firsts.push_back((int)ch.first);
-      for (long long cell = ch.first >> 8; cell <= (ch.first + kFusedL - 1) >> 8; ++cell) {
-        int& e = cells[(size_t)cell];
-        const int lo = e & 0xFFFF;
-        if (e == -1) e = (int)(tile | (0xFFFFu << 16));
-        else if ((e >> 16 & 0xFFFF) == 0xFFFF && lo != 0xFFFF) e = (int)(lo | (tile << 16));
-        else c->fused = false;
+        owner[(size_t)k] = (int)tile;
+        const int li = (int)tile * spk::FusedCfg<kFusedL>::STRIDE + spk::lds_pad((j + kFusedL / 2) & (kFusedL - 1));
+        if (k == M) {
+          nyq = li;
+        } else {
+          const int idx = spk::welch_row_index(k, M, R);
+          const int q = idx / T, t = idx % T;
+          tab[((size_t)(q >> 3) * T + t) * 8 + (q & 7)] = (uint16_t)li;
+        }
       }
     }
     if (c->fused) {
-      HIP_TRY(c, upload(c->fcells, cells));
-      HIP_TRY(c, upload(c->ffirst, firsts));
-      c->ncells = ncells;
+      HIP_TRY(c, upload(c->fscat, tab));
+      c->fscat_nyq = nyq;
     }
   }
   c->chans_dirty = false;
@@ -286,9 +289,11 @@ struct FwdVariant {
   int M, R, FL;
   hipError_t (*launch)(const spk::FwdParams&, int, hipStream_t);
   int (*occupancy)();
+  int lds_dummy;  // LDS cpx index of the fused stage's dummy scatter slot
 };
 #define SPK_FWD_VARIANT(M, R, R1, R2, FL) \
-  {M, R, FL, &launch_fwd<M, R, R1, R2, FL>, &fwd_wgs_per_cu<M, R, R1, R2, FL>}
+  {M, R, FL, &launch_fwd<M, R, R1, R2, FL>, &fwd_wgs_per_cu<M, R, R1, R2, FL>, \
+   spk::FwdRealCfg<M, R, R1, R2, FL>::LDS_DUMMY}
 const FwdVariant kFwdVariants[] = {
     SPK_FWD_VARIANT(2048, 16, 8, 16, 0),    SPK_FWD_VARIANT(2048, 16, 8, 16, 256),
     SPK_FWD_VARIANT(4096, 16, 16, 16, 0),   SPK_FWD_VARIANT(4096, 16, 16, 16, 256),
@@ -300,6 +305,10 @@ const FwdVariant* find_fwd_variant(int M, int FL) {
   for (const auto& v : kFwdVariants)
     if (v.M == M && v.FL == FL) return &v;
   return nullptr;
+}
+int fused_lds_dummy(int M) {
+  const FwdVariant* v = find_fwd_variant(M, kFusedL);
+  return v ? v->lds_dummy : 0;
 }
 template <int L, int R, int R1, int R2, int R3>
 void launch_chan(const spk::ChanParams& P, hipStream_t s) {
@@ -379,9 +388,8 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
     fg = &c->groups.at(kFusedL);
     P.fchans = fg->desc.as<spk::ChanDesc>();
     P.fnch = (int)fg->ids.size();
-    P.ffirst = c->ffirst.as<int>();
-    P.fcells = c->fcells.as<int>();
-    P.ncells = c->ncells;
+    P.fscat = c->fscat.as<uint4>();
+    P.fscat_nyq = c->fscat_nyq;
     P.weights = c->weights.as<float>();
     P.fout = c->chan_out.as<cpx>();
     P.ftw2 = fg->tw2.as<cpx>();
@@ -445,6 +453,8 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
     Fp.block0 = c->blocks_done;
     Fp.A = c->A;
     Fp.K = c->K;
+    Fp.M = c->M;
+    Fp.R = c->fplan->R;
     Fp.s_first = s_first;
     Fp.ngroups = (int)(s_last - s_first + 1);
     Fp.carry = c->carry.as<float>();
