@@ -31,10 +31,6 @@ struct ChanDesc {
   int w_off;          // float offset of the channel's weights
 };
 
-// Lanes of every wave whose spectrum values are published to the edge
-// table (global scratch, one region per workgroup, L2-resident): the wave
-// edges and their inner neighbours, plus wave 0's lanes 33/34 and the
-// Nyquist bin.  Index -1 = not published.
 // Word 3 of a raw (stride 0) buffer resource on gfx9-family GPUs: 32-bit
 // data format, accesses past num_records are dropped (stores) or read 0.
 constexpr int kBufferRsrcWord3 = 0x00020000;
@@ -43,8 +39,12 @@ typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 // (sc1, the bit agent-scope atomic loads use).
 constexpr int kCpolSc1 = 16;
 
+// Lanes of every wave whose spectrum values are published to the edge
+// ring (global scratch, one region per workgroup, L2-resident): the wave
+// edges and their inner neighbours, plus wave 0's lanes 33/34 and the
+// Nyquist bin.  Index -1 = not published.
 constexpr int kEdgeLanes = 10;
-__device__ __forceinline__ int edge_slot(int lane) {
+__host__ __device__ inline int edge_slot(int lane) {
   switch (lane) {
     case 0: return 0;
     case 1: return 1;
@@ -62,7 +62,7 @@ constexpr int kEdgeNyq = 9;  // wave 0, slot 0: X[M]
 
 struct FwdParams {
   const int16_t* src;   // stream samples; sample s >= 0 at src[s]
-  const int16_t* head;  // block 0's N samples: O samples of history, then src[0 .. N-O)
+  const int16_t* hist;  // the O samples before src[0] (block 0 reads them, then src[0 .. N-O))
   cpx* spec;            // per-block spectrum rows (written when keep_spec)
   long long spec_stride;
   int keep_spec;
@@ -72,7 +72,8 @@ struct FwdParams {
   float* wpart;         // Welch partial sums [workgroup][slot][wstride], lane-major rows (welch_row_index)
   long long wstride;
   int wslots;
-  cpx* edump;           // per block: spectrum values of the edge lanes [block][wave][kEdgeLanes][R]
+  cpx* edump;           // per workgroup: spectrum values of the edge lanes [workgroup][wave][kEdgeLanes][R]
+  const int4* eitems;   // Welch items of the edge bins (edge_items)
   int A;
   long long block0;     // stream index of this launch's block 0
   float wscale;
@@ -111,6 +112,7 @@ struct FwdRealCfg {
   static_assert(R1 * R2 * R == M, "radix plan must cover M");
   static_assert(P3 == T, "last pass must have one butterfly per thread");
   static_assert(R % 8 == 0, "slots are handled in groups of 8");
+  static_assert((M / 4) % T == 0, "block 0's history seam must fall between points");
   static_assert(T >= 64 && T <= 1024, "one wave minimum, 1024 threads maximum");
   static constexpr bool SPLIT = lds_padded_size(M) * 8 > kLdsBudgetBytes;
   static constexpr int PLANE_FLOATS = SPLIT ? lds_padded_size(M) : 2 * lds_padded_size(M);
@@ -123,7 +125,78 @@ struct FwdRealCfg {
   static constexpr int DUMMY_FLOATS = 2;
   static constexpr int LDS_DUMMY = (LDS_FLOATS + STASH_FLOATS) / 2;  // cpx index
   static_assert(LDS_DUMMY < 65536, "scatter indices are 16-bit");
+  // Welch items whose neighbours are not reachable by DPP (edge_items).
+  static constexpr int EDGE_ITEMS = W * 4 * R + R + 1;
+  static constexpr int EDGE_PER_THREAD = (EDGE_ITEMS + T - 1) / T;
+  static constexpr int EDGE_RING = W * kEdgeLanes * R;  // cpx per workgroup
 };
+
+// The bins whose Welch neighbours lie outside their wave: per wave the
+// lanes 0, 31, 32, 63 of every slot, wave 0's lane 33, and the Nyquist bin.
+// Item i = {edge-ring index of the bin, of its left neighbour, of its right
+// neighbour, partial-row index | conj-left << 28 | conj-right << 29}, for
+// the edge ring layout [wave][edge_slot(lane)][slot] and the last-pass lane
+// map.  Written by the host into a table of W * 4 R + R + 1 entries.
+inline void edge_items(int M, int R, int4* out) {
+  const int P3 = M / R, W = P3 / 64;
+  auto src = [&](int sv, int sln, int sq) { return (sv * kEdgeLanes + edge_slot(sln)) * R + sq; };
+  const int n = W * 4 * R + R + 1;
+  for (int item = 0; item < n; ++item) {
+    int v, ln, q;
+    if (item < W * 4 * R) {
+      v = item / (4 * R);
+      const int which = (item / R) % 4;
+      q = item % R;
+      ln = which == 0 ? 0 : (which == 1 ? 31 : (which == 2 ? 32 : 63));
+    } else if (item < W * 4 * R + R) {
+      v = 0;
+      ln = 33;
+      q = item - W * 4 * R;
+    } else {
+      v = 0;
+      ln = -1;
+      q = 0;
+    }
+    int io, il, ir, ridx, fl = 0;
+    if (ln < 0) {  // Nyquist: right neighbour X[M+1] = conj X[M-1]
+      io = kEdgeNyq * R;
+      il = src(0, 33, 0);
+      ir = il;
+      fl = 2;
+      ridx = M;
+    } else {
+      io = src(v, ln, q);
+      ridx = q * P3 + 64 * v + ln;
+      if (ln == 0) {
+        ir = src(v, 1, q);
+        if (v > 0) il = src(v - 1, 31, q);
+        else if (q > 0) il = src(0, 33, R - q);
+        else {
+          il = src(0, 1, 0);
+          fl = 1;  // X[-1] = conj X[1]
+        }
+      } else if (ln == 31) {
+        il = src(v, 30, q);
+        ir = (v < W - 1) ? src(v + 1, 0, q) : src(0, 32, R - 1 - q);
+      } else if (ln == 32) {
+        if (v > 0) {
+          il = src(v, 33, q);
+          ir = src(v - 1, 63, q);
+        } else {
+          il = src(W - 1, 31, R - 1 - q);
+          ir = src(W - 1, 63, q);
+        }
+      } else if (ln == 63) {
+        il = (v < W - 1) ? src(v + 1, 32, q) : src(0, 32, q);
+        ir = src(v, 62, q);
+      } else {  // wave 0, lane 33: bin P3 - 1 + P3 (R-1-q)
+        il = src(0, 34, q);
+        ir = q > 0 ? src(0, 0, R - q) : kEdgeNyq * R;
+      }
+    }
+    out[item] = make_int4(io, il, ir, ridx | (fl << 28));
+  }
+}
 
 // Lane map of the last forward pass: lane t = 64 v + l of a T = P3 thread
 // workgroup holds, in slot q, bin pi + P3 q (l < 32, pi = 32 v + l) or bin
@@ -209,9 +282,11 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   constexpr int W = C::W;
   constexpr int P3 = C::P3;
   constexpr bool SPLIT = C::SPLIT;
-  __shared__ __align__(16) float lds[C::LDS_FLOATS + C::STASH_FLOATS + C::DUMMY_FLOATS];
+  __shared__ __align__(16) float lds[C::LDS_FLOATS + C::STASH_FLOATS + C::DUMMY_FLOATS + C::EDGE_PER_THREAD * T];
   cpx* stash = reinterpret_cast<cpx*>(lds + C::LDS_FLOATS);
+  float* eacc = lds + C::LDS_FLOATS + C::STASH_FLOATS + C::DUMMY_FLOATS;  // edge item sums
 
+  constexpr int NI = C::EDGE_PER_THREAD;
   const int G = gridDim.x, wg = blockIdx.x;
   const int lo = (int)((long long)wg * P.nblocks / G);
   const int hi = (int)((long long)(wg + 1) * P.nblocks / G);
@@ -223,11 +298,23 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   // row pointer) + lane offset.  The next block's pairs are loaded while the
   // current block's channel stage runs.
   auto load_block = [&](int g, int (&vin)[R]) {
-    const int* blk = reinterpret_cast<const int*>(g == 0 ? P.head : P.src + ((long long)g * P.H - P.O));
+    // Block 0 starts with the O = M/2 history samples (M/4 pairs).  Every
+    // point's pair range [T u + j M/R1, + T) lies wholly on one side of that
+    // seam (M/4 is a multiple of T), so each point reads from one of two
+    // uniform row pointers, chosen at compile time.
+    uintptr_t lo_row, hi_row;
+    if (g == 0) {
+      lo_row = reinterpret_cast<uintptr_t>(P.hist);
+      hi_row = reinterpret_cast<uintptr_t>(P.src) - (uintptr_t)(M / 4) * sizeof(int);
+    } else {
+      lo_row = hi_row = reinterpret_cast<uintptr_t>(P.src + ((long long)g * P.H - P.O));
+    }
 #pragma unroll
     for (int e = 0; e < R; ++e) {
       const int u = e / R1, j = e % R1;
-      vin[e] = blk[T * u + j * (M / R1) + (int)threadIdx.x];
+      const int base = T * u + j * (M / R1);
+      const int* row = reinterpret_cast<const int*>(base < M / 4 ? lo_row : hi_row);
+      vin[e] = row[base + (int)threadIdx.x];
     }
   };
   // 1024-thread workgroups have no registers to spare for the prefetch
@@ -235,6 +322,7 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   constexpr bool PREFETCH = T < 1024;
   int vin[R];
   if (PREFETCH && lo < hi) load_block(lo, vin);
+
 
   for (int g = lo; g < hi; ++g) {
     if constexpr (!PREFETCH) load_block(g, vin);
@@ -287,7 +375,11 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     const int slot = (int)(gs / P.A - grp0);
     const bool fresh = (g == lo) || (gs % P.A == 0);
     const __amdgpu_buffer_rsrc_t prow = __builtin_amdgcn_make_buffer_rsrc(
-        P.wpart + ((long long)wg * P.wslots + slot) * P.wstride, (short)0, M * (int)sizeof(float), kBufferRsrcWord3);
+        P.wpart + ((long long)wg * P.wslots + slot) * P.wstride, (short)0, (M + 1) * (int)sizeof(float),
+        kBufferRsrcWord3);
+    // Lanes whose Welch neighbours are not both reachable by DPP accumulate
+    // wrong terms in their partial-row entries here; the edge items
+    // overwrite those entries at the end of each Welch group.
     const unsigned tvoff = (unsigned)t * (unsigned)sizeof(float);
     float pv[R];
     uint4 sw[2];
@@ -323,12 +415,13 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     __syncthreads();  // every lane is done reading pass-3 inputs from LDS
 
     const int es = edge_slot(lane);
-    // Edge-lane dump through a buffer resource: the other lanes store out of
-    // range, which the hardware drops.
-    const int ecount = W * kEdgeLanes * R;
-    const __amdgpu_buffer_rsrc_t edr = __builtin_amdgcn_make_buffer_rsrc(
-        P.edump + (long long)g * ecount, (short)0, ecount * (int)sizeof(cpx), kBufferRsrcWord3);
-    const unsigned evoff = es >= 0 ? (unsigned)((wave * kEdgeLanes + es) * R * (int)sizeof(cpx)) : 0x7ffff000u;
+    // Edge-lane dump into this workgroup's edge ring through a buffer
+    // resource: the other lanes store out of range, which is dropped.
+    constexpr int ecount = C::EDGE_RING;
+    cpx* ering = P.edump + (long long)wg * ecount;
+    const __amdgpu_buffer_rsrc_t edr =
+        __builtin_amdgcn_make_buffer_rsrc(ering, (short)0, ecount * (int)sizeof(cpx), kBufferRsrcWord3);
+    const unsigned evoff = (unsigned)((wave * kEdgeLanes + (es < 0 ? 0 : es)) * R * (int)sizeof(cpx));
     cpx* spec = P.spec + (long long)g * P.spec_stride;
     const int kb = upper ? ksrc + P3 * (R - 1) : pi;
     const int kstep = upper ? -P3 : P3;
@@ -343,9 +436,8 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     const cpx wbs = {0.5f * sg * wb.x, 0.5f * sg * wb.y};
     const bool lane0 = lane == 0, lane32 = lane == 32;
 
-    // Untangle; Welch terms into the partial row (the edge lanes' terms are
-    // wrong here and are overwritten by welch_edge_kernel); edge dump;
-    // channel scatter into the tile buffers.
+    // Untangle; Welch terms into the partial row (not for edge lanes); edge
+    // dump; channel scatter into the tile buffers.
     static_for<0, R>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       if constexpr (q % 8 == 0 && q + 8 < R) load_group(std::integral_constant<int, q / 8 + 1>{});
@@ -369,12 +461,13 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
       X.x = 0.5f * S1 + (ux * S2 + uy * D1);
       X.y = sg * (0.5f * D2 + (uy * S2 - ux * D1));
       if (P.keep_spec) at32(spec, (unsigned)(kb + q * kstep)) = X;
-      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, X), edr, evoff + q * (int)sizeof(cpx), 0, 0);
+      if (es >= 0)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, X), edr, evoff + q * (int)sizeof(cpx), 0, 0);
       if constexpr (q == 0) {
         if (t == 0) {  // Nyquist bin M = E - O of Z[0]
           const cpx XN = {0.5f * (S1 - S2), 0.5f * (D2 + D1)};
           if (P.keep_spec) spec[M] = XN;
-          P.edump[(long long)g * ecount + kEdgeNyq * R] = XN;
+          ering[kEdgeNyq * R] = XN;
           if constexpr (FL > 0) reinterpret_cast<cpx*>(lds)[P.fscat_nyq] = XN;
         }
       }
@@ -405,12 +498,60 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     if (PREFETCH && g + 1 < hi) load_block(g + 1, vin);
     __builtin_amdgcn_sched_barrier(0);
 
+    // Edge items: Welch terms of the bins whose neighbours sit in other
+    // waves, from the edge ring after a barrier.  Each item's thread keeps
+    // the item's sum in its LDS slot and writes it over the item's
+    // partial-row entry when the Welch group ends.  The item table is read
+    // before the barrier, the ring after it, and the terms are formed after
+    // the fused channel stage so that the loads overlap it.
+    const bool slot_end = (g + 1 == hi) || ((gs + 1) % P.A == 0);
+    int4 eit[NI];
+    u32x2 ex[NI], el[NI], er[NI];
+    // Threads past the last item repeat the last item's loads and keep
+    // their (unused) sums in spare LDS slots; only their row store is skipped.
+    auto item_ok = [&](int i) { return i * T + T <= C::EDGE_ITEMS || t + i * T < C::EDGE_ITEMS; };
+    auto edge_prep = [&]() {
+#pragma unroll
+      for (int i = 0; i < NI; ++i) eit[i] = P.eitems[item_ok(i) ? t + i * T : C::EDGE_ITEMS - 1];
+    };
+    auto edge_issue = [&]() {
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        ex[i] = __builtin_amdgcn_raw_buffer_load_b64(edr, (unsigned)eit[i].x * (unsigned)sizeof(cpx), 0, kCpolSc1);
+        el[i] = __builtin_amdgcn_raw_buffer_load_b64(edr, (unsigned)eit[i].y * (unsigned)sizeof(cpx), 0, kCpolSc1);
+        er[i] = __builtin_amdgcn_raw_buffer_load_b64(edr, (unsigned)eit[i].z * (unsigned)sizeof(cpx), 0, kCpolSc1);
+      }
+    };
+    auto edge_finish = [&]() {
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        const int fl = eit[i].w >> 28;
+        const cpx X = __builtin_bit_cast(cpx, ex[i]);
+        cpx L = __builtin_bit_cast(cpx, el[i]), Rn = __builtin_bit_cast(cpx, er[i]);
+        if (fl & 1) L.y = -L.y;
+        if (fl & 2) Rn.y = -Rn.y;
+        const float term = welch_term(X, L, Rn);
+        const float acc = fresh ? term : eacc[t + i * T] + term;
+        eacc[t + i * T] = acc;
+        if (slot_end && item_ok(i))
+          __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(acc), prow,
+                                                (unsigned)(eit[i].w & 0x0FFFFFFF) * (unsigned)sizeof(float), 0, 0);
+      }
+    };
+    if constexpr (FL == 0) {
+      edge_prep();
+      __syncthreads();  // edge ring complete
+      edge_issue();
+      edge_finish();
+    }
+
     // Fused channel stage (fcfb.rs:154-188): weight the gathered bins,
     // inverse FFT in the tile's LDS region, keep [FL/8, 7FL/8).
     if constexpr (FL > 0) {
       constexpr int TF = FC::TF, RF = FC::RF;
       const int fc = t / TF, ftt = t % TF;
       const bool fact = fc < P.fnch;
+      edge_prep();
       const ChanDesc fd = P.fchans[fact ? fc : 0];
       const float* wts = P.weights + fd.w_off;
       float wv[RF];
@@ -419,7 +560,8 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
         const int n = st_in_index<FL, RF, TF, FC::RF1>(ftt, e);
         wv[e] = wts[(n + FL / 2) & (FL - 1)];
       }
-      __syncthreads();  // tile buffers complete
+      __syncthreads();  // tile buffers and edge ring complete
+      edge_issue();
       float* tl = lds + fc * FC::STRIDE * 2;
       cpx gx[RF];
 #pragma unroll
@@ -442,6 +584,7 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
           if (idx >= FL / 8 && idx < (FL / 8) * 7) out[idx - FL / 8] = gx[e];
         }
       }
+      edge_finish();
     }
   }
 }
@@ -454,101 +597,6 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
 // partial row entry the forward kernel left out.  Items: lanes 0, 31, 32, 63
 // of every wave and slot, wave 0's lane 33, and the Nyquist bin.
 // ---------------------------------------------------------------------------
-struct EdgeParams {
-  const cpx* edump;
-  float* wpart;
-  long long wstride;
-  int wslots;
-  int G, nblocks, A;
-  long long block0;
-  int W, R, P3, M;
-};
-
-__global__ void welch_edge_kernel(EdgeParams E) {
-  const int items = E.W * 4 * E.R + E.R + 1;
-  const long long total = (long long)E.G * E.wslots * items;
-  const int stride = E.W * kEdgeLanes * E.R;  // cpx per block
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (long long)gridDim.x * blockDim.x) {
-    const int item = (int)(i % items);
-    const int sl = (int)((i / items) % E.wslots);
-    const int wg = (int)(i / ((long long)items * E.wslots));
-    const int lo = (int)((long long)wg * E.nblocks / E.G);
-    const int hi = (int)((long long)(wg + 1) * E.nblocks / E.G);
-    const long long grp = (E.block0 + lo) / E.A + sl;
-    long long b0 = grp * E.A - E.block0, b1 = (grp + 1) * E.A - E.block0;
-    if (b0 < lo) b0 = lo;
-    if (b1 > hi) b1 = hi;
-    if (b0 >= b1) continue;
-    // (wave, lane, slot) of the item, its own bin and neighbour sources
-    int v, ln, q;
-    if (item < E.W * 4 * E.R) {
-      v = item / (4 * E.R);
-      const int which = (item / E.R) % 4;
-      q = item % E.R;
-      ln = which == 0 ? 0 : (which == 1 ? 31 : (which == 2 ? 32 : 63));
-    } else if (item < E.W * 4 * E.R + E.R) {
-      v = 0;
-      ln = 33;
-      q = item - E.W * 4 * E.R;
-    } else {
-      v = 0;
-      ln = -1;  // Nyquist
-      q = 0;
-    }
-    auto src = [&](int sv, int sln, int sq) { return (sv * kEdgeLanes + edge_slot(sln)) * E.R + sq; };
-    int io, il, ir;
-    bool conj_l = false, conj_r = false, nyq_r = false;
-    const int W = E.W, R = E.R, P3 = E.P3;
-    if (ln < 0) {
-      io = kEdgeNyq * R;
-      il = src(0, 33, 0);
-      ir = il;
-      conj_r = true;
-    } else {
-      io = src(v, ln, q);
-      if (ln == 0) {
-        ir = src(v, 1, q);
-        if (v > 0) il = src(v - 1, 31, q);
-        else if (q > 0) il = src(0, 33, R - q);
-        else {
-          il = src(0, 1, 0);
-          conj_l = true;  // X[-1] = conj X[1]
-        }
-      } else if (ln == 31) {
-        il = src(v, 30, q);
-        ir = (v < W - 1) ? src(v + 1, 0, q) : src(0, 32, R - 1 - q);
-      } else if (ln == 32) {
-        if (v > 0) {
-          il = src(v, 33, q);
-          ir = src(v - 1, 63, q);
-        } else {
-          il = src(W - 1, 31, R - 1 - q);
-          ir = src(W - 1, 63, q);
-        }
-      } else if (ln == 63) {
-        il = (v < W - 1) ? src(v + 1, 32, q) : src(0, 32, q);
-        ir = src(v, 62, q);
-      } else {  // wave 0, lane 33: bin P3 - 1 + P3 (R-1-q)
-        il = src(0, 34, q);
-        if (q > 0) ir = src(0, 0, R - q);
-        else nyq_r = true;
-      }
-    }
-    float sum = 0.f;
-    for (long long b = b0; b < b1; ++b) {
-      const cpx* d = E.edump + b * stride;
-      const cpx X = d[io];
-      cpx L = d[il];
-      cpx Rn = nyq_r ? d[kEdgeNyq * R] : d[ir];
-      if (conj_l) L.y = -L.y;
-      if (conj_r) Rn.y = -Rn.y;
-      sum += welch_term(X, L, Rn);
-    }
-    E.wpart[((long long)wg * E.wslots + sl) * E.wstride + (ln < 0 ? E.M : q * P3 + 64 * v + ln)] = sum;
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Channel stage, one launch per IFFT size L (spektri/src/dsp/fcfb.rs:154-188):
 //   B[i] = W[j] get_bin(first + j), j = (i + L/2) mod L,
@@ -643,37 +691,46 @@ struct FinParams {
   int wslots;
   int G;               // workgroups of the forward launch
   int nblocks;
-  long long block0;
+  int blk0_mod;        // stream index of the launch's first block mod A
   int A, K, M, R;
-  long long s_first;
-  int ngroups;         // groups touched
   float* carry;        // K floats
   float* out;          // records [r][K]
+  // input history for the next call: the last hist_words 32-bit words of
+  // this call's input
+  const int* hist_src;
+  int* hist_dst;
+  int hist_words;
 };
 
+// Grid: x over bins, y over the Welch groups the launch touches (group 0 is
+// the group of the launch's first block).  Row y = 0 also saves the input
+// history for the next call.
 __global__ void welch_finalize_kernel(FinParams F) {
-  const long long total = (long long)F.ngroups * F.K;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (long long)gridDim.x * blockDim.x) {
-    const int gi = (int)(i / F.K), k = (int)(i % F.K);
-    const long long s = F.s_first + gi;
-    const long long b_lo = s * F.A > F.block0 ? s * F.A - F.block0 : 0;
-    long long b_hi = (s + 1) * F.A - F.block0;
-    if (b_hi > F.nblocks) b_hi = F.nblocks;
-    float sum = (gi == 0 && F.block0 % F.A != 0) ? F.carry[k] : 0.f;
-    // workgroup of block b: the w with floor(w nb / G) <= b < floor((w+1) nb / G)
-    const int w0 = (int)(((b_lo + 1) * F.G + F.nblocks - 1) / F.nblocks) - 1;
-    const int w1 = (int)((b_hi * F.G + F.nblocks - 1) / F.nblocks) - 1;
-    for (int wg = w0; wg <= w1; ++wg) {
-      const long long lo = (long long)wg * F.nblocks / F.G;
-      const long long slot = s - (F.block0 + lo) / F.A;
-      sum += F.wpart[((long long)wg * F.wslots + slot) * F.wstride + welch_row_index(k, F.M, F.R)];
-    }
-    if ((s + 1) * F.A <= F.block0 + F.nblocks) {
-      F.out[(long long)gi * F.K + k] = sum / (float)F.A;
-    } else {
-      F.carry[k] = sum;
-    }
+  const int gi = (int)blockIdx.y;
+  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (gi == 0) {
+    for (int i = k; i < F.hist_words; i += (int)(gridDim.x * blockDim.x)) F.hist_dst[i] = F.hist_src[i];
+  }
+  if (k >= F.K) return;
+  const int ridx = welch_row_index(k, F.M, F.R);
+  const int g_lo = gi * F.A - F.blk0_mod, g_hi = g_lo + F.A;
+  const int b_lo = g_lo > 0 ? g_lo : 0;
+  const int b_hi = g_hi < F.nblocks ? g_hi : F.nblocks;
+  float sum = (gi == 0 && F.blk0_mod != 0) ? F.carry[k] : 0.f;
+  // workgroup of block b: the w with floor(w nb / G) <= b < floor((w+1) nb / G)
+  const unsigned nb = (unsigned)F.nblocks, G = (unsigned)F.G;
+  const int w0 = (int)(((unsigned)(b_lo + 1) * G + nb - 1) / nb) - 1;
+  const int w1 = (int)(((unsigned)b_hi * G + nb - 1) / nb) - 1;
+#pragma unroll 4
+  for (int wg = w0; wg <= w1; ++wg) {
+    const int lo = (int)((unsigned)wg * nb / G);
+    const int slot = gi - (F.blk0_mod + lo) / F.A;
+    sum += F.wpart[((long long)wg * F.wslots + slot) * F.wstride + ridx];
+  }
+  if (g_hi <= F.nblocks) {
+    F.out[(long long)gi * F.K + k] = sum / (float)F.A;
+  } else {
+    F.carry[k] = sum;
   }
 }
 

@@ -122,10 +122,11 @@ struct spk_ctx {
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
 
-  DevBuf hist, head, staging, spec, wpart, carry, edump, welch_out, chan_out, weights;
+  DevBuf hist, staging, spec, wpart, carry, edump, welch_out, chan_out, weights;
   DevBuf fscat;              // fused channel stage: per-lane LDS scatter indices
   int fscat_nyq = 0;
   DevBuf ftw2, ftw3, ftwu;
+  DevBuf eitems;             // Welch items of the edge bins (spk::edge_items)
   long long spec_stride = 0, wstride = 0;
   int wslots = 0, welch_out_cap = 0;
   int gmax[2] = {0, 0};      // persistent workgroups of the unfused / fused forward kernel
@@ -357,13 +358,9 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
     c->ev_used += 4;
   }
 
-  // Block 0 reads [history | first N - O new samples] from the head copy.
-  HIP_TRY(c, hipMemcpyAsync(c->head.p, c->hist.p, (size_t)c->O * c->bps, hipMemcpyDeviceToDevice, s));
-  HIP_TRY(c, hipMemcpyAsync(c->head.as<char>() + (size_t)c->O * c->bps, src, (size_t)(c->N - c->O) * c->bps,
-                            hipMemcpyDeviceToDevice, s));
   spk::FwdParams P{};
   P.src = reinterpret_cast<const int16_t*>(src);
-  P.head = c->head.as<int16_t>();
+  P.hist = c->hist.as<int16_t>();
   P.spec = c->spec.as<cpx>();
   P.spec_stride = c->spec_stride;
   // Per-block spectrum rows are needed when kept or read by a channel
@@ -383,6 +380,7 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
   P.H = c->H;
   P.O = c->O;
   P.edump = c->edump.as<cpx>();
+  P.eitems = c->eitems.as<int4>();
   const ChanGroup* fg = nullptr;
   if (c->fused) {
     fg = &c->groups.at(kFusedL);
@@ -398,25 +396,7 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
   const int G = std::min(nb, c->gmax[c->fused ? 1 : 0]);
   if (ev) HIP_TRY(c, hipEventRecord(ev[0], s));
   HIP_TRY(c, fv->launch(P, G, s));
-  {
-    spk::EdgeParams E{};
-    E.edump = c->edump.as<cpx>();
-    E.wpart = c->wpart.as<float>();
-    E.wstride = c->wstride;
-    E.wslots = c->wslots;
-    E.G = G;
-    E.nblocks = nb;
-    E.A = c->A;
-    E.block0 = c->blocks_done;
-    E.R = c->fplan->R;
-    E.P3 = c->fplan->R1 * c->fplan->R2;
-    E.W = E.P3 / 64;
-    E.M = c->M;
-    const long long total = (long long)G * c->wslots * (E.W * 4 * E.R + E.R + 1);
-    const unsigned grid = (unsigned)std::min<long long>((total + 255) / 256, 16384);
-    hipLaunchKernelGGL(spk::welch_edge_kernel, dim3(grid), dim3(256), 0, s, E);
-    HIP_TRY(c, hipGetLastError());
-  }
+  const int blk0_mod = (int)(c->blocks_done % c->A);
   if (ev) HIP_TRY(c, hipEventRecord(ev[1], s));
 
   for (auto& kv : c->groups) {
@@ -450,25 +430,24 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
     Fp.wslots = c->wslots;
     Fp.G = G;
     Fp.nblocks = nb;
-    Fp.block0 = c->blocks_done;
+    Fp.blk0_mod = blk0_mod;
     Fp.A = c->A;
     Fp.K = c->K;
     Fp.M = c->M;
     Fp.R = c->fplan->R;
-    Fp.s_first = s_first;
-    Fp.ngroups = (int)(s_last - s_first + 1);
     Fp.carry = c->carry.as<float>();
     Fp.out = c->welch_out.as<float>();
-    const long long total = (long long)Fp.ngroups * c->K;
-    const unsigned grid = (unsigned)std::min<long long>((total + 255) / 256, 8192);
-    hipLaunchKernelGGL(spk::welch_finalize_kernel, dim3(grid), dim3(256), 0, s, Fp);
+    // New history: the last O samples consumed.
+    Fp.hist_src = reinterpret_cast<const int*>(reinterpret_cast<const char*>(src) +
+                                               ((long long)nb * c->H - c->O) * c->bps);
+    Fp.hist_dst = c->hist.as<int>();
+    Fp.hist_words = c->O * c->bps / 4;
+    const int ngroups = (int)(s_last - s_first + 1);
+    hipLaunchKernelGGL(spk::welch_finalize_kernel, dim3((c->K + 255) / 256, ngroups), dim3(256), 0, s, Fp);
     HIP_TRY(c, hipGetLastError());
   }
   if (ev) HIP_TRY(c, hipEventRecord(ev[3], s));
 
-  // New history: the last O samples consumed.
-  const char* hsrc = reinterpret_cast<const char*>(src) + ((long long)nb * c->H - c->O) * c->bps;
-  HIP_TRY(c, hipMemcpyAsync(c->hist.p, hsrc, (size_t)c->O * c->bps, hipMemcpyDeviceToDevice, s));
 
   c->blocks_done += nb;
   c->last_blocks = (uint32_t)nb;
@@ -538,18 +517,20 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   }
   const int gcap = std::max(c->gmax[0], c->gmax[1]);
   const int gmin = std::min(c->gmax[0], c->gmax[1]);
+  // The Welch kernels index blocks times workgroups in 32 bits.
+  if ((long long)(c->max_blocks + c->A) * (gcap + 1) >= (1LL << 31)) return cleanup(SPK_ERR_UNSUPPORTED);
   // A workgroup spans at most ceil(ceil(max_blocks / G) / A) + 1 Welch groups.
   const int bpw = (c->max_blocks + gmin - 1) / gmin;
   c->wslots = (bpw + c->A - 1) / c->A + 1;
   c->spec_stride = ((long long)c->K + 63) / 64 * 64;
   c->wstride = c->spec_stride;
   c->welch_out_cap = c->max_blocks / c->A + 1;
-  if (c->hist.alloc((size_t)c->O * c->bps) != hipSuccess || c->head.alloc((size_t)c->N * c->bps) != hipSuccess ||
+  if (c->hist.alloc((size_t)c->O * c->bps) != hipSuccess ||
       c->staging.alloc((size_t)c->F * c->H * c->bps) != hipSuccess ||
       c->spec.alloc((size_t)std::max(c->max_blocks, gcap) * c->spec_stride * sizeof(cpx)) != hipSuccess ||
       c->wpart.alloc((size_t)gcap * c->wslots * c->wstride * sizeof(float)) != hipSuccess ||
-      c->edump.alloc((size_t)c->max_blocks * (c->M / c->fplan->R / 64) * spk::kEdgeLanes * c->fplan->R *
-                     sizeof(cpx)) != hipSuccess ||
+      c->edump.alloc((size_t)gcap * (c->M / c->fplan->R / 64) * spk::kEdgeLanes * c->fplan->R * sizeof(cpx)) !=
+          hipSuccess ||
       c->carry.alloc((size_t)c->K * sizeof(float)) != hipSuccess ||
       c->welch_out.alloc((size_t)c->welch_out_cap * c->K * sizeof(float)) != hipSuccess)
     return cleanup(SPK_ERR_DEVICE);
@@ -562,6 +543,9 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
     const double a = 2.0 * M_PI * (double)k / (double)c->N;
     twu[k] = cpx{(float)std::cos(a), (float)(-std::sin(a))};
   }
+  std::vector<int4> items((size_t)(c->M / p.R / 64) * 4 * p.R + p.R + 1);
+  spk::edge_items(c->M, p.R, items.data());
+  if (upload(c->eitems, items) != hipSuccess) return cleanup(SPK_ERR_DEVICE);
   if (upload(c->ftw2, pass_table(p.R2, p.R1, -1)) != hipSuccess ||
       upload(c->ftw3, pass_table(p.R, p.R1 * p.R2, -1)) != hipSuccess || upload(c->ftwu, twu) != hipSuccess)
     return cleanup(SPK_ERR_DEVICE);
