@@ -11,6 +11,17 @@ Prints one JSON line with the metric, a ``roofline`` object for the dominant
 kernel (algorithmic bytes / its average event-timed duration, against the
 8 TB/s HBM peak), a ``path`` object for the whole path, and a
 ``cpu_baseline`` measured on this host's cores on a bounded sample.
+
+Algorithmic bytes per block follow the two-stage model of the path: new
+input H * 2 B, spectrum write K * 8 B, channel gather sum(L) * 8 B, channel
+output sum(3L/4) * 8 B and Welch output K * 4 / A B.  When every channel runs
+inside the forward kernel that kernel is the whole path and is charged all of
+it; otherwise the forward kernel is charged input + spectrum + Welch and the
+channel kernels gather + output.
+
+``--copy-probe-mib`` appends a plain device copy of that size (three times)
+after the timed steps: under ``rocprofv3 --pmc FETCH_SIZE`` / ``WRITE_SIZE``
+it calibrates the counters against a known byte count.
 """
 from __future__ import annotations
 
@@ -32,6 +43,8 @@ def parse():
     ap.add_argument("--workload", default="rx888_64k")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU work for the baseline sample (0 disables it)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after timing, copy this many MiB device-to-device three times")
     return ap.parse_args()
 
 
@@ -44,8 +57,8 @@ def traffic_from_profiles(workload: str):
         return None
     try:
         d = json.load(open(p))
-        return d.get("fwd_bytes_per_launch")
-    except Exception:
+        return d.get("forward_bytes_per_launch")
+    except (OSError, ValueError):
         return None
 
 
@@ -104,22 +117,28 @@ def main():
     kern = st.last_timing()
     nb = st.last_blocks()
     assert nb == wl.nblocks
+    fused = st.fused_channels()
 
     H, K, A = wl.hop, wl.bins, wl.averages
-    fwd_bytes = nb * (H * 2 + K * 8 + K * 4 / A)
     chan_bytes = nb * sum(L * 8 + (3 * L // 4) * 8 for _, L, _ in wl.channels)
     path_bytes = nb * wl.bytes_per_block()
+    all_fused = fused == len(wl.channels)
+    fwd_bytes = path_bytes if all_fused else nb * (H * 2 + K * 8 + K * 4 / A)
     value = nb * H / (ms_step * 1e-3) / 1e6
-    fwd_gbps = fwd_bytes / (kern["fwd_welch_ms"] * 1e-3) / 1e9
-    chan_gbps = chan_bytes / (kern["channels_ms"] * 1e-3) / 1e9
+    fwd_gbps = fwd_bytes / (kern["forward_ms"] * 1e-3) / 1e9
+    chan_ms = kern["unfused_channels_ms"]
+    chan_gbps = chan_bytes / (chan_ms * 1e-3) / 1e9 if not all_fused and chan_ms > 0 else None
     path_gbps = path_bytes / (ms_step * 1e-3) / 1e9
     b_per_sample = wl.bytes_per_block() / H
     roofline_msps = HBM_PEAK_GBPS * 1e9 / b_per_sample / 1e6
     traffic = traffic_from_profiles(args.workload)
-    dominant = "fwd" if kern["fwd_welch_ms"] >= kern["channels_ms"] else "chan"
+    dominant = "fwd" if all_fused or kern["forward_ms"] >= chan_ms else "chan"
     roof = {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBPS,
-            "kernel": f"fwd_real_kernel<M={wl.fft_size // 2}>" if dominant == "fwd" else "chan_kernel<L=256>",
+            "kernel": (f"fwd_real_kernel<M={wl.fft_size // 2}, FL={256 if all_fused else 0}>" if dominant == "fwd"
+                       else "chan_kernel"),
             "achieved": round(fwd_gbps if dominant == "fwd" else chan_gbps, 1),
+            "algorithmic_bytes_per_launch": int(fwd_bytes if dominant == "fwd" else chan_bytes),
+            "avg_launch_ms": round(kern["forward_ms"] if dominant == "fwd" else chan_ms, 5),
             "traffic": traffic if dominant == "fwd" else None}
     roof["frac"] = round(roof["achieved"] / HBM_PEAK_GBPS, 4)
 
@@ -134,9 +153,18 @@ def main():
                  "frac": round(path_gbps / HBM_PEAK_GBPS, 4), "bytes_per_input_sample": round(b_per_sample, 3),
                  "roofline_MSps": round(roofline_msps, 1)},
         "kernels_ms": {k: round(v, 4) for k, v in kern.items()},
-        "kernels_GBps": {"fwd_welch": round(fwd_gbps, 1), "channels": round(chan_gbps, 1)},
+        "fused_channels": fused,
     }
     st.close()
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = n * 4
+        del a, b
     if args.cpu_seconds > 0:
         x_host = x.cpu().numpy()
         del x

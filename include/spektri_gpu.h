@@ -127,10 +127,18 @@ spk_status spk_synchronize(spk_ctx* ctx);
 
 /* Per-kernel timing from HIP events recorded on the context's stream around
  * each launch: average ms per call since spk_enable_timing of ms[0] the
- * forward FFT + Welch kernel, ms[1] all channel launches, ms[2] the Welch
- * finaliser.  Enabling adds events only; spk_last_timing synchronizes. */
+ * forward kernel (FFT, Welch partial sums and the fused channels), ms[1] the
+ * channel launches of the channels not fused, ms[2] the Welch finaliser.
+ * Enabling adds events only; spk_last_timing synchronizes.  No counterpart
+ * in the reference (measurement hook). */
 spk_status spk_enable_timing(spk_ctx* ctx, int on);
 spk_status spk_last_timing(spk_ctx* ctx, float ms[3]);
+
+/* Number of channels whose gather, weighting, IFFT and discard run inside
+ * the forward kernel (all channels of IFFT size 256 when they lie in
+ * [0, K), do not overlap and fit the workgroup; 0 otherwise).  No
+ * counterpart in the reference (plan introspection). */
+spk_status spk_fused_channels(spk_ctx* ctx, uint32_t* n);
 
 /* freq_to_bins / bins_to_freq / freq_to_bins_exact (spektri/src/dsp/fcfb.rs:225-285).
  * Return SPK_ERR_INVALID when the reference returns None. */

@@ -63,6 +63,7 @@ _SIGS = {
     "spk_synchronize": (ctypes.c_int, [_P]),
     "spk_enable_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "spk_last_timing": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float)]),
+    "spk_fused_channels": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32)]),
     "spk_freq_to_bins": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_uint32,
                                         ctypes.c_double, ctypes.c_double,
                                         ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int64)]),

@@ -703,6 +703,16 @@ spk_status spk_enable_timing(spk_ctx* c, int on) {
   return SPK_OK;
 }
 
+spk_status spk_fused_channels(spk_ctx* c, uint32_t* n) {
+  if (!c || !n) return SPK_ERR_INVALID;
+  if (c->chans_dirty) {
+    spk_status st = rebuild_channels(c);
+    if (st != SPK_OK) return st;
+  }
+  *n = c->fused ? (uint32_t)c->groups.at(kFusedL).ids.size() : 0u;
+  return SPK_OK;
+}
+
 spk_status spk_last_timing(spk_ctx* c, float ms[3]) {
   if (!c || !ms) return SPK_ERR_INVALID;
   HIP_TRY(c, hipStreamSynchronize(c->stream));

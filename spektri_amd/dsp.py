@@ -169,7 +169,13 @@ class DspState:
     def last_timing(self):
         ms = (ctypes.c_float * 3)()
         _lib.check(_lib.load().spk_last_timing(self._ctx, ms), self._ctx)
-        return {"fwd_welch_ms": ms[0], "channels_ms": ms[1], "finalize_ms": ms[2]}
+        return {"forward_ms": ms[0], "unfused_channels_ms": ms[1], "finalize_ms": ms[2]}
+
+    def fused_channels(self) -> int:
+        """Channels whose stage runs inside the forward kernel."""
+        n = ctypes.c_uint32()
+        _lib.check(_lib.load().spk_fused_channels(self._ctx, ctypes.byref(n)), self._ctx)
+        return n.value
 
     # -- results ---------------------------------------------------------------
     def last_blocks(self) -> int:
