@@ -140,6 +140,27 @@ spk_status spk_last_timing(spk_ctx* ctx, float ms[3]);
  * counterpart in the reference (plan introspection). */
 spk_status spk_fused_channels(spk_ctx* ctx, uint32_t* n);
 
+/* ---- Differentiable channel weights (no counterpart in the reference; the
+ * reference's weights are fixed raised cosines, fcfb.rs:197-203).  Packed
+ * layouts follow channel id order: weights L floats per channel; channel
+ * outputs and their gradients nblocks * 3L/4 complex per channel; gathered-
+ * bin gradients nblocks * L complex per channel (nblocks = blocks of the
+ * last process call).  All pointers are device pointers; the calls are
+ * asynchronous on the context's stream unless stated. */
+
+/* Set every channel's weights from a packed device array (synchronous: the
+ * host copies are refreshed too). */
+spk_status spk_set_weights_device(spk_ctx* ctx, const float* weights);
+/* Copy the last call's channel outputs into a packed device array of cap floats. */
+spk_status spk_channels_output_packed(spk_ctx* ctx, float* dst, size_t cap);
+/* Adjoint of weight-multiply, IFFT and discard (FilterDsp::process,
+ * fcfb.rs:154-188) for the last process call: ybar holds dLoss/dRe + i
+ * dLoss/dIm of every packed output sample; writes wbar (packed weights
+ * layout, summed over blocks) and, when xbar is not NULL, the gradient with
+ * respect to every gathered bin get_bin(first + j).  Needs a context created
+ * with SPK_FLAG_KEEP_SPECTRA (SPK_ERR_STATE otherwise). */
+spk_status spk_channels_adjoint(spk_ctx* ctx, const float* ybar, float* wbar, float* xbar);
+
 /* freq_to_bins / bins_to_freq / freq_to_bins_exact (spektri/src/dsp/fcfb.rs:225-285).
  * Return SPK_ERR_INVALID when the reference returns None. */
 spk_status spk_freq_to_bins(double fs_in, double fc_in, uint32_t fft_size, double fs_out,

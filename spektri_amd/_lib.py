@@ -64,6 +64,9 @@ _SIGS = {
     "spk_enable_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "spk_last_timing": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float)]),
     "spk_fused_channels": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32)]),
+    "spk_set_weights_device": (ctypes.c_int, [_P, _P]),
+    "spk_channels_output_packed": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "spk_channels_adjoint": (ctypes.c_int, [_P, _P, _P, _P]),
     "spk_freq_to_bins": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_uint32,
                                         ctypes.c_double, ctypes.c_double,
                                         ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int64)]),

@@ -1,0 +1,93 @@
+"""torch.autograd wrapper of the channel stage with the hand-written adjoint.
+
+``ChannelBank`` owns a GPU context created with the forward spectra kept on
+the device and a device-resident input stream.  ``ChannelBank.apply(weights)``
+runs the whole path (forward FFT, gather x weights, IFFT, discard) with the
+given per-channel weights and returns every channel's output as one packed
+float32 tensor.  Its backward calls ``spk_channels_adjoint``: the gradient
+with respect to the weights comes from the library's adjoint kernel, not
+from torch.
+
+Packed layouts (channel id order): weights ``[sum L]``; outputs
+``[nblocks * sum(3L/4), 2]`` (re, im) with channel id's rows starting at
+``nblocks * sum_{c < id} 3 L_c / 4``.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from .dsp import DspParams, DspState
+
+
+class _StageFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, weights, bank):
+        bank._run(weights)
+        ctx.bank = bank
+        return bank._out.clone()
+
+    @staticmethod
+    def backward(ctx, gout):
+        bank = ctx.bank
+        g = gout.contiguous().to(torch.float32)
+        wbar = torch.empty(bank.weight_count, dtype=torch.float32, device=g.device)
+        # the library runs on its own stream: torch's work producing g must be done
+        torch.cuda.synchronize(g.device)
+        bank.state.channels_adjoint(g.data_ptr(), wbar.data_ptr(), 0)
+        bank.state.synchronize()
+        return wbar, None
+
+
+class ChannelBank:
+    """A fixed input stream and a channel set whose weights are trainable."""
+
+    def __init__(self, fft_size: int, fs: float, channels, samples: torch.Tensor, ffts_per_buf: int = 8,
+                 averages: int | None = None):
+        """channels: list of (first_bin, L); samples: device int16 tensor of
+        the real input stream."""
+        if samples.dtype != torch.int16 or not samples.is_cuda:
+            raise ValueError("samples must be a device int16 tensor")
+        self.samples = samples.contiguous()
+        H = fft_size - fft_size // 4
+        nblocks = (self.samples.numel() // (ffts_per_buf * H)) * ffts_per_buf
+        self.state, _ = DspState.init(DspParams(complex=False, fs_in=fs, fft_size=fft_size,
+                                                ffts_per_buf=ffts_per_buf, spectrum_averages=averages or nblocks,
+                                                max_blocks=nblocks, keep_spectra=True))
+        self.layout = []
+        for first, L in channels:
+            self.state.add_channel(int(first), int(L))
+            self.layout.append(int(L))
+        self.weight_count = sum(self.layout)
+        self.nblocks = nblocks
+        self.output_floats = 2 * nblocks * sum(3 * L // 4 for L in self.layout)
+        self._out = torch.empty(self.output_floats // 2, 2, dtype=torch.float32, device=samples.device)
+
+    def _run(self, weights: torch.Tensor):
+        w = weights.detach().contiguous().to(torch.float32)
+        if w.numel() != self.weight_count or not w.is_cuda:
+            raise ValueError(f"weights must be a device tensor of {self.weight_count} floats")
+        st = self.state
+        torch.cuda.synchronize(w.device)
+        st.set_weights_device(w.data_ptr())
+        st.reset()
+        st.process_device(self.samples.data_ptr(), self.samples.numel())
+        if st.last_blocks() != self.nblocks:
+            raise _lib.SpektriError(_lib.SPK_OK, "unexpected block count")
+        st.channels_output_packed(self._out.data_ptr(), self.output_floats)
+        st.synchronize()
+
+    def apply(self, weights: torch.Tensor) -> torch.Tensor:
+        return _StageFn.apply(weights, self)
+
+    def split(self, packed: torch.Tensor):
+        """Per-channel views [nblocks * 3L/4, 2] of a packed output tensor."""
+        out, off = [], 0
+        for L in self.layout:
+            n = self.nblocks * (3 * L // 4)
+            out.append(packed[off:off + n])
+            off += n
+        return out
+
+    def close(self):
+        self.state.close()

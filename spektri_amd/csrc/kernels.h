@@ -734,4 +734,144 @@ __global__ void welch_finalize_kernel(FinParams F) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the channel stage (no counterpart in the reference; the weights
+// become trainable).  For one (channel, block) the stage is
+//   B[i] = W[j] G[j],  j = (i + L/2) mod L,  G[j] = get_bin(first + j),
+//   y = IFFT_L(B) (unnormalised, + sign),  out[n] = y[n + L/8], n < 3L/4.
+// For a real loss with upstream gradient ybar[n] = dL/dRe out[n] + i dL/dIe out[n],
+// let Yh = FFT_L(z) (- sign) of z[i] = ybar[i - L/8] on [L/8, 7L/8), 0 else.
+// Then dL/dW[j] = sum over blocks of Re(G[j] conj(Yh[i])) and the gradient
+// with respect to the gathered bin is Gbar[j] = W[j] Yh[i].
+// Grid: x = channel * S + split; every tile of a workgroup walks the blocks
+// g = (it S + split) TPW + tile, keeps its W sums in registers, and the
+// workgroup adds its tiles in a fixed order into wpart[channel][split][j].
+// Packed layouts (channel id order): ybar nblocks * 3L/4 complex per
+// channel at nblocks * 3 w_off / 4, xbar nblocks * L complex at
+// nblocks * w_off, W sums L floats at w_off.
+// ---------------------------------------------------------------------------
+struct ChanAdjParams {
+  const cpx* spec;
+  long long spec_stride;
+  int N, K, real_input;
+  const float* weights;
+  const ChanDesc* chans;
+  int nch, nblocks, S;
+  const cpx* ybar;
+  cpx* xbar;     // may be null
+  float* wpart;  // [nch][S][L]
+  const cpx* tw2;  // forward-sign pass tables
+  const cpx* tw3;
+};
+
+template <int L, int R, int R1, int R2, int R3>
+__global__ __launch_bounds__(kChanThreads) void chan_adjoint_kernel(ChanAdjParams P) {
+  constexpr int T = L / R;
+  static_assert(R1 * R2 * R3 == L, "radix plan must cover L");
+  constexpr int TPW = kChanThreads / T;
+  constexpr int LP = lds_padded_size(L);
+  static_assert(LP * 2 >= L, "reduction rows fit the tile regions");
+  __shared__ float lds_all[TPW * LP * 2];
+
+  const int tl = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  float* lds = lds_all + tl * LP * 2;
+  const int c = (int)blockIdx.x / P.S, split = (int)blockIdx.x % P.S;
+  const ChanDesc d = P.chans[c];
+  const float* W = P.weights + d.w_off;
+  constexpr int RL = (R3 > 1) ? R3 : R2;
+  constexpr int PL = L / RL;
+
+  float wacc[R];
+#pragma unroll
+  for (int e = 0; e < R; ++e) wacc[e] = 0.f;
+  const int iters = (P.nblocks + P.S * TPW - 1) / (P.S * TPW);
+  for (int it = 0; it < iters; ++it) {  // uniform trip count: the exchanges hold barriers
+    const int g = (it * P.S + split) * TPW + tl;
+    const bool act = g < P.nblocks;
+    const int gg = act ? g : 0;
+    const cpx* yb = P.ybar + (long long)P.nblocks * (d.w_off / 4 * 3) + (long long)gg * (3 * L / 4);
+    cpx x[R];
+#pragma unroll
+    for (int e = 0; e < R; ++e) {
+      const int n = st_in_index<L, R, T, R1>(t, e);
+      x[e] = (n >= L / 8 && n < (L / 8) * 7) ? yb[n - L / 8] : cpx{0.f, 0.f};
+    }
+    st_butterflies<L, R, T, R1, 1, -1>(x, t, nullptr);
+    lds_exchange<R, false>(
+        x, lds, [&](int e) { return st_out_addr<L, R, T, R1, 1>(t, e); },
+        [&](int e) { return st_in_addr<L, R, T, R2>(t, e); });
+    st_butterflies<L, R, T, R2, R1, -1>(x, t, P.tw2);
+    if constexpr (R3 > 1) {
+      lds_exchange<R, false>(
+          x, lds, [&](int e) { return st_out_addr<L, R, T, R2, R1>(t, e); },
+          [&](int e) { return st_in_addr<L, R, T, R3>(t, e); });
+      st_butterflies<L, R, T, R3, R1 * R2, -1>(x, t, P.tw3);
+    }
+    const cpx* X = P.spec + (long long)gg * P.spec_stride;
+    cpx* xb = P.xbar ? P.xbar + (long long)P.nblocks * d.w_off + (long long)gg * L : nullptr;
+#pragma unroll
+    for (int e = 0; e < R; ++e) {
+      const int u = e / RL, q = e % RL;
+      const int i = t + T * u + q * PL;
+      const int j = (i + L / 2) & (L - 1);
+      int m = d.f_mod + j;
+      if (m >= P.N) m -= P.N;
+      const bool mirror = P.real_input && m >= P.K;
+      cpx G = X[mirror ? P.N - m : m];
+      if (mirror) G.y = -G.y;
+      if (act) {
+        wacc[e] += G.x * x[e].x + G.y * x[e].y;
+        if (xb) xb[j] = cscale(x[e], W[j]);
+      }
+    }
+  }
+  // Add the tiles of this workgroup in tile order.
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < R; ++e) {
+    const int u = e / RL, q = e % RL;
+    lds_all[tl * L + t + T * u + q * PL] = wacc[e];
+  }
+  __syncthreads();
+  for (int i = (int)threadIdx.x; i < L; i += kChanThreads) {
+    float sum = 0.f;
+    for (int k = 0; k < TPW; ++k) sum += lds_all[k * L + i];
+    P.wpart[((long long)c * P.S + split) * L + ((i + L / 2) & (L - 1))] = sum;
+  }
+}
+
+// wbar[w_off + j] = sum over splits (in order) of wpart[c][s][j].
+struct AdjReduceParams {
+  const float* wpart;
+  const ChanDesc* chans;
+  int L, S;
+  float* wbar;
+};
+__global__ void adjoint_reduce_kernel(AdjReduceParams P) {
+  const int c = (int)blockIdx.y;
+  const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (j >= P.L) return;
+  float sum = 0.f;
+  for (int s = 0; s < P.S; ++s) sum += P.wpart[((long long)c * P.S + s) * P.L + j];
+  P.wbar[P.chans[c].w_off + j] = sum;
+}
+
+// Copy channel outputs from the capacity layout (out_off) to the packed
+// layout (nblocks * 3 w_off / 4).  Grid y = channel.
+struct PackParams {
+  const cpx* src;
+  const ChanDesc* chans;
+  int L, nblocks;
+  cpx* dst;
+};
+__global__ void pack_outputs_kernel(PackParams P) {
+  const ChanDesc d = P.chans[blockIdx.y];
+  const long long n = (long long)P.nblocks * (3 * P.L / 4);
+  const cpx* s = P.src + d.out_off;
+  cpx* o = P.dst + (long long)P.nblocks * (d.w_off / 4 * 3);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    o[i] = s[i];
+}
+
 }  // namespace spk

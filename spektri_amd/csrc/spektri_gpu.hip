@@ -9,6 +9,7 @@
 // the channels not fused, and the Welch finaliser.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -58,7 +59,8 @@ struct ChanGroup {  // channels sharing one IFFT size
   int L;
   std::vector<uint32_t> ids;
   DevBuf desc;   // spk::ChanDesc[ids.size()]
-  DevBuf tw2, tw3;
+  DevBuf tw2, tw3;      // inverse-sign pass tables
+  DevBuf atw2, atw3;    // forward-sign pass tables (adjoint)
 };
 
 struct FwdPlan {
@@ -127,6 +129,8 @@ struct spk_ctx {
   int fscat_nyq = 0;
   DevBuf ftw2, ftw3, ftwu;
   DevBuf eitems;             // Welch items of the edge bins (spk::edge_items)
+  DevBuf adj_part;           // adjoint per-split weight sums
+  int ncu = 0;
   long long spec_stride = 0, wstride = 0;
   int wslots = 0, welch_out_cap = 0;
   int gmax[2] = {0, 0};      // persistent workgroups of the unfused / fused forward kernel
@@ -223,6 +227,8 @@ spk_status rebuild_channels(spk_ctx* c) {
     HIP_TRY(c, upload(g.desc, d));
     HIP_TRY(c, upload(g.tw2, pass_table(p->R2, p->R1, +1)));
     HIP_TRY(c, upload(g.tw3, pass_table(p->R3, p->R1 * p->R2, +1)));
+    HIP_TRY(c, upload(g.atw2, pass_table(p->R2, p->R1, -1)));
+    HIP_TRY(c, upload(g.atw3, pass_table(p->R3, p->R1 * p->R2, -1)));
   }
   // The forward kernel runs the L = 256 channels itself when one 16-thread
   // tile per channel fits in its workgroup.
@@ -333,6 +339,30 @@ hipError_t dispatch_chan(int L, const spk::ChanParams& P, hipStream_t s) {
     case 2048: launch_chan<2048, 32, 32, 32, 2>(P, s); break;
     case 4096: launch_chan<4096, 16, 16, 16, 16>(P, s); break;
     case 8192: launch_chan<8192, 32, 32, 16, 16>(P, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int L, int R, int R1, int R2, int R3>
+void launch_chan_adjoint(const spk::ChanAdjParams& P, hipStream_t s) {
+  hipLaunchKernelGGL((spk::chan_adjoint_kernel<L, R, R1, R2, R3>), dim3(P.nch * P.S), dim3(spk::kChanThreads), 0, s,
+                     P);
+}
+
+hipError_t dispatch_chan_adjoint(int L, const spk::ChanAdjParams& P, hipStream_t s) {
+  switch (L) {
+    case 8: launch_chan_adjoint<8, 4, 4, 2, 1>(P, s); break;
+    case 16: launch_chan_adjoint<16, 4, 4, 4, 1>(P, s); break;
+    case 32: launch_chan_adjoint<32, 8, 8, 4, 1>(P, s); break;
+    case 64: launch_chan_adjoint<64, 8, 8, 8, 1>(P, s); break;
+    case 128: launch_chan_adjoint<128, 16, 16, 8, 1>(P, s); break;
+    case 256: launch_chan_adjoint<256, 16, 16, 16, 1>(P, s); break;
+    case 512: launch_chan_adjoint<512, 32, 32, 16, 1>(P, s); break;
+    case 1024: launch_chan_adjoint<1024, 32, 32, 32, 1>(P, s); break;
+    case 2048: launch_chan_adjoint<2048, 32, 32, 32, 2>(P, s); break;
+    case 4096: launch_chan_adjoint<4096, 16, 16, 16, 16>(P, s); break;
+    case 8192: launch_chan_adjoint<8192, 32, 32, 16, 16>(P, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -509,6 +539,7 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
     return cleanup(SPK_ERR_DEVICE);
+  c->ncu = ncu;
   for (int f = 0; f < 2; ++f) {
     const FwdVariant* v = find_fwd_variant(c->M, f ? kFusedL : 0);
     const int per_cu = v ? v->occupancy() : 0;
@@ -700,6 +731,98 @@ spk_status spk_enable_timing(spk_ctx* c, int on) {
   c->ev_used = c->ev_done = 0;
   c->timed_calls = 0;
   for (double& v : c->ms_sum) v = 0;
+  return SPK_OK;
+}
+
+size_t packed_weight_floats(const spk_ctx* c) {
+  size_t n = 0;
+  for (const auto& ch : c->chans) n += (size_t)ch.L;
+  return n;
+}
+
+spk_status spk_set_weights_device(spk_ctx* c, const float* w) {
+  if (!c || !w) return SPK_ERR_INVALID;
+  if (c->chans.empty()) return fail(c, SPK_ERR_STATE, "no channels");
+  if (c->chans_dirty) {
+    spk_status st = rebuild_channels(c);
+    if (st != SPK_OK) return st;
+  }
+  const size_t n = packed_weight_floats(c);
+  HIP_TRY(c, hipMemcpyAsync(c->weights.p, w, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  // keep the host copies current so a later rebuild uploads the same weights
+  std::vector<float> hw(n);
+  HIP_TRY(c, hipMemcpyAsync(hw.data(), c->weights.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  size_t off = 0;
+  for (auto& ch : c->chans) {
+    ch.w.assign(hw.begin() + off, hw.begin() + off + ch.L);
+    off += (size_t)ch.L;
+  }
+  return SPK_OK;
+}
+
+spk_status spk_channels_output_packed(spk_ctx* c, float* dst, size_t cap) {
+  if (!c || !dst) return SPK_ERR_INVALID;
+  if (!c->have_output) return fail(c, SPK_ERR_STATE, "no process call yet");
+  const size_t need = 2 * (size_t)c->last_blocks * (packed_weight_floats(c) / 4 * 3);
+  if (cap < need) return fail(c, SPK_ERR_CAPACITY, "destination smaller than the packed outputs");
+  for (auto& kv : c->groups) {
+    const ChanGroup& g = kv.second;
+    spk::PackParams Q{};
+    Q.src = c->chan_out.as<cpx>();
+    Q.chans = g.desc.as<spk::ChanDesc>();
+    Q.L = g.L;
+    Q.nblocks = (int)c->last_blocks;
+    Q.dst = reinterpret_cast<cpx*>(dst);
+    const long long n = (long long)c->last_blocks * (3 * g.L / 4);
+    const unsigned gx = (unsigned)std::min<long long>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(spk::pack_outputs_kernel, dim3(gx, (unsigned)g.ids.size()), dim3(256), 0, c->stream, Q);
+    HIP_TRY(c, hipGetLastError());
+  }
+  return SPK_OK;
+}
+
+spk_status spk_channels_adjoint(spk_ctx* c, const float* ybar, float* wbar, float* xbar) {
+  if (!c || !ybar || !wbar) return SPK_ERR_INVALID;
+  if (!c->have_output) return fail(c, SPK_ERR_STATE, "no process call yet");
+  if (!c->keep_spec) return fail(c, SPK_ERR_STATE, "the adjoint needs a context created with SPK_FLAG_KEEP_SPECTRA");
+  const int nb = (int)c->last_blocks;
+  for (auto& kv : c->groups) {
+    ChanGroup& g = kv.second;
+    const int nch = (int)g.ids.size();
+    const int T = g.L / find_chan_plan(g.L)->R;
+    const int tpw = spk::kChanThreads / T;
+    // splits over blocks: enough workgroups for every CU, at least one tile-row of blocks each
+    int S = (2 * c->ncu + nch - 1) / nch;
+    S = std::max(1, std::min(S, (nb + tpw - 1) / tpw));
+    const size_t need = (size_t)nch * S * g.L * sizeof(float);
+    if (c->adj_part.bytes < need) HIP_TRY(c, c->adj_part.alloc(need));
+    spk::ChanAdjParams Q{};
+    Q.spec = c->spec.as<cpx>();
+    Q.spec_stride = c->spec_stride;
+    Q.N = c->N;
+    Q.K = c->K;
+    Q.real_input = c->complex_in ? 0 : 1;
+    Q.weights = c->weights.as<float>();
+    Q.chans = g.desc.as<spk::ChanDesc>();
+    Q.nch = nch;
+    Q.nblocks = nb;
+    Q.S = S;
+    Q.ybar = reinterpret_cast<const cpx*>(ybar);
+    Q.xbar = reinterpret_cast<cpx*>(xbar);
+    Q.wpart = c->adj_part.as<float>();
+    Q.tw2 = g.atw2.as<cpx>();
+    Q.tw3 = g.atw3.as<cpx>();
+    HIP_TRY(c, dispatch_chan_adjoint(g.L, Q, c->stream));
+    spk::AdjReduceParams Rp{};
+    Rp.wpart = c->adj_part.as<float>();
+    Rp.chans = g.desc.as<spk::ChanDesc>();
+    Rp.L = g.L;
+    Rp.S = S;
+    Rp.wbar = wbar;
+    hipLaunchKernelGGL(spk::adjoint_reduce_kernel, dim3((g.L + 255) / 256, nch), dim3(256), 0, c->stream, Rp);
+    HIP_TRY(c, hipGetLastError());
+  }
   return SPK_OK;
 }
 

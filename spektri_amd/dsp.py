@@ -171,6 +171,25 @@ class DspState:
         _lib.check(_lib.load().spk_last_timing(self._ctx, ms), self._ctx)
         return {"forward_ms": ms[0], "unfused_channels_ms": ms[1], "finalize_ms": ms[2]}
 
+    # -- differentiable weights (device pointers, packed in channel id order) --
+    def packed_sizes(self):
+        """(weight floats, output floats, gathered-bin floats) of the packed
+        layouts for the last call's block count."""
+        nb = self.last_blocks()
+        ls = [b.bins for _, b in self.channels]
+        return sum(ls), 2 * nb * sum(3 * L // 4 for L in ls), 2 * nb * sum(ls)
+
+    def set_weights_device(self, dptr: int):
+        _lib.check(_lib.load().spk_set_weights_device(self._ctx, ctypes.c_void_p(dptr)), self._ctx)
+
+    def channels_output_packed(self, dptr: int, cap_floats: int):
+        _lib.check(_lib.load().spk_channels_output_packed(self._ctx, ctypes.c_void_p(dptr), cap_floats),
+                   self._ctx)
+
+    def channels_adjoint(self, ybar_ptr: int, wbar_ptr: int, xbar_ptr: int = 0):
+        _lib.check(_lib.load().spk_channels_adjoint(self._ctx, ctypes.c_void_p(ybar_ptr), ctypes.c_void_p(wbar_ptr),
+                                                    ctypes.c_void_p(xbar_ptr or None)), self._ctx)
+
     def fused_channels(self) -> int:
         """Channels whose stage runs inside the forward kernel."""
         n = ctypes.c_uint32()

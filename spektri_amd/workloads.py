@@ -69,6 +69,7 @@ class Workload:
     sweep_amp: float = 0.0
     noise_sigma: float = 0.0
     complex: bool = False
+    tone_kind: list = field(default_factory=list)  # per tone, for the weight-design loss
 
     @property
     def hop(self):
@@ -130,7 +131,35 @@ def rx888_64k(nsamples: int | None = None) -> Workload:
     return wl
 
 
-WORKLOADS = {"plain_8k": plain_8k, "rx888_64k": rx888_64k}
+def design_16k(nsamples: int | None = None) -> Workload:
+    """Differentiable weight design: real s16, N = 16384 (fs 16.384 MHz,
+    1 kHz bins), 256 blocks at F = 8, 256 channels of L = 256 at
+    round4(linspace(200, 7990, 256)), raised-cosine (Hann) initial weights.
+    64 interferer tones and 8 in-band tones at seeded integer bins; the
+    interferers share -10 dBFS (each -10 - 10 log10(64) dBFS, so that their
+    sum does not clip int16), the in-band tones are at -40 dBFS each.
+    ``tone_kind`` marks each tone "interferer" or "inband"."""
+    seed = 4
+    rng = np.random.default_rng(seed)
+    N, fs = 16384, 16.384e6
+    wl = Workload("design_16k", seed, N, fs, 8, 256, nsamples or 256 * 12288)
+    centres = round4(np.linspace(200, 7990, 256))
+    wl.channels = [(int(c) - 128, 256, None) for c in centres]
+    df = fs / N
+    kinds = []
+    amp_i = FULL_SCALE * 10 ** ((-10 - 10 * np.log10(64)) / 20)
+    for b in rng.choice(np.arange(100, 8090), size=64, replace=False):
+        wl.tones.append(Tone(float(b) * df, amp_i, float(rng.uniform(0, 2 * np.pi))))
+        kinds.append("interferer")
+    amp_p = FULL_SCALE * 10 ** (-40 / 20)
+    for b in rng.choice(np.arange(100, 8090), size=8, replace=False):
+        wl.tones.append(Tone(float(b) * df, amp_p, float(rng.uniform(0, 2 * np.pi))))
+        kinds.append("inband")
+    wl.tone_kind = kinds
+    return wl
+
+
+WORKLOADS = {"plain_8k": plain_8k, "rx888_64k": rx888_64k, "design_16k": design_16k}
 
 
 def _deterministic_part_np(wl: Workload, n0: int, n1: int) -> np.ndarray:
