@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--workload", default="rx888_64k")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU work for the baseline sample (0 disables it)")
+    ap.add_argument("--mode", choices=["auto", "per_size", "batched"], default="auto",
+                    help="channel stage launch form for channels not fused (auto: autotune picks)")
+    ap.add_argument("--no-graphs", action="store_true", help="launch eagerly instead of replaying HIP graphs")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after timing, copy this many MiB device-to-device three times")
     return ap.parse_args()
@@ -91,9 +94,17 @@ def main():
 
     st, _ = DspState.init(DspParams(complex=False, fs_in=wl.fs, fft_size=wl.fft_size,
                                     ffts_per_buf=wl.ffts_per_buf, spectrum_averages=wl.averages,
-                                    max_blocks=wl.nblocks))
+                                    max_blocks=wl.nblocks, graphs=not args.no_graphs))
     for first, L, w in wl.channels:
         st.add_channel(first, L, w)
+    tune = None
+    if st.fused_channels() < len(wl.channels):
+        if args.mode == "auto":
+            st.process_device(x.data_ptr(), wl.nsamples)  # spectra for the tuner
+            tune = st.autotune()
+            st.reset()
+        else:
+            st.set_channel_mode(args.mode == "batched")
     # A dedicated (non-null, default-priority) stream shared by the library
     # launches and the torch events that time them.
     stream = torch.cuda.Stream(dev)
@@ -104,7 +115,6 @@ def main():
     for _ in range(args.warmup):
         st.process_device(ptr, n)
     torch.cuda.synchronize()
-    st.enable_timing(True)
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -114,6 +124,11 @@ def main():
     t1.record(stream)
     torch.cuda.synchronize()
     ms_step = t0.elapsed_time(t1) / args.steps
+    # Per-kernel breakdown: the same steps launched eagerly with HIP events
+    # around each launch (events are not recorded inside graphs).
+    st.enable_timing(True)
+    for _ in range(max(3, args.steps // 5)):
+        st.process_device(ptr, n)
     kern = st.last_timing()
     nb = st.last_blocks()
     assert nb == wl.nblocks
@@ -154,7 +169,10 @@ def main():
                  "roofline_MSps": round(roofline_msps, 1)},
         "kernels_ms": {k: round(v, 4) for k, v in kern.items()},
         "fused_channels": fused,
+        "graphs": not args.no_graphs,
     }
+    if tune is not None:
+        out["channel_stage_tune"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tune.items()}
     st.close()
     if args.copy_probe_mib > 0:
         n = args.copy_probe_mib * (1 << 20) // 4

@@ -57,6 +57,10 @@ typedef struct spk_config {
  * Without it the spectrum is only materialised when a channel kernel needs
  * it; channels of L = 256 are then computed inside the forward kernel. */
 #define SPK_FLAG_KEEP_SPECTRA 1u
+/* Capture each distinct (input pointer, block count) launch sequence into a
+ * HIP graph once and replay it (no counterpart in the reference).  Ignored
+ * while per-kernel timing is enabled. */
+#define SPK_FLAG_GRAPHS 2u
 
 /* InputBufferSize (spektri/src/dsp.rs:38-42), in samples. */
 typedef struct spk_buffer_size {
@@ -133,6 +137,30 @@ spk_status spk_synchronize(spk_ctx* ctx);
  * in the reference (measurement hook). */
 spk_status spk_enable_timing(spk_ctx* ctx, int on);
 spk_status spk_last_timing(spk_ctx* ctx, float ms[3]);
+
+/* Launch form of the channel stage for the channels not fused into the
+ * forward kernel (no counterpart in the reference, whose filters run one by
+ * one on rayon, fcfb.rs:66-100):
+ *   SPK_CHAN_PER_SIZE  one launch per distinct IFFT size (the default);
+ *   SPK_CHAN_BATCHED   one persistent launch for every channel of every
+ *                      size, with XCD-local block chunks.  */
+#define SPK_CHAN_PER_SIZE 0
+#define SPK_CHAN_BATCHED 1
+spk_status spk_set_channel_mode(spk_ctx* ctx, int mode);
+
+/* Time both launch forms on the context's current spectra for nblocks
+ * blocks (0 = max_blocks), search the batched form's block chunk, rounds
+ * per work item and workgroups per CU, keep the fastest settings, and
+ * select the faster form.  Synchronous; overwrites channel outputs. */
+typedef struct spk_tune_result {
+  float per_size_ms;
+  float batched_ms;
+  uint32_t chunk_blocks;
+  uint32_t rounds_per_item;
+  uint32_t wgs_per_cu; /* 0 = as many as fit */
+  int mode;            /* SPK_CHAN_* selected */
+} spk_tune_result;
+spk_status spk_autotune(spk_ctx* ctx, uint32_t nblocks, spk_tune_result* out);
 
 /* Number of channels whose gather, weighting, IFFT and discard run inside
  * the forward kernel (all channels of IFFT size 256 when they lie in

@@ -17,6 +17,7 @@ STATUS_NAMES = {0: "SPK_OK", 1: "SPK_ERR_INVALID", 2: "SPK_ERR_UNSUPPORTED", 3: 
 FMT_S16LE = 0x08
 FMT_CF32LE = 0x5C
 FLAG_KEEP_SPECTRA = 1
+FLAG_GRAPHS = 2
 
 
 class SpkConfig(ctypes.Structure):
@@ -29,6 +30,15 @@ class SpkConfig(ctypes.Structure):
 class SpkBufferSize(ctypes.Structure):
     _fields_ = [("new_samples", ctypes.c_uint64), ("overlap", ctypes.c_uint64),
                 ("total", ctypes.c_uint64)]
+
+
+class SpkTuneResult(ctypes.Structure):
+    _fields_ = [("per_size_ms", ctypes.c_float), ("batched_ms", ctypes.c_float),
+                ("chunk_blocks", ctypes.c_uint32), ("rounds_per_item", ctypes.c_uint32),
+                ("wgs_per_cu", ctypes.c_uint32), ("mode", ctypes.c_int)]
+
+
+CHAN_PER_SIZE, CHAN_BATCHED = 0, 1
 
 
 class SpektriError(RuntimeError):
@@ -65,6 +75,8 @@ _SIGS = {
     "spk_last_timing": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float)]),
     "spk_fused_channels": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32)]),
     "spk_set_weights_device": (ctypes.c_int, [_P, _P]),
+    "spk_set_channel_mode": (ctypes.c_int, [_P, ctypes.c_int]),
+    "spk_autotune": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.POINTER(SpkTuneResult)]),
     "spk_channels_output_packed": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "spk_channels_adjoint": (ctypes.c_int, [_P, _P, _P, _P]),
     "spk_freq_to_bins": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_uint32,

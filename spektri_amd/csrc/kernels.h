@@ -75,7 +75,7 @@ struct FwdParams {
   cpx* edump;           // per workgroup: spectrum values of the edge lanes [workgroup][wave][kEdgeLanes][R]
   const int4* eitems;   // Welch items of the edge bins (edge_items)
   int A;
-  long long block0;     // stream index of this launch's block 0
+  const long long* pos;  // device stream position: stream index of this launch's block 0
   float wscale;
   int nblocks;
   int H, O;
@@ -290,7 +290,8 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   const int G = gridDim.x, wg = blockIdx.x;
   const int lo = (int)((long long)wg * P.nblocks / G);
   const int hi = (int)((long long)(wg + 1) * P.nblocks / G);
-  const long long grp0 = (P.block0 + lo) / P.A;
+  const long long block0 = *P.pos;
+  const long long grp0 = (block0 + lo) / P.A;
   const float w = P.wscale;
 
   // Raw s16 pairs of the block about to be transformed: point n = b + j M/R1
@@ -302,19 +303,16 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     // point's pair range [T u + j M/R1, + T) lies wholly on one side of that
     // seam (M/4 is a multiple of T), so each point reads from one of two
     // uniform row pointers, chosen at compile time.
-    uintptr_t lo_row, hi_row;
-    if (g == 0) {
-      lo_row = reinterpret_cast<uintptr_t>(P.hist);
-      hi_row = reinterpret_cast<uintptr_t>(P.src) - (uintptr_t)(M / 4) * sizeof(int);
-    } else {
-      lo_row = hi_row = reinterpret_cast<uintptr_t>(P.src + ((long long)g * P.H - P.O));
-    }
+    // (typed pointers, not integers: the loads keep the global address space)
+    const int* blk = reinterpret_cast<const int*>(P.src + (g == 0 ? 0 : (long long)g * P.H - P.O));
+    const int* lo_row = g == 0 ? reinterpret_cast<const int*>(P.hist) : blk;
+    const int* hi_row = g == 0 ? reinterpret_cast<const int*>(P.src) : blk;
+    const int hi_off = g == 0 ? -(M / 4) : 0;
 #pragma unroll
     for (int e = 0; e < R; ++e) {
       const int u = e / R1, j = e % R1;
       const int base = T * u + j * (M / R1);
-      const int* row = reinterpret_cast<const int*>(base < M / 4 ? lo_row : hi_row);
-      vin[e] = row[base + (int)threadIdx.x];
+      vin[e] = base < M / 4 ? lo_row[base + (int)threadIdx.x] : hi_row[base + hi_off + (int)threadIdx.x];
     }
   };
   // 1024-thread workgroups have no registers to spare for the prefetch
@@ -329,12 +327,16 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     // Per-block values that do not depend on g (lane maps, table reads) are
     // recomputed and reloaded every block: kept live across the loop they
     // would cost more registers than the 1024-thread workgroups have.
-    int t = threadIdx.x;
-    const cpx* tw2 = P.tw2;
-    const cpx* tw3 = P.tw3;
-    const cpx* twu = P.twu;
+    // (The opaque value is an integer offset, not the pointers themselves:
+    // a pointer passed through asm loses its global address space and its
+    // loads become flat loads, which also count against the LDS wait
+    // counter.)
+    int t = threadIdx.x, zero = 0;
     asm volatile("" : "+v"(t));
-    asm volatile("" : "+s"(tw2), "+s"(tw3), "+s"(twu));
+    asm volatile("" : "+s"(zero));
+    const cpx* tw2 = P.tw2 + zero;
+    const cpx* tw3 = P.tw3 + zero;
+    const cpx* twu = P.twu + zero;
 
     // Lane map of the last pass: lane l < 32 of wave v takes sub-transform
     // k = pi = 32 v + l; lane l + 32 takes its mirror P3 - pi (P3/2 for pi = 0).
@@ -371,7 +373,7 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     // indices, loaded ahead of the pass-3 DFT.  The row was written by this
     // workgroup's previous block: it is read at agent scope, which bypasses
     // the CU's L1 and so sees those stores.
-    const long long gs = P.block0 + g;
+    const long long gs = block0 + g;
     const int slot = (int)(gs / P.A - grp0);
     const bool fresh = (g == lo) || (gs % P.A == 0);
     const __amdgpu_buffer_rsrc_t prow = __builtin_amdgcn_make_buffer_rsrc(
@@ -571,8 +573,9 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
       lds_exchange<RF, false>(
           gx, tl, [&](int e) { return st_out_addr<FL, RF, TF, FC::RF1, 1>(ftt, e); },
           [&](int e) { return st_in_addr<FL, RF, TF, FC::RF2>(ftt, e); });
-      const cpx* ftw2 = P.ftw2;
-      asm volatile("" : "+s"(ftw2));
+      int zero2 = 0;
+      asm volatile("" : "+s"(zero2));
+      const cpx* ftw2 = P.ftw2 + zero2;
       st_butterflies<FL, RF, TF, FC::RF2, FC::RF1, +1>(gx, ftt, ftw2);
       constexpr int PL = FL / FC::RF2;
       if (fact) {
@@ -611,13 +614,61 @@ struct ChanParams {
   const float* weights;
   const ChanDesc* chans;
   int nch, nblocks;
-  cpx* out;
-  long long out_block_stride;  // cpx per block in a channel's output = 3L/4
+  cpx* out;  // channel c's block g at out + out_off + g * 3L/4
   const cpx* tw2;
   const cpx* tw3;
 };
 
 constexpr int kChanThreads = 256;
+
+// One tile: gather x weights, L-point inverse FFT, discard, for (channel d,
+// block g).  T = L/R threads of the workgroup run it on their tile's LDS
+// region; inactive tiles compute but do not store (the exchanges hold
+// barriers).  tw2/tw3: inverse-sign pass tables of the plan.
+template <int L, int R, int R1, int R2, int R3>
+__device__ __forceinline__ void chan_tile(const cpx* __restrict__ spec, long long spec_stride, int N, int K,
+                                          int real_input, const float* __restrict__ weights, const ChanDesc& d,
+                                          int g, bool active, float* lds, int t, const cpx* tw2, const cpx* tw3,
+                                          cpx* __restrict__ out_base) {
+  constexpr int T = L / R;
+  const cpx* X = spec + (long long)g * spec_stride;
+  const float* W = weights + d.w_off;
+  cpx x[R];
+#pragma unroll
+  for (int e = 0; e < R; ++e) {
+    const int n = st_in_index<L, R, T, R1>(t, e);
+    const int jj = (n + L / 2) & (L - 1);
+    int m = d.f_mod + jj;
+    if (m >= N) m -= N;
+    const bool mirror = real_input && m >= K;
+    cpx v = X[mirror ? N - m : m];
+    if (mirror) v.y = -v.y;
+    x[e] = cscale(v, W[jj]);
+  }
+  st_butterflies<L, R, T, R1, 1, +1>(x, t, nullptr);
+  lds_exchange<R, false>(
+      x, lds, [&](int e) { return st_out_addr<L, R, T, R1, 1>(t, e); },
+      [&](int e) { return st_in_addr<L, R, T, R2>(t, e); });
+  st_butterflies<L, R, T, R2, R1, +1>(x, t, tw2);
+  constexpr int RL = (R3 > 1) ? R3 : R2;
+  if constexpr (R3 > 1) {
+    lds_exchange<R, false>(
+        x, lds, [&](int e) { return st_out_addr<L, R, T, R2, R1>(t, e); },
+        [&](int e) { return st_in_addr<L, R, T, R3>(t, e); });
+    st_butterflies<L, R, T, R3, R1 * R2, +1>(x, t, tw3);
+  }
+  // Last pass radix RL, p = L / RL: slot u*RL + q holds point b + q p.
+  constexpr int PL = L / RL;
+  cpx* out = out_base + d.out_off + (long long)g * (3 * L / 4);
+  if (active) {
+#pragma unroll
+    for (int e = 0; e < R; ++e) {
+      const int u = e / RL, q = e % RL;
+      const int idx = t + T * u + q * PL;
+      if (idx >= L / 8 && idx < (L / 8) * 7) out[idx - L / 8] = x[e];
+    }
+  }
+}
 
 template <int L, int R, int R1, int R2, int R3>
 __global__ __launch_bounds__(kChanThreads) void chan_kernel(ChanParams P) {
@@ -630,7 +681,6 @@ __global__ __launch_bounds__(kChanThreads) void chan_kernel(ChanParams P) {
 
   const int tl = threadIdx.x / T;
   const int t = threadIdx.x % T;
-  float* lds = lds_all + tl * LP * 2;
   const long long ntiles = (long long)P.nch * P.nblocks;
   long long tile = (long long)blockIdx.x * TPW + tl;
   const bool active = tile < ntiles;
@@ -638,42 +688,128 @@ __global__ __launch_bounds__(kChanThreads) void chan_kernel(ChanParams P) {
   const int c = (int)(tile % P.nch);
   const int g = (int)(tile / P.nch);
   const ChanDesc d = P.chans[c];
-  const cpx* X = P.spec + (long long)g * P.spec_stride;
-  const float* W = P.weights + d.w_off;
+  chan_tile<L, R, R1, R2, R3>(P.spec, P.spec_stride, P.N, P.K, P.real_input, P.weights, d, g, active,
+                              lds_all + tl * LP * 2, t, P.tw2, P.tw3, P.out);
+}
 
-  cpx x[R];
-#pragma unroll
-  for (int e = 0; e < R; ++e) {
-    const int n = st_in_index<L, R, T, R1>(t, e);
-    const int jj = (n + L / 2) & (L - 1);
-    int m = d.f_mod + jj;
-    if (m >= P.N) m -= P.N;
-    const bool mirror = P.real_input && m >= P.K;
-    cpx v = X[mirror ? P.N - m : m];
-    if (mirror) v.y = -v.y;
-    x[e] = cscale(v, W[jj]);
+// ---------------------------------------------------------------------------
+// Batched persistent channel stage: every channel of every size in one
+// launch.  The host cuts the (block, channel) tiles into items: an item
+// holds tiles of one IFFT size class from one chunk of consecutive blocks,
+// block-major (tile t -> block g0 + t / nch, channel ch0 + t % nch of the
+// class-sorted descriptor list).  Chunks are dealt to XCD queues round
+// robin; workgroup w serves queue w mod X (the dispatcher places
+// workgroup w on XCD w mod X), so one chunk's spectrum rows are read into
+// one L2.  Items are taken in order through one atomic counter per queue,
+// which welch_finalize_kernel resets for the next launch.
+// ---------------------------------------------------------------------------
+constexpr int kMaxClasses = 11;
+struct BatchItem {
+  int cls, ch0, nch, g0, t0, t1;
+};
+struct BatchParams {
+  const cpx* spec;
+  long long spec_stride;
+  int N, K, real_input;
+  const float* weights;
+  const ChanDesc* descs;  // class-sorted
+  cpx* out;
+  const BatchItem* items;
+  const int* item_off;  // [X + 1] start of each queue
+  int* counters;        // [X]
+  int X;
+  int cls_L[kMaxClasses];
+  const cpx* tw2[kMaxClasses];
+  const cpx* tw3[kMaxClasses];
+};
+
+// Radix plans of the batched kernel: at most 16 points per thread up to
+// L = 4096 (4096 points per workgroup in LDS), 32 for L = 8192.
+template <int L>
+struct BatchPlan;
+template <> struct BatchPlan<8> { static constexpr int R = 4, R1 = 4, R2 = 2, R3 = 1; };
+template <> struct BatchPlan<16> { static constexpr int R = 4, R1 = 4, R2 = 4, R3 = 1; };
+template <> struct BatchPlan<32> { static constexpr int R = 8, R1 = 8, R2 = 4, R3 = 1; };
+template <> struct BatchPlan<64> { static constexpr int R = 8, R1 = 8, R2 = 8, R3 = 1; };
+template <> struct BatchPlan<128> { static constexpr int R = 16, R1 = 16, R2 = 8, R3 = 1; };
+template <> struct BatchPlan<256> { static constexpr int R = 16, R1 = 16, R2 = 16, R3 = 1; };
+template <> struct BatchPlan<512> { static constexpr int R = 16, R1 = 16, R2 = 16, R3 = 2; };
+template <> struct BatchPlan<1024> { static constexpr int R = 16, R1 = 16, R2 = 16, R3 = 4; };
+template <> struct BatchPlan<2048> { static constexpr int R = 16, R1 = 16, R2 = 16, R3 = 8; };
+template <> struct BatchPlan<4096> { static constexpr int R = 16, R1 = 16, R2 = 16, R3 = 16; };
+template <> struct BatchPlan<8192> { static constexpr int R = 32, R1 = 32, R2 = 16, R3 = 16; };
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+template <int L>
+constexpr int batch_lds_floats() {
+  constexpr int T = L / BatchPlan<L>::R;
+  return (kChanThreads / T) * lds_padded_size(L) * 2;
+}
+
+template <int L>
+__device__ __forceinline__ void batch_item(const BatchParams& P, const BatchItem& bi, float* lds_all) {
+  using BP = BatchPlan<L>;
+  constexpr int T = L / BP::R;
+  constexpr int TPW = kChanThreads / T;
+  constexpr int LP = lds_padded_size(L);
+  const int rounds = (bi.t1 - bi.t0 + TPW - 1) / TPW;
+  for (int r = 0; r < rounds; ++r) {
+    // Lane indices and table pointers are re-derived every round: kept live
+    // across rounds (the compiler would hoist the twiddle loads) they cost
+    // more registers than four workgroups per CU leave.
+    // (an opaque integer offset keeps the tables' global address space)
+    int tid = threadIdx.x, zero = 0;
+    asm volatile("" : "+v"(tid));
+    asm volatile("" : "+s"(zero));
+    const cpx* tw2 = P.tw2[bi.cls] + zero;
+    const cpx* tw3 = P.tw3[bi.cls] + zero;
+    const int tl = tid / T, t = tid % T;
+    int tile = bi.t0 + r * TPW + tl;
+    const bool active = tile < bi.t1;
+    if (!active) tile = bi.t0;
+    const ChanDesc d = P.descs[bi.ch0 + tile % bi.nch];
+    chan_tile<L, BP::R, BP::R1, BP::R2, BP::R3>(P.spec, P.spec_stride, P.N, P.K, P.real_input, P.weights, d,
+                                                bi.g0 + tile / bi.nch, active, lds_all + tl * LP * 2, t, tw2, tw3,
+                                                P.out);
   }
-  st_butterflies<L, R, T, R1, 1, +1>(x, t, nullptr);
-  lds_exchange<R, false>(
-      x, lds, [&](int e) { return st_out_addr<L, R, T, R1, 1>(t, e); },
-      [&](int e) { return st_in_addr<L, R, T, R2>(t, e); });
-  st_butterflies<L, R, T, R2, R1, +1>(x, t, P.tw2);
-  constexpr int RL = (R3 > 1) ? R3 : R2;
-  if constexpr (R3 > 1) {
-    lds_exchange<R, false>(
-        x, lds, [&](int e) { return st_out_addr<L, R, T, R2, R1>(t, e); },
-        [&](int e) { return st_in_addr<L, R, T, R3>(t, e); });
-    st_butterflies<L, R, T, R3, R1 * R2, +1>(x, t, P.tw3);
-  }
-  // Last pass radix RL, p = L / RL: slot u*RL + q holds point b + q p.
-  constexpr int PL = L / RL;
-  cpx* out = P.out + d.out_off + (long long)g * P.out_block_stride;
-  if (active) {
-#pragma unroll
-    for (int e = 0; e < R; ++e) {
-      const int u = e / RL, q = e % RL;
-      const int idx = t + T * u + q * PL;
-      if (idx >= L / 8 && idx < (L / 8) * 7) out[idx - L / 8] = x[e];
+}
+
+// MAXL: largest IFFT size the instantiation serves (4096 or 8192).
+// Four waves per SIMD (four 256-thread workgroups per CU): at most 128
+// VGPRs, which every class plan fits.
+template <int MAXL>
+__global__ __launch_bounds__(kChanThreads, 4) void chan_batched_kernel(BatchParams P) {
+  constexpr int LDSF = cmax(
+      cmax(cmax(batch_lds_floats<8>(), batch_lds_floats<16>()), cmax(batch_lds_floats<32>(), batch_lds_floats<64>())),
+      cmax(cmax(cmax(batch_lds_floats<128>(), batch_lds_floats<256>()),
+                cmax(batch_lds_floats<512>(), batch_lds_floats<1024>())),
+           cmax(cmax(batch_lds_floats<2048>(), batch_lds_floats<4096>()),
+                MAXL > 4096 ? batch_lds_floats<8192>() : 0)));
+  __shared__ float lds_all[LDSF];
+  __shared__ int s_item;
+  const int x = (int)blockIdx.x % P.X;
+  const int base = P.item_off[x], n = P.item_off[x + 1] - base;
+  for (;;) {
+    __syncthreads();  // everyone has read the previous item number
+    if (threadIdx.x == 0) s_item = atomicAdd(P.counters + x, 1);
+    __syncthreads();
+    const int it = s_item;
+    if (it >= n) break;
+    const BatchItem bi = P.items[base + it];
+    switch (P.cls_L[bi.cls]) {
+      case 8: batch_item<8>(P, bi, lds_all); break;
+      case 16: batch_item<16>(P, bi, lds_all); break;
+      case 32: batch_item<32>(P, bi, lds_all); break;
+      case 64: batch_item<64>(P, bi, lds_all); break;
+      case 128: batch_item<128>(P, bi, lds_all); break;
+      case 256: batch_item<256>(P, bi, lds_all); break;
+      case 512: batch_item<512>(P, bi, lds_all); break;
+      case 1024: batch_item<1024>(P, bi, lds_all); break;
+      case 2048: batch_item<2048>(P, bi, lds_all); break;
+      case 4096: batch_item<4096>(P, bi, lds_all); break;
+      default:
+        if constexpr (MAXL > 4096) batch_item<8192>(P, bi, lds_all);
+        break;
     }
   }
 }
@@ -691,7 +827,7 @@ struct FinParams {
   int wslots;
   int G;               // workgroups of the forward launch
   int nblocks;
-  int blk0_mod;        // stream index of the launch's first block mod A
+  const long long* pos;  // device stream position of the launch's first block
   int A, K, M, R;
   float* carry;        // K floats
   float* out;          // records [r][K]
@@ -700,6 +836,8 @@ struct FinParams {
   const int* hist_src;
   int* hist_dst;
   int hist_words;
+  int* chan_counters;  // queue counters of chan_batched_kernel, reset here
+  int ncounters;
 };
 
 // Grid: x over bins, y over the Welch groups the launch touches (group 0 is
@@ -710,13 +848,16 @@ __global__ void welch_finalize_kernel(FinParams F) {
   const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (gi == 0) {
     for (int i = k; i < F.hist_words; i += (int)(gridDim.x * blockDim.x)) F.hist_dst[i] = F.hist_src[i];
+    if (k < F.ncounters) F.chan_counters[k] = 0;
   }
-  if (k >= F.K) return;
+  const int blk0_mod = (int)(*F.pos % F.A);
+  // rows beyond the groups this launch touches (the grid is sized for the worst case)
+  if (k >= F.K || gi > (blk0_mod + F.nblocks - 1) / F.A) return;
   const int ridx = welch_row_index(k, F.M, F.R);
-  const int g_lo = gi * F.A - F.blk0_mod, g_hi = g_lo + F.A;
+  const int g_lo = gi * F.A - blk0_mod, g_hi = g_lo + F.A;
   const int b_lo = g_lo > 0 ? g_lo : 0;
   const int b_hi = g_hi < F.nblocks ? g_hi : F.nblocks;
-  float sum = (gi == 0 && F.blk0_mod != 0) ? F.carry[k] : 0.f;
+  float sum = (gi == 0 && blk0_mod != 0) ? F.carry[k] : 0.f;
   // workgroup of block b: the w with floor(w nb / G) <= b < floor((w+1) nb / G)
   const unsigned nb = (unsigned)F.nblocks, G = (unsigned)F.G;
   const int w0 = (int)(((unsigned)(b_lo + 1) * G + nb - 1) / nb) - 1;
@@ -724,7 +865,7 @@ __global__ void welch_finalize_kernel(FinParams F) {
 #pragma unroll 4
   for (int wg = w0; wg <= w1; ++wg) {
     const int lo = (int)((unsigned)wg * nb / G);
-    const int slot = gi - (F.blk0_mod + lo) / F.A;
+    const int slot = gi - (blk0_mod + lo) / F.A;
     sum += F.wpart[((long long)wg * F.wslots + slot) * F.wstride + ridx];
   }
   if (g_hi <= F.nblocks) {
@@ -839,6 +980,11 @@ __global__ __launch_bounds__(kChanThreads) void chan_adjoint_kernel(ChanAdjParam
     for (int k = 0; k < TPW; ++k) sum += lds_all[k * L + i];
     P.wpart[((long long)c * P.S + split) * L + ((i + L / 2) & (L - 1))] = sum;
   }
+}
+
+// Advance the device stream position after a launch sequence.
+__global__ void advance_kernel(long long* pos, int nblocks) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *pos += nblocks;
 }
 
 // wbar[w_off + j] = sum over splits (in order) of wpart[c][s][j].

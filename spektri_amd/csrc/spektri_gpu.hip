@@ -113,6 +113,18 @@ int bytes_per_sample(uint32_t fmt) { return fmt == SPK_FMT_S16LE ? 2 : (fmt == S
 
 }  // namespace
 
+// Batched channel stage plan (chan_batched_kernel): classes, class-sorted
+// descriptors, and the per-XCD item queues for one block count.
+struct BatchPlan {
+  std::vector<int> cls_L;
+  DevBuf descs, items, item_off;
+  std::vector<DevBuf> tw2, tw3;
+  int nb = -1;     // block count the items were cut for
+  int maxL = 0;
+  size_t nitems = 0;
+};
+constexpr int kXcdQueues = 8;  // MI355X: 8 XCDs; workgroup w is dispatched to XCD w mod 8
+
 struct spk_ctx {
   spk_config cfg{};
   int N = 0, M = 0, K = 0, H = 0, O = 0, F = 0, A = 0, bps = 0;
@@ -131,6 +143,14 @@ struct spk_ctx {
   DevBuf eitems;             // Welch items of the edge bins (spk::edge_items)
   DevBuf adj_part;           // adjoint per-split weight sums
   int ncu = 0;
+  // channel stage launch form and batched tuning (chunk blocks, rounds per item, workgroups per CU)
+  int chan_mode = SPK_CHAN_PER_SIZE;
+  int tune_chunk = 8, tune_rounds = 2, tune_wgs_per_cu = 0;
+  BatchPlan batch;
+  DevBuf chan_counters;      // kXcdQueues ints
+  DevBuf pos;                // device stream position (blocks consumed), long long
+  bool use_graphs = false;
+  std::map<std::pair<const void*, int>, hipGraphExec_t> graphs;
   long long spec_stride = 0, wstride = 0;
   int wslots = 0, welch_out_cap = 0;
   int gmax[2] = {0, 0};      // persistent workgroups of the unfused / fused forward kernel
@@ -273,6 +293,7 @@ spk_status rebuild_channels(spk_ctx* c) {
       c->fscat_nyq = nyq;
     }
   }
+  c->batch.nb = -1;
   c->chans_dirty = false;
   return SPK_OK;
 }
@@ -368,26 +389,146 @@ hipError_t dispatch_chan_adjoint(int L, const spk::ChanAdjParams& P, hipStream_t
   return hipGetLastError();
 }
 
-// One launch sequence over nb blocks whose samples start at src.
-spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
-  if (c->chans_dirty) {
-    spk_status st = rebuild_channels(c);
+template <int MAXL>
+int batched_wgs_per_cu() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)spk::chan_batched_kernel<MAXL>,
+                                                   spk::kChanThreads, 0) != hipSuccess)
+    return 0;
+  return n;
+}
+
+int batch_T(int L) {
+  switch (L) {
+    case 8: case 16: return L / 4;
+    case 32: case 64: return L / 8;
+    case 8192: return L / 32;
+    default: return L / 16;
+  }
+}
+int batch_R1(int L) { return L == 8 || L == 16 ? 4 : (L <= 64 ? 8 : (L == 8192 ? 32 : 16)); }
+int batch_R2(int L) {
+  switch (L) {
+    case 8: return 2;
+    case 16: return 4;
+    case 32: return 4;
+    case 64: return 8;
+    case 128: return 8;
+    default: return 16;
+  }
+}
+
+// Channels that the batched kernel serves: every group except the fused one.
+std::vector<int> batched_groups(const spk_ctx* c) {
+  std::vector<int> ls;
+  for (const auto& kv : c->groups)
+    if (!(c->fused && kv.first == kFusedL)) ls.push_back(kv.first);
+  return ls;
+}
+
+// Cut the (block, channel) tiles of nb blocks into per-XCD item queues.
+spk_status build_batch(spk_ctx* c, int nb) {
+  BatchPlan& B = c->batch;
+  std::vector<int> ls = batched_groups(c);
+  std::sort(ls.begin(), ls.end(), std::greater<int>());  // big tiles first within a chunk
+  B.cls_L = ls;
+  B.maxL = ls.empty() ? 0 : ls.front();
+  std::vector<spk::ChanDesc> descs;
+  std::vector<int> ch0(ls.size()), nch(ls.size());
+  B.tw2.clear();
+  B.tw3.clear();
+  B.tw2.resize(ls.size());
+  B.tw3.resize(ls.size());
+  for (size_t k = 0; k < ls.size(); ++k) {
+    const ChanGroup& g = c->groups.at(ls[k]);
+    ch0[k] = (int)descs.size();
+    nch[k] = (int)g.ids.size();
+    for (uint32_t id : g.ids) {
+      const Channel& ch = c->chans[id];
+      long long fm = ch.first % c->N;
+      if (fm < 0) fm += c->N;
+      long long w_off = 0;
+      for (uint32_t j = 0; j < id; ++j) w_off += c->chans[j].L;
+      descs.push_back(spk::ChanDesc{c->chan_out_off[id], (int)fm, (int)w_off});
+    }
+    const int L = ls[k], R1 = batch_R1(L), R2 = batch_R2(L), R = L / batch_T(L);
+    const int R3 = L / (R1 * R2);
+    (void)R;
+    HIP_TRY(c, upload(B.tw2[k], pass_table(R2, R1, +1)));
+    HIP_TRY(c, upload(B.tw3[k], pass_table(R3, R1 * R2, +1)));
+  }
+  std::vector<std::vector<spk::BatchItem>> q(kXcdQueues);
+  const int chunk = std::max(1, c->tune_chunk);
+  for (int g0 = 0, k = 0; g0 < nb; g0 += chunk, ++k) {
+    const int nbk = std::min(chunk, nb - g0);
+    auto& qx = q[k % kXcdQueues];
+    for (size_t cl = 0; cl < ls.size(); ++cl) {
+      const int tpw = spk::kChanThreads / batch_T(ls[cl]);
+      const int per = tpw * std::max(1, c->tune_rounds);
+      const int ntiles = nch[cl] * nbk;
+      for (int t0 = 0; t0 < ntiles; t0 += per)
+        qx.push_back(spk::BatchItem{(int)cl, ch0[cl], nch[cl], g0, t0, std::min(t0 + per, ntiles)});
+    }
+  }
+  std::vector<spk::BatchItem> all;
+  std::vector<int> off(kXcdQueues + 1, 0);
+  for (int x = 0; x < kXcdQueues; ++x) {
+    off[x] = (int)all.size();
+    all.insert(all.end(), q[x].begin(), q[x].end());
+  }
+  off[kXcdQueues] = (int)all.size();
+  HIP_TRY(c, upload(B.descs, descs));
+  HIP_TRY(c, upload(B.items, all));
+  HIP_TRY(c, upload(B.item_off, off));
+  B.nitems = all.size();
+  B.nb = nb;
+  return SPK_OK;
+}
+
+spk_status launch_batched(spk_ctx* c, int nb, hipStream_t s) {
+  if (c->batch.nb != nb) {
+    spk_status st = build_batch(c, nb);
     if (st != SPK_OK) return st;
   }
-  hipStream_t s = c->stream;
-  hipEvent_t* ev = nullptr;
-  if (c->timing) {
-    if (c->ev_used + 4 > c->ev_pool.size()) {
-      for (int i = 0; i < 64; ++i) {
-        hipEvent_t e;
-        HIP_TRY(c, hipEventCreate(&e));
-        c->ev_pool.push_back(e);
-      }
-    }
-    ev = &c->ev_pool[c->ev_used];
-    c->ev_used += 4;
+  const BatchPlan& B = c->batch;
+  if (B.cls_L.empty()) return SPK_OK;
+  spk::BatchParams P{};
+  P.spec = c->spec.as<cpx>();
+  P.spec_stride = c->spec_stride;
+  P.N = c->N;
+  P.K = c->K;
+  P.real_input = c->complex_in ? 0 : 1;
+  P.weights = c->weights.as<float>();
+  P.descs = B.descs.as<spk::ChanDesc>();
+  P.out = c->chan_out.as<cpx>();
+  P.items = B.items.as<spk::BatchItem>();
+  P.item_off = B.item_off.as<int>();
+  P.counters = c->chan_counters.as<int>();
+  P.X = kXcdQueues;
+  for (size_t k = 0; k < B.cls_L.size(); ++k) {
+    P.cls_L[k] = B.cls_L[k];
+    P.tw2[k] = B.tw2[k].as<cpx>();
+    P.tw3[k] = B.tw3[k].as<cpx>();
   }
+  const bool big = B.maxL > 4096;
+  int per_cu = big ? batched_wgs_per_cu<8192>() : batched_wgs_per_cu<4096>();
+  if (c->tune_wgs_per_cu > 0) per_cu = std::min(per_cu, c->tune_wgs_per_cu);
+  int G = std::max(1, per_cu) * c->ncu;
+  G = (G + kXcdQueues - 1) / kXcdQueues * kXcdQueues;
+  if (big)
+    hipLaunchKernelGGL(spk::chan_batched_kernel<8192>, dim3(G), dim3(spk::kChanThreads), 0, s, P);
+  else
+    hipLaunchKernelGGL(spk::chan_batched_kernel<4096>, dim3(G), dim3(spk::kChanThreads), 0, s, P);
+  HIP_TRY(c, hipGetLastError());
+  return SPK_OK;
+}
 
+// One launch sequence over nb blocks whose samples start at src.
+// Enqueue the launch sequence over nb blocks whose samples start at src:
+// forward kernel, channel stage, Welch finaliser, stream-position advance.
+// Kernels only (no memset or copy nodes), so it can be captured into a
+// graph; every per-call quantity the kernels need is in device memory.
+spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hipEvent_t* ev) {
   spk::FwdParams P{};
   P.src = reinterpret_cast<const int16_t*>(src);
   P.hist = c->hist.as<int16_t>();
@@ -404,7 +545,7 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
   P.wstride = c->wstride;
   P.wslots = c->wslots;
   P.A = c->A;
-  P.block0 = c->blocks_done;
+  P.pos = c->pos.as<long long>();
   P.wscale = c->wscale;
   P.nblocks = nb;
   P.H = c->H;
@@ -426,12 +567,15 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
   const int G = std::min(nb, c->gmax[c->fused ? 1 : 0]);
   if (ev) HIP_TRY(c, hipEventRecord(ev[0], s));
   HIP_TRY(c, fv->launch(P, G, s));
-  const int blk0_mod = (int)(c->blocks_done % c->A);
   if (ev) HIP_TRY(c, hipEventRecord(ev[1], s));
 
+  if (c->chan_mode == SPK_CHAN_BATCHED) {
+    spk_status st = launch_batched(c, nb, s);
+    if (st != SPK_OK) return st;
+  }
   for (auto& kv : c->groups) {
     ChanGroup& g = kv.second;
-    if (&g == fg) continue;
+    if (&g == fg || c->chan_mode == SPK_CHAN_BATCHED) continue;
     spk::ChanParams Q{};
     Q.spec = c->spec.as<cpx>();
     Q.spec_stride = c->spec_stride;
@@ -443,16 +587,12 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
     Q.nch = (int)g.ids.size();
     Q.nblocks = nb;
     Q.out = c->chan_out.as<cpx>();
-    Q.out_block_stride = 3LL * g.L / 4;
     Q.tw2 = g.tw2.as<cpx>();
     Q.tw3 = g.tw3.as<cpx>();
     HIP_TRY(c, dispatch_chan(g.L, Q, s));
   }
   if (ev) HIP_TRY(c, hipEventRecord(ev[2], s));
 
-  const long long s_first = c->blocks_done / c->A;
-  const long long s_last = (c->blocks_done + nb - 1) / c->A;
-  const int nrec = (int)((c->blocks_done + nb) / c->A - s_first);
   {
     spk::FinParams Fp{};
     Fp.wpart = c->wpart.as<float>();
@@ -460,7 +600,7 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
     Fp.wslots = c->wslots;
     Fp.G = G;
     Fp.nblocks = nb;
-    Fp.blk0_mod = blk0_mod;
+    Fp.pos = c->pos.as<long long>();
     Fp.A = c->A;
     Fp.K = c->K;
     Fp.M = c->M;
@@ -472,16 +612,79 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
                                                ((long long)nb * c->H - c->O) * c->bps);
     Fp.hist_dst = c->hist.as<int>();
     Fp.hist_words = c->O * c->bps / 4;
-    const int ngroups = (int)(s_last - s_first + 1);
-    hipLaunchKernelGGL(spk::welch_finalize_kernel, dim3((c->K + 255) / 256, ngroups), dim3(256), 0, s, Fp);
+    Fp.chan_counters = c->chan_counters.as<int>();
+    Fp.ncounters = kXcdQueues;
+    // groups touched: at most (A - 1 + nb - 1) / A + 1; rows past the actual count exit
+    const int gmax = (c->A - 1 + nb - 1) / c->A + 1;
+    hipLaunchKernelGGL(spk::welch_finalize_kernel, dim3((c->K + 255) / 256, gmax), dim3(256), 0, s, Fp);
     HIP_TRY(c, hipGetLastError());
   }
+  hipLaunchKernelGGL(spk::advance_kernel, dim3(1), dim3(64), 0, s, c->pos.as<long long>(), nb);
+  HIP_TRY(c, hipGetLastError());
   if (ev) HIP_TRY(c, hipEventRecord(ev[3], s));
+  return SPK_OK;
+}
 
+void drop_graphs(spk_ctx* c) {
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+}
 
+// One launch sequence over nb blocks whose samples start at src.  With
+// graphs enabled (and timing off) the sequence for (src, nb) is captured
+// once and replayed.
+spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
+  if (c->chans_dirty) {
+    drop_graphs(c);
+    spk_status st = rebuild_channels(c);
+    if (st != SPK_OK) return st;
+  }
+  hipStream_t s = c->stream;
+  if (c->use_graphs && !c->timing) {
+    const auto key = std::make_pair(src, nb);
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+      if (c->graphs.size() >= 8) drop_graphs(c);
+      if (c->chan_mode == SPK_CHAN_BATCHED && c->batch.nb != nb) {
+        spk_status st = build_batch(c, nb);  // uploads happen outside the capture
+        if (st != SPK_OK) return st;
+      }
+      hipGraph_t graph = nullptr;
+      HIP_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      spk_status st = enqueue_blocks(c, src, nb, s, nullptr);
+      hipError_t e = hipStreamEndCapture(s, &graph);
+      if (st != SPK_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+      }
+      HIP_TRY(c, e);
+      hipGraphExec_t exec = nullptr;
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIP_TRY(c, e);
+      it = c->graphs.emplace(key, exec).first;
+    }
+    HIP_TRY(c, hipGraphLaunch(it->second, s));
+  } else {
+    hipEvent_t* ev = nullptr;
+    if (c->timing) {
+      if (c->ev_used + 4 > c->ev_pool.size()) {
+        for (int i = 0; i < 64; ++i) {
+          hipEvent_t e;
+          HIP_TRY(c, hipEventCreate(&e));
+          c->ev_pool.push_back(e);
+        }
+      }
+      ev = &c->ev_pool[c->ev_used];
+      c->ev_used += 4;
+    }
+    spk_status st = enqueue_blocks(c, src, nb, s, ev);
+    if (st != SPK_OK) return st;
+  }
+  const long long s_first = c->blocks_done / c->A;
+  c->last_records = (uint32_t)((c->blocks_done + nb) / c->A - s_first);
   c->blocks_done += nb;
   c->last_blocks = (uint32_t)nb;
-  c->last_records = (uint32_t)nrec;
   c->have_output = true;
   return SPK_OK;
 }
@@ -565,6 +768,11 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
       c->carry.alloc((size_t)c->K * sizeof(float)) != hipSuccess ||
       c->welch_out.alloc((size_t)c->welch_out_cap * c->K * sizeof(float)) != hipSuccess)
     return cleanup(SPK_ERR_DEVICE);
+  if (c->chan_counters.alloc(kXcdQueues * sizeof(int)) != hipSuccess ||
+      hipMemset(c->chan_counters.p, 0, c->chan_counters.bytes) != hipSuccess ||
+      c->pos.alloc(sizeof(long long)) != hipSuccess || hipMemset(c->pos.p, 0, c->pos.bytes) != hipSuccess)
+    return cleanup(SPK_ERR_DEVICE);
+  c->use_graphs = (cfg->flags & SPK_FLAG_GRAPHS) != 0;
   if (hipMemset(c->hist.p, 0, c->hist.bytes) != hipSuccess || hipMemset(c->carry.p, 0, c->carry.bytes) != hipSuccess)
     return cleanup(SPK_ERR_DEVICE);
 
@@ -587,6 +795,7 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
 spk_status spk_destroy(spk_ctx* c) {
   if (!c) return SPK_ERR_INVALID;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graphs(c);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
@@ -704,6 +913,7 @@ spk_status spk_reset(spk_ctx* c) {
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   HIP_TRY(c, hipMemset(c->hist.p, 0, c->hist.bytes));
   HIP_TRY(c, hipMemset(c->carry.p, 0, c->carry.bytes));
+  HIP_TRY(c, hipMemset(c->pos.p, 0, c->pos.bytes));
   c->blocks_done = 0;
   c->last_blocks = 0;
   c->last_records = 0;
@@ -715,6 +925,7 @@ spk_status spk_set_stream(spk_ctx* c, void* stream) {
   if (!c) return SPK_ERR_INVALID;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   c->stream = stream ? reinterpret_cast<hipStream_t>(stream) : c->own_stream;
+  drop_graphs(c);
   return SPK_OK;
 }
 
@@ -822,6 +1033,109 @@ spk_status spk_channels_adjoint(spk_ctx* c, const float* ybar, float* wbar, floa
     Rp.wbar = wbar;
     hipLaunchKernelGGL(spk::adjoint_reduce_kernel, dim3((g.L + 255) / 256, nch), dim3(256), 0, c->stream, Rp);
     HIP_TRY(c, hipGetLastError());
+  }
+  return SPK_OK;
+}
+
+spk_status spk_set_channel_mode(spk_ctx* c, int mode) {
+  if (!c || (mode != SPK_CHAN_PER_SIZE && mode != SPK_CHAN_BATCHED)) return SPK_ERR_INVALID;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->chan_mode = mode;
+  drop_graphs(c);
+  return SPK_OK;
+}
+
+spk_status spk_autotune(spk_ctx* c, uint32_t nblocks, spk_tune_result* out) {
+  if (!c) return SPK_ERR_INVALID;
+  if (c->chans_dirty) {
+    spk_status st = rebuild_channels(c);
+    if (st != SPK_OK) return st;
+  }
+  const int nb = nblocks ? (int)nblocks : c->max_blocks;
+  if (nb > c->max_blocks || nb > (int)std::max<long long>(c->max_blocks, 1)) return fail(c, SPK_ERR_CAPACITY, "nblocks");
+  if (batched_groups(c).empty()) return fail(c, SPK_ERR_STATE, "no channel needs the channel stage");
+  hipStream_t s = c->stream;
+  hipEvent_t e0, e1;
+  HIP_TRY(c, hipEventCreate(&e0));
+  HIP_TRY(c, hipEventCreate(&e1));
+  const int reps = 3;
+  auto time_it = [&](auto&& launch) -> float {
+    float best = 1e30f;
+    for (int r = 0; r < reps + 1; ++r) {
+      if (hipMemsetAsync(c->chan_counters.p, 0, c->chan_counters.bytes, s) != hipSuccess) return -1.f;
+      if (hipEventRecord(e0, s) != hipSuccess) return -1.f;
+      if (launch() != SPK_OK) return -1.f;
+      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return -1.f;
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.f;
+      if (r > 0) best = std::min(best, ms);  // the first run warms up
+    }
+    return best;
+  };
+  // the plain per-size launch
+  const float per_size = time_it([&]() -> spk_status {
+    for (auto& kv : c->groups) {
+      ChanGroup& g = kv.second;
+      if (c->fused && kv.first == kFusedL) continue;
+      spk::ChanParams Q{};
+      Q.spec = c->spec.as<cpx>();
+      Q.spec_stride = c->spec_stride;
+      Q.N = c->N;
+      Q.K = c->K;
+      Q.real_input = c->complex_in ? 0 : 1;
+      Q.weights = c->weights.as<float>();
+      Q.chans = g.desc.as<spk::ChanDesc>();
+      Q.nch = (int)g.ids.size();
+      Q.nblocks = nb;
+      Q.out = c->chan_out.as<cpx>();
+      Q.tw2 = g.tw2.as<cpx>();
+      Q.tw3 = g.tw3.as<cpx>();
+      if (dispatch_chan(g.L, Q, s) != hipSuccess) return SPK_ERR_DEVICE;
+    }
+    return SPK_OK;
+  });
+  const int chunks[] = {2, 4, 8, 16};
+  const int rounds[] = {1, 2, 4};
+  const int wpcu[] = {0, 2};
+  float best = 1e30f;
+  int bc = c->tune_chunk, br = c->tune_rounds, bw = c->tune_wgs_per_cu;
+  for (int ch : chunks)
+    for (int rd : rounds)
+      for (int w : wpcu) {
+        c->tune_chunk = ch;
+        c->tune_rounds = rd;
+        c->tune_wgs_per_cu = w;
+        c->batch.nb = -1;
+        const float ms = time_it([&]() { return launch_batched(c, nb, s); });
+        if (ms < 0.f) {
+          (void)hipEventDestroy(e0);
+          (void)hipEventDestroy(e1);
+          return fail(c, SPK_ERR_DEVICE, "autotune launch failed");
+        }
+        if (ms < best) {
+          best = ms;
+          bc = ch;
+          br = rd;
+          bw = w;
+        }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIP_TRY(c, hipMemsetAsync(c->chan_counters.p, 0, c->chan_counters.bytes, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  c->tune_chunk = bc;
+  c->tune_rounds = br;
+  c->tune_wgs_per_cu = bw;
+  c->batch.nb = -1;
+  c->chan_mode = best < per_size ? SPK_CHAN_BATCHED : SPK_CHAN_PER_SIZE;
+  drop_graphs(c);
+  if (out) {
+    out->per_size_ms = per_size;
+    out->batched_ms = best;
+    out->chunk_blocks = (uint32_t)bc;
+    out->rounds_per_item = (uint32_t)br;
+    out->wgs_per_cu = (uint32_t)bw;
+    out->mode = c->chan_mode;
   }
   return SPK_OK;
 }

@@ -54,6 +54,7 @@ class DspParams:
     input_format: int = _lib.FMT_S16LE
     max_blocks: int = 0             # capacity for process_device; 0 = ffts_per_buf
     keep_spectra: bool = False      # keep every block's spectrum on the device
+    graphs: bool = False            # capture and replay each launch sequence as a HIP graph
 
 
 @dataclass
@@ -101,9 +102,9 @@ class DspState:
     def init(cls, params: DspParams):
         """dsp.rs:76-112: returns (state, InputBufferSize)."""
         lib = _lib.load()
+        flags = (_lib.FLAG_KEEP_SPECTRA if params.keep_spectra else 0) | (_lib.FLAG_GRAPHS if params.graphs else 0)
         cfg = _lib.SpkConfig(params.fft_size, params.input_format, params.scaling,
-                             params.ffts_per_buf, params.spectrum_averages, params.max_blocks,
-                             _lib.FLAG_KEEP_SPECTRA if params.keep_spectra else 0)
+                             params.ffts_per_buf, params.spectrum_averages, params.max_blocks, flags)
         ctx = ctypes.c_void_p()
         _lib.check(lib.spk_create(ctypes.byref(cfg), ctypes.byref(ctx)))
         bs = _lib.SpkBufferSize()
@@ -189,6 +190,21 @@ class DspState:
     def channels_adjoint(self, ybar_ptr: int, wbar_ptr: int, xbar_ptr: int = 0):
         _lib.check(_lib.load().spk_channels_adjoint(self._ctx, ctypes.c_void_p(ybar_ptr), ctypes.c_void_p(wbar_ptr),
                                                     ctypes.c_void_p(xbar_ptr or None)), self._ctx)
+
+    # -- channel stage launch form --------------------------------------------
+    def set_channel_mode(self, batched: bool):
+        """Plain per-size launches (False) or the batched persistent launch (True)."""
+        mode = _lib.CHAN_BATCHED if batched else _lib.CHAN_PER_SIZE
+        _lib.check(_lib.load().spk_set_channel_mode(self._ctx, mode), self._ctx)
+
+    def autotune(self, nblocks: int = 0) -> dict:
+        """Time both forms on the current spectra, keep the fastest batched
+        settings and select the faster form."""
+        r = _lib.SpkTuneResult()
+        _lib.check(_lib.load().spk_autotune(self._ctx, nblocks, ctypes.byref(r)), self._ctx)
+        return {"per_size_ms": r.per_size_ms, "batched_ms": r.batched_ms, "chunk_blocks": r.chunk_blocks,
+                "rounds_per_item": r.rounds_per_item, "wgs_per_cu": r.wgs_per_cu,
+                "mode": "batched" if r.mode == _lib.CHAN_BATCHED else "per_size"}
 
     def fused_channels(self) -> int:
         """Channels whose stage runs inside the forward kernel."""

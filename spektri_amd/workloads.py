@@ -131,6 +131,31 @@ def rx888_64k(nsamples: int | None = None) -> Workload:
     return wl
 
 
+def uneven_64k(nsamples: int | None = None) -> Workload:
+    """Batched uneven channel set: as rx888_64k (signal, N, Welch over 32
+    blocks) at F = 4 (1364 blocks), with 512 channels of IFFT size drawn
+    from {32, 64, 128, 256, 1024, 4096} with probabilities
+    {.40, .25, .15, .10, .07, .03}, distinct round4 centres drawn uniformly
+    so that each channel lies inside [0, K), and Kaiser weights with beta
+    drawn from U(4, 10)."""
+    seed = 2
+    rng = np.random.default_rng(seed)
+    base = rx888_64k(nsamples)
+    wl = Workload("uneven_64k", seed, base.fft_size, base.fs, 4, 32, base.nsamples)
+    wl.tones, wl.sweep_amp, wl.noise_sigma = base.tones, base.sweep_amp, base.noise_sigma
+    sizes = rng.choice([32, 64, 128, 256, 1024, 4096], size=512, p=[.40, .25, .15, .10, .07, .03])
+    K = wl.fft_size // 2 + 1
+    used = set()
+    for L in sizes:
+        while True:
+            c = int(round4(rng.uniform(L / 2, K - L / 2)))
+            if c not in used and c - L // 2 >= 0 and c + L // 2 <= K:
+                used.add(c)
+                break
+        wl.channels.append((c - int(L) // 2, int(L), kaiser_weights(int(L), float(rng.uniform(4, 10)))))
+    return wl
+
+
 def design_16k(nsamples: int | None = None) -> Workload:
     """Differentiable weight design: real s16, N = 16384 (fs 16.384 MHz,
     1 kHz bins), 256 blocks at F = 8, 256 channels of L = 256 at
@@ -159,7 +184,7 @@ def design_16k(nsamples: int | None = None) -> Workload:
     return wl
 
 
-WORKLOADS = {"plain_8k": plain_8k, "rx888_64k": rx888_64k, "design_16k": design_16k}
+WORKLOADS = {"plain_8k": plain_8k, "rx888_64k": rx888_64k, "uneven_64k": uneven_64k, "design_16k": design_16k}
 
 
 def _deterministic_part_np(wl: Workload, n0: int, n1: int) -> np.ndarray:

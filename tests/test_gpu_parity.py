@@ -238,3 +238,122 @@ def test_rx888_full_size_sampled_blocks():
     assert deterministic
     st.close()
     st2.close()
+
+
+def uneven_channels(N, K, rng, n=40, sizes=(8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096)):
+    """Mixed IFFT sizes, distinct centres, a few reaching past DC / Nyquist."""
+    out = []
+    for i in range(n):
+        L = int(sizes[i % len(sizes)])
+        if L > N:
+            continue
+        first = int(rng.integers(-L // 2, K - L // 2)) // 4 * 4
+        out.append((first, L, rng.random(L).astype(np.float32)))
+    return out
+
+
+@pytest.mark.parametrize("N", [8192, 65536])
+def test_batched_channel_stage_matches_per_size_and_oracle(N):
+    """The batched persistent launch against the plain per-size launches (same
+    arithmetic per tile up to the radix plan) and the oracle."""
+    rng = np.random.default_rng(21)
+    F, A = 4, 6
+    K = N // 2 + 1
+    H = N - N // 4
+    chans = uneven_channels(N, K, rng)
+    x = np.clip(np.round(rng.standard_normal(3 * F * H) * 2000), -32768, 32767).astype(np.int16)
+    outs = {}
+    for mode in (False, True):
+        st, ids = run_gpu(x, N, F, A, chans, max_blocks=3 * F, use_device=False)
+        st.set_channel_mode(mode)
+        dev = torch.from_numpy(x).to("cuda")
+        for rep in range(2):  # second call reuses the cut items and the reset queue counters
+            st.reset()
+            st.process_device(dev.data_ptr(), len(x))
+            st.synchronize()
+            outs[(mode, rep)] = [st.channel_output(cid) for cid in ids]
+        st.close()
+    X, recs, ref = oracle_path(x, N, F, A, chans)
+    e_or = max(rms(o, r) for o, r in zip(outs[(True, 1)], ref))
+    e_ps = max(rms(a, b) for a, b in zip(outs[(True, 1)], outs[(False, 1)]))
+    same_rep = all(np.array_equal(a, b) for a, b in zip(outs[(True, 0)], outs[(True, 1)]))
+    report(f"batched_vs_per_size_N{N}", channels=len(chans), batched_vs_oracle=e_or, batched_vs_per_size=e_ps,
+           repeat_identical=same_rep)
+    assert e_or < TOL and e_ps < TOL and same_rep
+
+
+def test_uneven_64k_batched_autotuned_sampled_blocks():
+    """The 512-channel uneven set at full size: autotune, then the selected
+    form against the oracle on sampled blocks and against the other form."""
+    wl = W.uneven_64k()
+    dev = W.make_signal_torch(wl, "cuda")
+    N, F, A = wl.fft_size, wl.ffts_per_buf, wl.averages
+    st, _ = DspState.init(DspParams(complex=False, fs_in=wl.fs, fft_size=N, ffts_per_buf=F, spectrum_averages=A,
+                                    max_blocks=wl.nblocks, keep_spectra=True))
+    ids = [st.add_channel(first, L, w) for first, L, w in wl.channels]
+    st.process_device(dev.data_ptr(), wl.nsamples)
+    st.synchronize()
+    tune = st.autotune()
+    outs = {}
+    for mode in (True, False):
+        st.set_channel_mode(mode)
+        st.reset()
+        st.process_device(dev.data_ptr(), wl.nsamples)
+        st.synchronize()
+        outs[mode] = {i: st.channel_output(i) for i in ids[::17]}
+    nb = st.last_blocks()
+    x = dev.cpu().numpy()
+    geom = O.Geometry(N, False, F)
+    s = O.input_format_scaling(O.FMT_S16LE)
+    errs = []
+    for g in (0, 1, nb // 2, nb - 1):
+        bl = O.frame_blocks(x, geom, 1, first=g)
+        Xg = O.forward_spectra(bl, geom, s)
+        for i in ids[::17]:
+            first, L, w = wl.channels[i]
+            ref = O.channel_process(Xg, N, first, np.asarray(w, np.float64))[0]
+            errs.append(rms(outs[True][i][g], ref))
+    e_ps = max(rms(outs[True][i], outs[False][i]) for i in ids[::17])
+    report("uneven_64k_batched", blocks=nb, channels=len(ids), tune=tune, batched_vs_oracle_sampled=max(errs),
+           batched_vs_per_size=e_ps)
+    assert max(errs) < TOL and e_ps < TOL
+    st.close()
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_graph_replay_matches_eager_across_calls(batched):
+    """Graph-captured launch sequences replayed over a stream of calls (Welch
+    records straddling calls, history carried) equal the eager launches bit
+    for bit; each replay is checked, with eager work and synchronisation in
+    between."""
+    rng = np.random.default_rng(5)
+    N, F, A = 8192, 4, 6
+    K, H = N // 2 + 1, N - N // 4
+    chans = uneven_channels(N, K, rng, n=12)
+    calls = 5
+    x = np.clip(np.round(rng.standard_normal(calls * F * H) * 3000), -32768, 32767).astype(np.int16)
+    dev = torch.from_numpy(x).to("cuda")
+    seg = F * H
+    results = {}
+    for graphs in (False, True):
+        st, _ = DspState.init(DspParams(complex=False, fs_in=1.0, fft_size=N, ffts_per_buf=F, spectrum_averages=A,
+                                        max_blocks=F, graphs=graphs))
+        ids = [st.add_channel(first, L, w) for first, L, w in chans]
+        st.set_channel_mode(batched)
+        outs = []
+        for k in range(calls):
+            # the same device buffer each call (one graph), refilled between calls
+            buf = dev[k * seg:(k + 1) * seg].clone() if k == 0 else buf.copy_(dev[k * seg:(k + 1) * seg])
+            torch.cuda.synchronize()
+            st.process_device(buf.data_ptr(), seg)
+            st.synchronize()
+            outs.append(([st.channel_output(i) for i in ids], st.spectrum_records()))
+        results[graphs] = outs
+        st.close()
+    same = all(all(np.array_equal(a, b) for a, b in zip(e[0], g[0])) and np.array_equal(e[1], g[1])
+               for e, g in zip(results[False], results[True]))
+    X, recs, ref = oracle_path(x, N, F, A, chans)
+    got = np.concatenate([np.asarray(o[1]).reshape(-1, K) for o in results[True]])
+    e_w = rms(got, recs) if recs.size else 0.0
+    report(f"graph_replay_batched{int(batched)}", calls=calls, identical_to_eager=same, welch_rms=e_w)
+    assert same and got.shape == recs.shape and e_w < TOL
