@@ -775,10 +775,10 @@ __device__ __forceinline__ void batch_item(const BatchParams& P, const BatchItem
 }
 
 // MAXL: largest IFFT size the instantiation serves (4096 or 8192).
-// Four waves per SIMD (four 256-thread workgroups per CU): at most 128
-// VGPRs, which every class plan fits.
+// (A minimum-occupancy launch bound of four workgroups per CU made this
+// kernel 1.7x slower on gfx950; LDS already limits it to four.)
 template <int MAXL>
-__global__ __launch_bounds__(kChanThreads, 4) void chan_batched_kernel(BatchParams P) {
+__global__ __launch_bounds__(kChanThreads) void chan_batched_kernel(BatchParams P) {
   constexpr int LDSF = cmax(
       cmax(cmax(batch_lds_floats<8>(), batch_lds_floats<16>()), cmax(batch_lds_floats<32>(), batch_lds_floats<64>())),
       cmax(cmax(cmax(batch_lds_floats<128>(), batch_lds_floats<256>()),
