@@ -89,10 +89,10 @@ def main():
 
     wl = W.WORKLOADS[args.workload]()
     dev = torch.device("cuda:0")
-    x = W.make_signal_torch(wl, dev)
+    x = W.make_wideband_torch(wl, dev) if wl.complex else W.make_signal_torch(wl, dev)
     torch.cuda.synchronize()
 
-    st, _ = DspState.init(DspParams(complex=False, fs_in=wl.fs, fft_size=wl.fft_size,
+    st, _ = DspState.init(DspParams(complex=wl.complex, fs_in=wl.fs, fft_size=wl.fft_size,
                                     ffts_per_buf=wl.ffts_per_buf, spectrum_averages=wl.averages,
                                     max_blocks=wl.nblocks, graphs=not args.no_graphs))
     for first, L, w in wl.channels:
@@ -138,7 +138,8 @@ def main():
     chan_bytes = nb * sum(L * 8 + (3 * L // 4) * 8 for _, L, _ in wl.channels)
     path_bytes = nb * wl.bytes_per_block()
     all_fused = fused == len(wl.channels)
-    fwd_bytes = path_bytes if all_fused else nb * (H * 2 + K * 8 + K * 4 / A)
+    s_in = 8 if wl.complex else 2
+    fwd_bytes = path_bytes if all_fused else nb * (H * s_in + K * 8 + K * 4 / A)
     value = nb * H / (ms_step * 1e-3) / 1e6
     fwd_gbps = fwd_bytes / (kern["forward_ms"] * 1e-3) / 1e9
     chan_ms = kern["unfused_channels_ms"]
@@ -149,8 +150,9 @@ def main():
     traffic = traffic_from_profiles(args.workload)
     dominant = "fwd" if all_fused or kern["forward_ms"] >= chan_ms else "chan"
     roof = {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBPS,
-            "kernel": (f"fwd_real_kernel<M={wl.fft_size // 2}, FL={256 if all_fused else 0}>" if dominant == "fwd"
-                       else "chan_kernel"),
+            "kernel": (("cplx_row_kernel + cplx_col_kernel" if wl.complex else
+                        f"fwd_real_kernel<M={wl.fft_size // 2}, FL={256 if all_fused else 0}>") if dominant == "fwd"
+                       else "channel stage (" + ("batched" if tune and tune["mode"] == "batched" else "per-size") + ")"),
             "achieved": round(fwd_gbps if dominant == "fwd" else chan_gbps, 1),
             "algorithmic_bytes_per_launch": int(fwd_bytes if dominant == "fwd" else chan_bytes),
             "avg_launch_ms": round(kern["forward_ms"] if dominant == "fwd" else chan_ms, 5),
@@ -183,7 +185,10 @@ def main():
         torch.cuda.synchronize()
         out["copy_probe_bytes"] = n * 4
         del a, b
-    if args.cpu_seconds > 0:
+    if args.cpu_seconds > 0 and wl.complex:
+        out["cpu_baseline"] = None
+        out["cpu_baseline_note"] = "the CPU restatement covers s16 real input only"
+    elif args.cpu_seconds > 0:
         x_host = x.cpu().numpy()
         del x
         out["cpu_baseline"] = cpu_baseline(wl, x_host, args.cpu_seconds)

@@ -44,8 +44,8 @@ typedef enum spk_status {
  * the frequency<->bin helpers below need, and minus the filter list, which
  * is added with spk_add_channel. */
 typedef struct spk_config {
-  uint32_t fft_size;          /* N: 4096..65536 (real input) */
-  uint32_t input_format;      /* SPK_FMT_S16LE; SPK_FMT_CF32LE is reserved */
+  uint32_t fft_size;          /* N: 4096..65536 (s16 real input), 4096..262144 (cf32 complex input) */
+  uint32_t input_format;      /* SPK_FMT_S16LE or SPK_FMT_CF32LE; other DataFormat codes: SPK_ERR_UNSUPPORTED */
   float scaling;              /* input scaling; 0 = input_format_scaling(fmt) */
   uint32_t ffts_per_buf;      /* F: blocks per spk_process call */
   uint32_t spectrum_averages; /* A: blocks per Welch record */

@@ -1,0 +1,183 @@
+// Complex-input path (cf32le): windowed complex forward FFT of N points as
+// two passes through HBM, Welch accumulated directly from the spectra.
+// spektri/src/dsp.rs:114-136 (process_complex), spektri/src/dsp/multifft.rs:11-47
+// (apply_window + FFT), spektri/src/dsp/spectrum.rs:82-116 (Welch).
+//
+// N = 64 N1, n = 64 n1 + n2, k = k1 + N1 k2:
+//   X[k1 + N1 k2] = sum_{n2} W64^{n2 k2} W_N^{n2 k1} [ sum_{n1} x[64 n1 + n2] W_{N1}^{n1 k1} ]
+// cplx_row_kernel: for every (block, n2) the N1-point FFT over n1 (strided
+// input, read through the L2 of one XCD per block), times W_N^{n2 k1}, stored
+// as rows Y[n2][k1].  cplx_col_kernel: for every k1 the 64-point DFT over
+// n2, stored in natural bin order.
+#pragma once
+#include "kernels.h"
+
+namespace spk {
+
+struct CplxRowParams {
+  const cpx* src;   // stream samples; sample s >= 0 at src[s]
+  const cpx* hist;  // the O samples before src[0]
+  int H, O;
+  float wscale;
+  int nblocks;
+  cpx* Y;  // [block][64][N1]
+  const cpx* tw2;   // forward-sign pass tables of the N1 plan
+  const cpx* tw3;
+  const cpx* ta;    // W_N^{n2 a}, [64][64]
+  const cpx* tb;    // W_N^{64 n2 b}, [64][N1/64]
+};
+
+// Workgroup w serves XCD w mod 8; block g goes to XCD g mod 8, so the 64
+// row transforms of one block (which read interleaved samples of the same
+// cache lines) share one L2.  Grid: 8 * ceil(nblocks / 8) * (64 / TPW).
+template <int N1, int R, int R1, int R2, int R3>
+__global__ __launch_bounds__(kChanThreads) void cplx_row_kernel(CplxRowParams P) {
+  constexpr int T = N1 / R;
+  static_assert(R1 * R2 * R3 == N1, "radix plan must cover N1");
+  static_assert(T <= kChanThreads && kChanThreads % T == 0, "tile must divide the workgroup");
+  constexpr int TPW = kChanThreads / T;
+  static_assert(64 % TPW == 0, "row tiles divide the 64 rows");
+  constexpr int TILES = 64 / TPW;  // workgroups per block
+  constexpr int LP = lds_padded_size(N1);
+  __shared__ float lds_all[TPW * LP * 2];
+
+  const int x = (int)blockIdx.x % 8, r = (int)blockIdx.x / 8;
+  const int g = 8 * (r / TILES) + x;
+  if (g >= P.nblocks) return;  // whole workgroup
+  const int tl = threadIdx.x / T, t = threadIdx.x % T;
+  const int n2 = (r % TILES) * TPW + tl;
+  float* lds = lds_all + tl * LP * 2;
+
+  const long long s0 = (long long)g * P.H - P.O;  // stream index of sample 0 of the block
+  cpx v[R];
+#pragma unroll
+  for (int e = 0; e < R; ++e) {
+    const int n1 = st_in_index<N1, R, T, R1>(t, e);
+    const long long si = s0 + 64LL * n1 + n2;
+    const cpx a = si < 0 ? P.hist[P.O + si] : P.src[si];
+    v[e] = cscale(a, P.wscale);
+  }
+  st_butterflies<N1, R, T, R1, 1, -1>(v, t, nullptr);
+  lds_exchange<R, false>(
+      v, lds, [&](int e) { return st_out_addr<N1, R, T, R1, 1>(t, e); },
+      [&](int e) { return st_in_addr<N1, R, T, R2>(t, e); });
+  st_butterflies<N1, R, T, R2, R1, -1>(v, t, P.tw2);
+  constexpr int RL = (R3 > 1) ? R3 : R2;
+  if constexpr (R3 > 1) {
+    lds_exchange<R, false>(
+        v, lds, [&](int e) { return st_out_addr<N1, R, T, R2, R1>(t, e); },
+        [&](int e) { return st_in_addr<N1, R, T, R3>(t, e); });
+    st_butterflies<N1, R, T, R3, R1 * R2, -1>(v, t, P.tw3);
+  }
+  constexpr int PL = N1 / RL;
+  cpx* y = P.Y + ((long long)g * 64 + n2) * N1;
+  const cpx* ta = P.ta + n2 * 64;
+  const cpx* tb = P.tb + n2 * (N1 / 64 > 0 ? N1 / 64 : 1);
+#pragma unroll
+  for (int e = 0; e < R; ++e) {
+    const int u = e / RL, q = e % RL;
+    const int k1 = t + T * u + q * PL;
+    y[k1] = cmul(v[e], cmul(ta[k1 & 63], tb[k1 >> 6]));
+  }
+}
+
+struct CplxColParams {
+  const cpx* Y;  // [block][64][N1]
+  int N1;
+  int nblocks;
+  cpx* spec;
+  long long spec_stride;
+  const cpx* tw64;  // W64^m, m < 64 (forward sign)
+};
+
+// 64 columns k1 per 256-thread workgroup, 4 threads per column: thread
+// (c, s) takes rows n2 = s + 4 j, j < 16, runs the 16-point DFT over j,
+// twiddles by W64^{s q}, and after an LDS exchange each thread finishes 4
+// of the 16 q values with 4-point DFTs over s: X[k1 + N1 (q + 16 p)].
+// Grid: nblocks * N1 / 64.
+__global__ __launch_bounds__(kChanThreads) void cplx_col_kernel(CplxColParams P) {
+  __shared__ cpx lds[64 * 16 * 4 + 64];  // [c][q][s], padded per column
+  const int tiles = P.N1 / 64;
+  const int g = (int)blockIdx.x / tiles, k1_0 = ((int)blockIdx.x % tiles) * 64;
+  const int c = threadIdx.x & 63, s = threadIdx.x >> 6;
+  const cpx* y = P.Y + (long long)g * 64 * P.N1 + k1_0 + c;
+  cpx z[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) z[j] = y[(long long)(s + 4 * j) * P.N1];
+  dft<16, -1>(z);
+#pragma unroll
+  for (int q = 1; q < 16; ++q) z[q] = cmul(z[q], P.tw64[(s * q) & 63]);
+  auto la = [](int cc, int q, int ss) { return cc * 65 + q * 4 + ss; };
+#pragma unroll
+  for (int q = 0; q < 16; ++q) lds[la(c, q, s)] = z[q];
+  __syncthreads();
+  cpx* out = P.spec + (long long)g * P.spec_stride + k1_0 + c;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int q = 4 * s + i;
+    cpx b[4];
+#pragma unroll
+    for (int ss = 0; ss < 4; ++ss) b[ss] = lds[la(c, q, ss)];
+    dft<4, -1>(b);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) out[(long long)(q + 16 * p) * P.N1] = b[p];
+  }
+}
+
+// Welch records straight from the spectra (complex input), spectrum.rs:95-116:
+// for each group of A blocks, sum over its blocks of
+// |X[k] - (X[k-1] + X[k+1]) / 2|^2 with get_bin neighbours.  Grid: x over
+// bins, y over the groups the launch touches (rows past the count exit).
+// Row 0 also saves the next call's input history and resets the batched
+// channel queue counters.
+struct WelchDirectParams {
+  const cpx* spec;
+  long long spec_stride;
+  int N, K, real_input;
+  int nblocks, A;
+  const long long* pos;
+  float* carry;
+  float* out;
+  const int* hist_src;
+  int* hist_dst;
+  int hist_words;
+  int* chan_counters;
+  int ncounters;
+};
+
+__device__ __forceinline__ cpx bin_at(const cpx* X, int N, int K, int real_input, int i) {
+  int m = i % N;
+  if (m < 0) m += N;
+  if (real_input && m >= K) {
+    const cpx v = X[N - m];
+    return {v.x, -v.y};
+  }
+  return X[m];
+}
+
+__global__ void welch_direct_kernel(WelchDirectParams F) {
+  const int gi = (int)blockIdx.y;
+  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (gi == 0) {
+    for (int i = k; i < F.hist_words; i += (int)(gridDim.x * blockDim.x)) F.hist_dst[i] = F.hist_src[i];
+    if (k < F.ncounters) F.chan_counters[k] = 0;
+  }
+  const int blk0_mod = (int)(*F.pos % F.A);
+  if (k >= F.K || gi > (blk0_mod + F.nblocks - 1) / F.A) return;
+  const int g_lo = gi * F.A - blk0_mod, g_hi = g_lo + F.A;
+  const int b_lo = g_lo > 0 ? g_lo : 0;
+  const int b_hi = g_hi < F.nblocks ? g_hi : F.nblocks;
+  float sum = (gi == 0 && blk0_mod != 0) ? F.carry[k] : 0.f;
+  for (int b = b_lo; b < b_hi; ++b) {
+    const cpx* X = F.spec + (long long)b * F.spec_stride;
+    sum += welch_term(bin_at(X, F.N, F.K, F.real_input, k), bin_at(X, F.N, F.K, F.real_input, k - 1),
+                      bin_at(X, F.N, F.K, F.real_input, k + 1));
+  }
+  if (g_hi <= F.nblocks) {
+    F.out[(long long)gi * F.K + k] = sum / (float)F.A;
+  } else {
+    F.carry[k] = sum;
+  }
+}
+
+}  // namespace spk

@@ -19,6 +19,7 @@
 
 #include "spektri_gpu.h"
 #include "kernels.h"
+#include "cplx.h"
 
 using spk::cpx;
 
@@ -109,6 +110,17 @@ std::vector<cpx> pass_table(int r, int p, int sign) {
   return t;
 }
 
+// DataFormat codes of spektri/src/dsp/data.rs:81-95.
+bool is_reference_format(uint32_t fmt) {
+  switch (fmt) {
+    case 0x04: case 0x08: case 0x09: case 0x1C: case 0x1D: case 0x24:
+    case 0x44: case 0x48: case 0x49: case 0x5C: case 0x5D: case 0x64:
+      return true;
+    default:
+      return false;
+  }
+}
+
 int bytes_per_sample(uint32_t fmt) { return fmt == SPK_FMT_S16LE ? 2 : (fmt == SPK_FMT_CF32LE ? 8 : 0); }
 
 }  // namespace
@@ -141,6 +153,9 @@ struct spk_ctx {
   int fscat_nyq = 0;
   DevBuf ftw2, ftw3, ftwu;
   DevBuf eitems;             // Welch items of the edge bins (spk::edge_items)
+  // complex input: row-transform scratch and tables (N = 64 N1)
+  DevBuf cY, ctw2, ctw3, cta, ctb, ctw64;
+  const ChanPlan* cplan = nullptr;
   DevBuf adj_part;           // adjoint per-split weight sums
   int ncu = 0;
   // channel stage launch form and batched tuning (chunk blocks, rounds per item, workgroups per CU)
@@ -253,7 +268,7 @@ spk_status rebuild_channels(spk_ctx* c) {
   // The forward kernel runs the L = 256 channels itself when one 16-thread
   // tile per channel fits in its workgroup.
   const auto it = c->groups.find(kFusedL);
-  c->fused = it != c->groups.end() && (int)it->second.ids.size() * 16 <= c->M / c->fplan->R;
+  c->fused = !c->complex_in && it != c->groups.end() && (int)it->second.ids.size() * 16 <= c->M / c->fplan->R;
   if (c->fused) {
     // The fused stage takes channels inside bins [0, K) with no bin shared
     // (get_bin mirrors and overlapping channels go to the channel kernel).
@@ -528,7 +543,114 @@ spk_status launch_batched(spk_ctx* c, int nb, hipStream_t s) {
 // forward kernel, channel stage, Welch finaliser, stream-position advance.
 // Kernels only (no memset or copy nodes), so it can be captured into a
 // graph; every per-call quantity the kernels need is in device memory.
+template <int N1, int R, int R1, int R2, int R3>
+void launch_cplx_rows(const spk::CplxRowParams& P, int nb, hipStream_t s) {
+  constexpr int TPW = spk::kChanThreads / (N1 / R);
+  const int grid = 8 * ((nb + 7) / 8) * (64 / TPW);
+  hipLaunchKernelGGL((spk::cplx_row_kernel<N1, R, R1, R2, R3>), dim3(grid), dim3(spk::kChanThreads), 0, s, P);
+}
+
+hipError_t dispatch_cplx_rows(int N1, const spk::CplxRowParams& P, int nb, hipStream_t s) {
+  switch (N1) {
+    case 64: launch_cplx_rows<64, 8, 8, 8, 1>(P, nb, s); break;
+    case 128: launch_cplx_rows<128, 16, 16, 8, 1>(P, nb, s); break;
+    case 256: launch_cplx_rows<256, 16, 16, 16, 1>(P, nb, s); break;
+    case 512: launch_cplx_rows<512, 32, 32, 16, 1>(P, nb, s); break;
+    case 1024: launch_cplx_rows<1024, 32, 32, 32, 1>(P, nb, s); break;
+    case 2048: launch_cplx_rows<2048, 32, 32, 32, 2>(P, nb, s); break;
+    case 4096: launch_cplx_rows<4096, 16, 16, 16, 16>(P, nb, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+spk_status enqueue_channels(spk_ctx* c, int nb, hipStream_t s, const ChanGroup* fg);
+
+// Complex input: row transforms, column transforms (natural-order spectra),
+// Welch records, channel stage, stream-position advance.
+spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, hipEvent_t* ev) {
+  const int N1 = c->N / 64;
+  if (ev) HIP_TRY(c, hipEventRecord(ev[0], s));
+  spk::CplxRowParams P{};
+  P.src = reinterpret_cast<const cpx*>(src);
+  P.hist = c->hist.as<cpx>();
+  P.H = c->H;
+  P.O = c->O;
+  P.wscale = c->wscale;
+  P.nblocks = nb;
+  P.Y = c->cY.as<cpx>();
+  P.tw2 = c->ctw2.as<cpx>();
+  P.tw3 = c->ctw3.as<cpx>();
+  P.ta = c->cta.as<cpx>();
+  P.tb = c->ctb.as<cpx>();
+  HIP_TRY(c, dispatch_cplx_rows(N1, P, nb, s));
+  spk::CplxColParams Q{};
+  Q.Y = c->cY.as<cpx>();
+  Q.N1 = N1;
+  Q.nblocks = nb;
+  Q.spec = c->spec.as<cpx>();
+  Q.spec_stride = c->spec_stride;
+  Q.tw64 = c->ctw64.as<cpx>();
+  hipLaunchKernelGGL(spk::cplx_col_kernel, dim3(nb * (N1 / 64)), dim3(spk::kChanThreads), 0, s, Q);
+  HIP_TRY(c, hipGetLastError());
+  if (ev) HIP_TRY(c, hipEventRecord(ev[1], s));
+  spk_status st = enqueue_channels(c, nb, s, nullptr);
+  if (st != SPK_OK) return st;
+  if (ev) HIP_TRY(c, hipEventRecord(ev[2], s));
+  {
+    spk::WelchDirectParams W{};
+    W.spec = c->spec.as<cpx>();
+    W.spec_stride = c->spec_stride;
+    W.N = c->N;
+    W.K = c->K;
+    W.real_input = 0;
+    W.nblocks = nb;
+    W.A = c->A;
+    W.pos = c->pos.as<long long>();
+    W.carry = c->carry.as<float>();
+    W.out = c->welch_out.as<float>();
+    W.hist_src = reinterpret_cast<const int*>(reinterpret_cast<const char*>(src) +
+                                              ((long long)nb * c->H - c->O) * c->bps);
+    W.hist_dst = c->hist.as<int>();
+    W.hist_words = c->O * c->bps / 4;
+    W.chan_counters = c->chan_counters.as<int>();
+    W.ncounters = kXcdQueues;
+    const int gmax = (c->A - 1 + nb - 1) / c->A + 1;
+    hipLaunchKernelGGL(spk::welch_direct_kernel, dim3((c->K + 255) / 256, gmax), dim3(256), 0, s, W);
+    HIP_TRY(c, hipGetLastError());
+  }
+  hipLaunchKernelGGL(spk::advance_kernel, dim3(1), dim3(64), 0, s, c->pos.as<long long>(), nb);
+  HIP_TRY(c, hipGetLastError());
+  if (ev) HIP_TRY(c, hipEventRecord(ev[3], s));
+  return SPK_OK;
+}
+
+// Channel stage for every channel not fused into the forward kernel.
+spk_status enqueue_channels(spk_ctx* c, int nb, hipStream_t s, const ChanGroup* fg) {
+  if (c->chan_mode == SPK_CHAN_BATCHED) return launch_batched(c, nb, s);
+  for (auto& kv : c->groups) {
+    ChanGroup& g = kv.second;
+    if (&g == fg) continue;
+    spk::ChanParams Q{};
+    Q.spec = c->spec.as<cpx>();
+    Q.spec_stride = c->spec_stride;
+    Q.N = c->N;
+    Q.K = c->K;
+    Q.real_input = c->complex_in ? 0 : 1;
+    Q.weights = c->weights.as<float>();
+    Q.chans = g.desc.as<spk::ChanDesc>();
+    Q.nch = (int)g.ids.size();
+    Q.nblocks = nb;
+    Q.out = c->chan_out.as<cpx>();
+    Q.tw2 = g.tw2.as<cpx>();
+    Q.tw3 = g.tw3.as<cpx>();
+    HIP_TRY(c, dispatch_chan(g.L, Q, s));
+  }
+  return SPK_OK;
+}
+
 spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hipEvent_t* ev) {
+  if (c->complex_in) return enqueue_complex(c, src, nb, s, ev);
   spk::FwdParams P{};
   P.src = reinterpret_cast<const int16_t*>(src);
   P.hist = c->hist.as<int16_t>();
@@ -569,27 +691,9 @@ spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hi
   HIP_TRY(c, fv->launch(P, G, s));
   if (ev) HIP_TRY(c, hipEventRecord(ev[1], s));
 
-  if (c->chan_mode == SPK_CHAN_BATCHED) {
-    spk_status st = launch_batched(c, nb, s);
+  {
+    spk_status st = enqueue_channels(c, nb, s, fg);
     if (st != SPK_OK) return st;
-  }
-  for (auto& kv : c->groups) {
-    ChanGroup& g = kv.second;
-    if (&g == fg || c->chan_mode == SPK_CHAN_BATCHED) continue;
-    spk::ChanParams Q{};
-    Q.spec = c->spec.as<cpx>();
-    Q.spec_stride = c->spec_stride;
-    Q.N = c->N;
-    Q.K = c->K;
-    Q.real_input = c->complex_in ? 0 : 1;
-    Q.weights = c->weights.as<float>();
-    Q.chans = g.desc.as<spk::ChanDesc>();
-    Q.nch = (int)g.ids.size();
-    Q.nblocks = nb;
-    Q.out = c->chan_out.as<cpx>();
-    Q.tw2 = g.tw2.as<cpx>();
-    Q.tw3 = g.tw3.as<cpx>();
-    HIP_TRY(c, dispatch_chan(g.L, Q, s));
   }
   if (ev) HIP_TRY(c, hipEventRecord(ev[2], s));
 
@@ -705,29 +809,34 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   if (!c) return SPK_ERR_DEVICE;
   c->cfg = *cfg;
   const uint32_t N = cfg->fft_size;
-  if (cfg->input_format == SPK_FMT_CF32LE) {
-    delete c;
-    return SPK_ERR_UNSUPPORTED;
-  }
-  if (cfg->input_format != SPK_FMT_S16LE || N == 0 || (N & (N - 1)) != 0 || !find_fwd_plan((int)N / 2) ||
-      cfg->ffts_per_buf == 0 || cfg->spectrum_averages == 0) {
+  const bool cplx = cfg->input_format == SPK_FMT_CF32LE;
+  if (N == 0 || (N & (N - 1)) != 0 || cfg->ffts_per_buf == 0 || cfg->spectrum_averages == 0 ||
+      !is_reference_format(cfg->input_format)) {
     delete c;
     return SPK_ERR_INVALID;
   }
+  // formats and sizes the reference accepts but this library does not implement
+  if ((cfg->input_format != SPK_FMT_S16LE && !cplx) ||
+      !(cplx ? (N >= 4096 && N <= 262144) : find_fwd_plan((int)N / 2) != nullptr)) {
+    delete c;
+    return SPK_ERR_UNSUPPORTED;
+  }
   c->N = (int)N;
   c->M = c->N / 2;
-  c->complex_in = false;
-  c->K = c->N / 2 + 1;
+  c->complex_in = cplx;
+  c->K = cplx ? c->N : c->N / 2 + 1;  // dsp.rs:79
   c->O = c->N / 4;
   c->H = c->N - c->O;
   c->F = (int)cfg->ffts_per_buf;
   c->A = (int)cfg->spectrum_averages;
   c->bps = bytes_per_sample(cfg->input_format);
-  const float scaling = cfg->scaling != 0.f ? cfg->scaling : 1.0f / 32767.0f;
+  // input_format_scaling (inputformats.rs:47-62): 1/32767 for s16, 1 for f32
+  const float scaling = cfg->scaling != 0.f ? cfg->scaling : (cplx ? 1.0f : 1.0f / 32767.0f);
   c->wscale = scaling / (float)c->N;  // dsp.rs:70-73
   c->max_blocks = (int)(cfg->max_blocks ? cfg->max_blocks : cfg->ffts_per_buf);
   if (c->max_blocks < c->F) c->max_blocks = c->F;
-  c->fplan = find_fwd_plan(c->M);
+  c->fplan = cplx ? nullptr : find_fwd_plan(c->M);
+  c->cplan = cplx ? find_chan_plan(c->N / 64) : nullptr;
 
   auto cleanup = [&](spk_status s) {
     spk_destroy(c);
@@ -736,13 +845,54 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) return cleanup(SPK_ERR_DEVICE);
   c->stream = c->own_stream;
 
-  c->keep_spec = (cfg->flags & SPK_FLAG_KEEP_SPECTRA) != 0;
+  // complex input always keeps the spectra (the channel stage reads them)
+  c->keep_spec = c->complex_in || (cfg->flags & SPK_FLAG_KEEP_SPECTRA) != 0;
   // Persistent forward grid: resident workgroups per CU times CUs.
   int dev = 0, ncu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
     return cleanup(SPK_ERR_DEVICE);
   c->ncu = ncu;
+  c->spec_stride = ((long long)c->K + 63) / 64 * 64;
+  c->welch_out_cap = c->max_blocks / c->A + 1;
+  if (c->chan_counters.alloc(kXcdQueues * sizeof(int)) != hipSuccess ||
+      hipMemset(c->chan_counters.p, 0, c->chan_counters.bytes) != hipSuccess ||
+      c->pos.alloc(sizeof(long long)) != hipSuccess || hipMemset(c->pos.p, 0, c->pos.bytes) != hipSuccess)
+    return cleanup(SPK_ERR_DEVICE);
+  c->use_graphs = (cfg->flags & SPK_FLAG_GRAPHS) != 0;
+  if (c->complex_in) {
+    // every spectrum row is kept: the channel stage always reads them
+    if ((long long)(c->max_blocks + c->A) * 64 >= (1LL << 31)) return cleanup(SPK_ERR_UNSUPPORTED);
+    const ChanPlan& p = *c->cplan;
+    const int N1 = c->N / 64;
+    if (c->hist.alloc((size_t)c->O * c->bps) != hipSuccess ||
+        c->staging.alloc((size_t)c->F * c->H * c->bps) != hipSuccess ||
+        c->spec.alloc((size_t)c->max_blocks * c->spec_stride * sizeof(cpx)) != hipSuccess ||
+        c->cY.alloc((size_t)c->max_blocks * c->N * sizeof(cpx)) != hipSuccess ||
+        c->carry.alloc((size_t)c->K * sizeof(float)) != hipSuccess ||
+        c->welch_out.alloc((size_t)c->welch_out_cap * c->K * sizeof(float)) != hipSuccess)
+      return cleanup(SPK_ERR_DEVICE);
+    if (hipMemset(c->hist.p, 0, c->hist.bytes) != hipSuccess ||
+        hipMemset(c->carry.p, 0, c->carry.bytes) != hipSuccess)
+      return cleanup(SPK_ERR_DEVICE);
+    // W_N^m with m reduced exactly, forward sign
+    auto wn = [&](long long m, long long n) {
+      const double a = 2.0 * M_PI * (double)(m % n) / (double)n;
+      return cpx{(float)std::cos(a), (float)(-std::sin(a))};
+    };
+    std::vector<cpx> ta(64 * 64), tb((size_t)64 * std::max(1, N1 / 64)), t64(64);
+    for (int n2 = 0; n2 < 64; ++n2) {
+      for (int a = 0; a < 64; ++a) ta[(size_t)n2 * 64 + a] = wn((long long)n2 * a, c->N);
+      for (int b = 0; b < std::max(1, N1 / 64); ++b) tb[(size_t)n2 * std::max(1, N1 / 64) + b] = wn(64LL * n2 * b, c->N);
+    }
+    for (int m = 0; m < 64; ++m) t64[m] = wn(m, 64);
+    if (upload(c->ctw2, pass_table(p.R2, p.R1, -1)) != hipSuccess ||
+        upload(c->ctw3, pass_table(p.R3, p.R1 * p.R2, -1)) != hipSuccess || upload(c->cta, ta) != hipSuccess ||
+        upload(c->ctb, tb) != hipSuccess || upload(c->ctw64, t64) != hipSuccess)
+      return cleanup(SPK_ERR_DEVICE);
+    *out = c;
+    return SPK_OK;
+  }
   for (int f = 0; f < 2; ++f) {
     const FwdVariant* v = find_fwd_variant(c->M, f ? kFusedL : 0);
     const int per_cu = v ? v->occupancy() : 0;
@@ -756,9 +906,7 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   // A workgroup spans at most ceil(ceil(max_blocks / G) / A) + 1 Welch groups.
   const int bpw = (c->max_blocks + gmin - 1) / gmin;
   c->wslots = (bpw + c->A - 1) / c->A + 1;
-  c->spec_stride = ((long long)c->K + 63) / 64 * 64;
   c->wstride = c->spec_stride;
-  c->welch_out_cap = c->max_blocks / c->A + 1;
   if (c->hist.alloc((size_t)c->O * c->bps) != hipSuccess ||
       c->staging.alloc((size_t)c->F * c->H * c->bps) != hipSuccess ||
       c->spec.alloc((size_t)std::max(c->max_blocks, gcap) * c->spec_stride * sizeof(cpx)) != hipSuccess ||
@@ -768,11 +916,6 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
       c->carry.alloc((size_t)c->K * sizeof(float)) != hipSuccess ||
       c->welch_out.alloc((size_t)c->welch_out_cap * c->K * sizeof(float)) != hipSuccess)
     return cleanup(SPK_ERR_DEVICE);
-  if (c->chan_counters.alloc(kXcdQueues * sizeof(int)) != hipSuccess ||
-      hipMemset(c->chan_counters.p, 0, c->chan_counters.bytes) != hipSuccess ||
-      c->pos.alloc(sizeof(long long)) != hipSuccess || hipMemset(c->pos.p, 0, c->pos.bytes) != hipSuccess)
-    return cleanup(SPK_ERR_DEVICE);
-  c->use_graphs = (cfg->flags & SPK_FLAG_GRAPHS) != 0;
   if (hipMemset(c->hist.p, 0, c->hist.bytes) != hipSuccess || hipMemset(c->carry.p, 0, c->carry.bytes) != hipSuccess)
     return cleanup(SPK_ERR_DEVICE);
 

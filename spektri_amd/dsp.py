@@ -103,7 +103,10 @@ class DspState:
         """dsp.rs:76-112: returns (state, InputBufferSize)."""
         lib = _lib.load()
         flags = (_lib.FLAG_KEEP_SPECTRA if params.keep_spectra else 0) | (_lib.FLAG_GRAPHS if params.graphs else 0)
-        cfg = _lib.SpkConfig(params.fft_size, params.input_format, params.scaling,
+        fmt = params.input_format
+        if params.complex and fmt == _lib.FMT_S16LE:
+            fmt = _lib.FMT_CF32LE  # the complex format this library supports
+        cfg = _lib.SpkConfig(params.fft_size, fmt, params.scaling,
                              params.ffts_per_buf, params.spectrum_averages, params.max_blocks, flags)
         ctx = ctypes.c_void_p()
         _lib.check(lib.spk_create(ctypes.byref(cfg), ctypes.byref(ctx)))

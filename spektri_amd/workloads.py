@@ -70,6 +70,8 @@ class Workload:
     noise_sigma: float = 0.0
     complex: bool = False
     tone_kind: list = field(default_factory=list)  # per tone, for the weight-design loss
+    carriers: list = field(default_factory=list)   # (fc Hz, width Hz, dBFS), complex workloads
+    noise_dbfs_per_bin: float = 0.0
 
     @property
     def hop(self):
@@ -156,6 +158,73 @@ def uneven_64k(nsamples: int | None = None) -> Workload:
     return wl
 
 
+def wideband_256k(nsamples: int | None = None) -> Workload:
+    """Complex wideband channelizer: cf32 input, 2^27 complex samples,
+    N = 2^18 complex FFT (fs 262.144 MHz, 1 kHz bins) with 25 % overlap,
+    F = 2 (682 blocks), Welch over 8 blocks, 2000 channels with IFFT sizes
+    drawn uniformly from {64, 256, 1024, 8192} at seeded round4 centres
+    (channels may wrap around the spectrum edge), raised-cosine weights with
+    20 % transition.  Signal: 500 QPSK-like carriers (5-200 kHz wide,
+    -70..-20 dBFS) plus complex noise at -90 dBFS per bin; see
+    make_wideband_torch."""
+    seed = 3
+    rng = np.random.default_rng(seed)
+    N, fs = 1 << 18, 262.144e6
+    wl = Workload("wideband_256k", seed, N, fs, 2, 8, nsamples or (1 << 27), complex=True)
+    for L in rng.choice([64, 256, 1024, 8192], size=2000):
+        c = int(round4(rng.uniform(0, N)))
+        wl.channels.append((c - int(L) // 2, int(L), raised_cosine_transition(int(L), 0.2)))
+    wl.carriers = [(float(rng.uniform(-0.5, 0.5) * fs), float(rng.uniform(5e3, 200e3)),
+                    float(rng.uniform(-70, -20))) for _ in range(500)]
+    wl.noise_dbfs_per_bin = -90.0
+    return wl
+
+
+def make_wideband_torch(wl: Workload, device, chunk: int = 1 << 20):
+    """cf32 stream of ``wideband_256k`` as a [S, 2] float32 tensor.
+
+    Each chunk of ``chunk`` samples is synthesised in the frequency domain:
+    every carrier occupies the bins of its band (centre fc, width B) with
+    seeded random-phase values shaped by a root-raised-cosine magnitude
+    (roll-off 0.35), scaled to the carrier's power in dBFS (a full-scale
+    complex tone = 0 dBFS); complex Gaussian noise is added with a per-bin
+    power of ``noise_dbfs_per_bin`` at the N-point analysis FFT.  The chunk
+    is brought to the time domain by an inverse FFT; chunks are independent
+    (the QPSK-like carriers are random processes, so nothing is lost at the
+    seams for testing).  Deterministic for a device and seed."""
+    import torch
+    S = wl.nsamples
+    out = torch.empty(S, 2, dtype=torch.float32, device=device)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(1000 + wl.seed)
+    df = wl.fs / chunk
+    f = torch.fft.fftfreq(chunk, d=1.0 / wl.fs, device=device, dtype=torch.float64)
+    shape = torch.zeros(chunk, dtype=torch.float64, device=device)
+    beta = 0.35
+    for fc, bw, db in wl.carriers:
+        rs = bw / (1 + beta)
+        u = torch.abs(f - fc) / rs  # in symbol rates
+        lo, hi = (1 - beta) / 2, (1 + beta) / 2
+        h = torch.where(u <= lo, torch.ones_like(u),
+                        torch.where(u <= hi, torch.sqrt(0.5 * (1 + torch.cos(np.pi / beta * (u - lo)))),
+                                    torch.zeros_like(u)))
+        nb = max(1.0, float((h * h).sum()))
+        # power 10^(db/10) spread over the band: per-bin |X|^2 share, X = chunk * amplitude
+        shape += (10 ** (db / 10)) * (h * h) / nb
+    amp_bins = torch.sqrt(shape) * chunk
+    noise_sigma = 10 ** (wl.noise_dbfs_per_bin / 20) * np.sqrt(wl.fft_size)
+    for a in range(0, S, chunk):
+        b = min(S, a + chunk)
+        ph = torch.rand(chunk, generator=gen, device=device, dtype=torch.float64) * (2 * np.pi)
+        spec = amp_bins * torch.exp(1j * ph)
+        x = torch.fft.ifft(spec)[: b - a]
+        x += (torch.randn(b - a, generator=gen, device=device, dtype=torch.float64)
+              + 1j * torch.randn(b - a, generator=gen, device=device, dtype=torch.float64)) * (noise_sigma / np.sqrt(2))
+        out[a:b, 0] = x.real.to(torch.float32)
+        out[a:b, 1] = x.imag.to(torch.float32)
+    return out
+
+
 def design_16k(nsamples: int | None = None) -> Workload:
     """Differentiable weight design: real s16, N = 16384 (fs 16.384 MHz,
     1 kHz bins), 256 blocks at F = 8, 256 channels of L = 256 at
@@ -184,7 +253,8 @@ def design_16k(nsamples: int | None = None) -> Workload:
     return wl
 
 
-WORKLOADS = {"plain_8k": plain_8k, "rx888_64k": rx888_64k, "uneven_64k": uneven_64k, "design_16k": design_16k}
+WORKLOADS = {"plain_8k": plain_8k, "rx888_64k": rx888_64k, "uneven_64k": uneven_64k, "wideband_256k": wideband_256k,
+             "design_16k": design_16k}
 
 
 def _deterministic_part_np(wl: Workload, n0: int, n1: int) -> np.ndarray:

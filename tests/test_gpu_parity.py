@@ -357,3 +357,94 @@ def test_graph_replay_matches_eager_across_calls(batched):
     e_w = rms(got, recs) if recs.size else 0.0
     report(f"graph_replay_batched{int(batched)}", calls=calls, identical_to_eager=same, welch_rms=e_w)
     assert same and got.shape == recs.shape and e_w < TOL
+
+
+def run_complex(x_c64: np.ndarray, N, F, A, channels, batched=False, graphs=False):
+    """Complex (cf32) stream through the GPU path; returns (state, ids)."""
+    st, _ = DspState.init(DspParams(complex=True, fs_in=1.0, fft_size=N, ffts_per_buf=F, spectrum_averages=A,
+                                    max_blocks=len(x_c64) // (N - N // 4), graphs=graphs))
+    ids = [st.add_channel(first, L, w) for first, L, w in channels]
+    st.set_channel_mode(batched)
+    dev = torch.from_numpy(np.ascontiguousarray(x_c64).view(np.float32)).to("cuda")
+    st.process_device(dev.data_ptr(), len(x_c64))
+    st.synchronize()
+    return st, ids
+
+
+@pytest.mark.parametrize("N", [4096, 16384, 65536, 262144])
+def test_complex_input_every_size(N):
+    """cf32 input: spectra, Welch records (A straddling the call) and channels
+    of every IFFT size, including channels wrapping around the spectrum edge."""
+    rng = np.random.default_rng(31)
+    F, A = 2, 3
+    H = N - N // 4
+    nb = 4
+    x = ((rng.standard_normal(nb * H) + 1j * rng.standard_normal(nb * H)) * 0.05
+         + 0.3 * np.exp(2j * np.pi * 0.0123 * np.arange(nb * H))).astype(np.complex64)
+    sizes = [L for L in (8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192) if L <= N]
+    chans = []
+    for i, L in enumerate(sizes + sizes[::-1]):
+        first = int(rng.integers(-L, N)) // 4 * 4 if i % 3 else N - L // 2  # some wrap past N
+        chans.append((first, L, rng.random(L).astype(np.float32)))
+    errs = {}
+    for batched in (False, True):
+        st, ids = run_complex(x, N, F, A, chans, batched=batched)
+        geom = O.Geometry(N, True, F)
+        bl = O.frame_blocks(x.astype(np.complex128), geom, nb)
+        X = O.forward_spectra(bl, geom, np.float32(1.0))
+        acc = O.WelchAccumulator(N, N, A)
+        recs = np.array(acc.accumulate(X))
+        p, stride = st.spectra_device()
+        Xg = d2h(p, nb * stride * 8).view(np.complex64).reshape(nb, stride)[:, :N]
+        rg = st.spectrum_records()
+        e_ch = max(rms(st.channel_output(i), O.channel_process(X, N, f, np.asarray(w, np.float64)))
+                   for i, (f, L, w) in zip(ids, chans))
+        errs[batched] = dict(spectra=rms(Xg, X), welch=rms(rg, recs) / max(1e-30, float(np.abs(recs).max())),
+                             channels=e_ch, records=len(rg))
+        st.close()
+        assert rg.shape == recs.shape
+    report(f"complex_N{N}", **{("batched_" if b else "per_size_") + k: v for b, d in errs.items() for k, v in d.items()})
+    for d in errs.values():
+        assert d["spectra"] < TOL and d["channels"] < TOL and d["welch"] < 1e-5
+
+
+def test_wideband_256k_sampled_blocks():
+    """The 2000-channel complex wideband set at full size (2^27 cf32 samples,
+    N = 2^18): sampled blocks and channels against the oracle, and the first
+    Welch record."""
+    wl = W.wideband_256k()
+    dev = W.make_wideband_torch(wl, "cuda")
+    N, F, A = wl.fft_size, wl.ffts_per_buf, wl.averages
+    st, _ = DspState.init(DspParams(complex=True, fs_in=wl.fs, fft_size=N, ffts_per_buf=F, spectrum_averages=A,
+                                    max_blocks=wl.nblocks))
+    ids = [st.add_channel(first, L, w) for first, L, w in wl.channels]
+    st.process_device(dev.data_ptr(), wl.nsamples)
+    st.synchronize()
+    nb = st.last_blocks()
+    assert nb == wl.nblocks
+    recs_g = st.spectrum_records()
+    geom = O.Geometry(N, True, F)
+    sample_ch = ids[::97]
+    outs = {i: st.channel_output(i) for i in sample_ch}
+    # host copy of the first A blocks' samples and of a few later blocks
+    H, Ov = geom.hop, geom.overlap
+    def blocks(g0, n):
+        a = max(0, g0 * H - Ov)
+        seg = dev[a:(g0 + n) * H - Ov + N].cpu().numpy().view(np.complex64).reshape(-1).astype(np.complex128)
+        xs = np.zeros((g0 + n) * H - Ov + N, np.complex128)
+        xs[a:a + len(seg)] = seg
+        return O.forward_spectra(O.frame_blocks(xs, geom, n, first=g0), geom, np.float32(1.0))
+    X0 = blocks(0, A)
+    acc = O.WelchAccumulator(N, N, A)
+    rec0 = np.array(acc.accumulate(X0))[0]
+    e_w = rms(recs_g[0], rec0) / float(np.abs(rec0).max())
+    errs = []
+    for g, Xg in [(0, X0[0]), (1, X0[1])] + [(gg, blocks(gg, 1)[0]) for gg in (nb // 2, nb - 1)]:
+        for i in sample_ch:
+            first, L, w = wl.channels[i]
+            ref = O.channel_process(Xg[None], N, first, np.asarray(w, np.float64))[0]
+            errs.append(rms(outs[i][g], ref))
+    report("wideband_256k", blocks=nb, channels=len(ids), sampled_channels=len(sample_ch),
+           channel_rms_max=max(errs), welch_rel=e_w, records=len(recs_g))
+    assert max(errs) < TOL and e_w < 1e-5
+    st.close()
