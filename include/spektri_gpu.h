@@ -61,6 +61,10 @@ typedef struct spk_config {
  * HIP graph once and replay it (no counterpart in the reference).  Ignored
  * while per-kernel timing is enabled. */
 #define SPK_FLAG_GRAPHS 2u
+/* SpectrumFormat (spektri/src/dsp/spectrum.rs:29-32): also quantise every
+ * Welch record to dB, U8 (0.5 dB per LSB) or U16 (0.05 dB per LSB). */
+#define SPK_FLAG_SPECTRUM_U8 4u
+#define SPK_FLAG_SPECTRUM_U16 8u
 
 /* InputBufferSize (spektri/src/dsp.rs:38-42), in samples. */
 typedef struct spk_buffer_size {
@@ -116,6 +120,15 @@ spk_status spk_channel_output_device(spk_ctx* ctx, uint32_t id, const float** dp
  * floats. */
 spk_status spk_spectrum_records(spk_ctx* ctx, float* out, size_t cap, uint32_t* nrec);
 spk_status spk_spectrum_records_device(spk_ctx* ctx, const float** dptr, uint32_t* nrec);
+
+/* Quantised Welch records of the last call (contexts created with
+ * SPK_FLAG_SPECTRUM_U8 or SPK_FLAG_SPECTRUM_U16): the record body of
+ * spectrum.rs:128-152, K bytes (U8) or K big-endian 16-bit words (U16) per
+ * record, computed on the device from the f32 sum with the reference's
+ * formula.  cap counts bytes. */
+spk_status spk_spectrum_quantized(spk_ctx* ctx, void* out, size_t cap, uint32_t* nrec);
+spk_status spk_spectrum_quantized_device(spk_ctx* ctx, const void** dptr, uint32_t* nrec,
+                                         uint32_t* bytes_per_bin);
 
 /* Forward spectra of the last call: nblocks rows of K complex f32 with a
  * row stride of *stride complex values (the reference's fft_result_buf).

@@ -18,6 +18,8 @@ FMT_S16LE = 0x08
 FMT_CF32LE = 0x5C
 FLAG_KEEP_SPECTRA = 1
 FLAG_GRAPHS = 2
+FLAG_SPECTRUM_U8 = 4
+FLAG_SPECTRUM_U16 = 8
 
 
 class SpkConfig(ctypes.Structure):
@@ -68,6 +70,9 @@ _SIGS = {
     "spk_spectrum_records_device": (ctypes.c_int, [_P, ctypes.POINTER(_P),
                                                    ctypes.POINTER(ctypes.c_uint32)]),
     "spk_spectra_device": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_uint64)]),
+    "spk_spectrum_quantized": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32)]),
+    "spk_spectrum_quantized_device": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_uint32),
+                                                     ctypes.POINTER(ctypes.c_uint32)]),
     "spk_reset": (ctypes.c_int, [_P]),
     "spk_set_stream": (ctypes.c_int, [_P, _P]),
     "spk_synchronize": (ctypes.c_int, [_P]),

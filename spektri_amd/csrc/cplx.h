@@ -143,6 +143,8 @@ struct WelchDirectParams {
   int hist_words;
   int* chan_counters;
   int ncounters;
+  unsigned char* qout;
+  int qfmt;
 };
 
 __device__ __forceinline__ cpx bin_at(const cpx* X, int N, int K, int real_input, int i) {
@@ -175,6 +177,7 @@ __global__ void welch_direct_kernel(WelchDirectParams F) {
   }
   if (g_hi <= F.nblocks) {
     F.out[(long long)gi * F.K + k] = sum / (float)F.A;
+    if (F.qfmt) quantize_bin(F.qout, F.qfmt, (long long)gi * F.K + k, sum, F.A);
   } else {
     F.carry[k] = sum;
   }

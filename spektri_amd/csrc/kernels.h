@@ -838,7 +838,26 @@ struct FinParams {
   int hist_words;
   int* chan_counters;  // queue counters of chan_batched_kernel, reset here
   int ncounters;
+  unsigned char* qout;  // quantised records (qfmt 1: K bytes, 2: K big-endian u16) or null
+  int qfmt;
 };
+
+// dB quantiser of one accumulated bin, spectrum.rs:128-152 in f32:
+// dB = 10 log10(acc) - 10 log10(n); U8 = clamp(2 dB + 250, 0, 255),
+// U16 = clamp(20 dB + 4000, 0, 4095) big-endian; NaN and -inf clamp to 0,
+// the conversion truncates.
+__device__ __forceinline__ void quantize_bin(unsigned char* q, int qfmt, long long idx, float acc, int n) {
+  const float db = log10f(acc) * 10.0f + log10f((float)n) * -10.0f;
+  if (qfmt == 1) {
+    const float v = fminf(fmaxf(db * 2.0f + 250.0f, 0.0f), 255.0f);
+    q[idx] = (unsigned char)(unsigned)v;
+  } else {
+    const float v = fminf(fmaxf(db * 20.0f + 4000.0f, 0.0f), 4095.0f);
+    const unsigned o = (unsigned)v;
+    q[2 * idx] = (unsigned char)(o >> 8);
+    q[2 * idx + 1] = (unsigned char)(o & 0xFF);
+  }
+}
 
 // Grid: x over bins, y over the Welch groups the launch touches (group 0 is
 // the group of the launch's first block).  Row y = 0 also saves the input
@@ -870,6 +889,7 @@ __global__ void welch_finalize_kernel(FinParams F) {
   }
   if (g_hi <= F.nblocks) {
     F.out[(long long)gi * F.K + k] = sum / (float)F.A;
+    if (F.qfmt) quantize_bin(F.qout, F.qfmt, (long long)gi * F.K + k, sum, F.A);
   } else {
     F.carry[k] = sum;
   }

@@ -164,6 +164,8 @@ struct spk_ctx {
   BatchPlan batch;
   DevBuf chan_counters;      // kXcdQueues ints
   DevBuf pos;                // device stream position (blocks consumed), long long
+  DevBuf qout;               // quantised Welch records (SPK_FLAG_SPECTRUM_U8 / _U16)
+  int qfmt = 0;              // 0 none, 1 U8, 2 U16
   bool use_graphs = false;
   std::map<std::pair<const void*, int>, hipGraphExec_t> graphs;
   long long spec_stride = 0, wstride = 0;
@@ -615,6 +617,8 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
     W.hist_words = c->O * c->bps / 4;
     W.chan_counters = c->chan_counters.as<int>();
     W.ncounters = kXcdQueues;
+    W.qout = c->qout.as<unsigned char>();
+    W.qfmt = c->qfmt;
     const int gmax = (c->A - 1 + nb - 1) / c->A + 1;
     hipLaunchKernelGGL(spk::welch_direct_kernel, dim3((c->K + 255) / 256, gmax), dim3(256), 0, s, W);
     HIP_TRY(c, hipGetLastError());
@@ -718,6 +722,8 @@ spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hi
     Fp.hist_words = c->O * c->bps / 4;
     Fp.chan_counters = c->chan_counters.as<int>();
     Fp.ncounters = kXcdQueues;
+    Fp.qout = c->qout.as<unsigned char>();
+    Fp.qfmt = c->qfmt;
     // groups touched: at most (A - 1 + nb - 1) / A + 1; rows past the actual count exit
     const int gmax = (c->A - 1 + nb - 1) / c->A + 1;
     hipLaunchKernelGGL(spk::welch_finalize_kernel, dim3((c->K + 255) / 256, gmax), dim3(256), 0, s, Fp);
@@ -860,6 +866,10 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
       c->pos.alloc(sizeof(long long)) != hipSuccess || hipMemset(c->pos.p, 0, c->pos.bytes) != hipSuccess)
     return cleanup(SPK_ERR_DEVICE);
   c->use_graphs = (cfg->flags & SPK_FLAG_GRAPHS) != 0;
+  if ((cfg->flags & SPK_FLAG_SPECTRUM_U8) && (cfg->flags & SPK_FLAG_SPECTRUM_U16)) return cleanup(SPK_ERR_INVALID);
+  c->qfmt = (cfg->flags & SPK_FLAG_SPECTRUM_U8) ? 1 : ((cfg->flags & SPK_FLAG_SPECTRUM_U16) ? 2 : 0);
+  if (c->qfmt && c->qout.alloc((size_t)c->welch_out_cap * c->K * c->qfmt) != hipSuccess)
+    return cleanup(SPK_ERR_DEVICE);
   if (c->complex_in) {
     // every spectrum row is kept: the channel stage always reads them
     if ((long long)(c->max_blocks + c->A) * 64 >= (1LL << 31)) return cleanup(SPK_ERR_UNSUPPORTED);
@@ -1041,6 +1051,30 @@ spk_status spk_spectrum_records(spk_ctx* c, float* out, size_t cap, uint32_t* nr
   if (!out || n > cap) return fail(c, SPK_ERR_CAPACITY, "output buffer too small");
   HIP_TRY(c, hipMemcpyAsync(out, d, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return SPK_OK;
+}
+
+spk_status spk_spectrum_quantized_device(spk_ctx* c, const void** dptr, uint32_t* nrec, uint32_t* bytes_per_bin) {
+  if (!c || !dptr || !nrec || !bytes_per_bin) return SPK_ERR_INVALID;
+  if (!c->qfmt) return fail(c, SPK_ERR_STATE, "context created without SPK_FLAG_SPECTRUM_U8 / _U16");
+  *dptr = c->qout.p;
+  *nrec = c->have_output ? c->last_records : 0;
+  *bytes_per_bin = (uint32_t)c->qfmt;
+  return SPK_OK;
+}
+
+spk_status spk_spectrum_quantized(spk_ctx* c, void* out, size_t cap, uint32_t* nrec) {
+  if (!c || !nrec) return SPK_ERR_INVALID;
+  if (!c->qfmt) return fail(c, SPK_ERR_STATE, "context created without SPK_FLAG_SPECTRUM_U8 / _U16");
+  const uint32_t n = c->have_output ? c->last_records : 0;
+  const size_t need = (size_t)n * c->K * c->qfmt;
+  if (need > cap) return fail(c, SPK_ERR_CAPACITY, "output buffer too small");
+  if (need) {
+    if (!out) return SPK_ERR_INVALID;
+    HIP_TRY(c, hipMemcpyAsync(out, c->qout.p, need, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+  }
+  *nrec = n;
   return SPK_OK;
 }
 

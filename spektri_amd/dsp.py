@@ -55,6 +55,7 @@ class DspParams:
     max_blocks: int = 0             # capacity for process_device; 0 = ffts_per_buf
     keep_spectra: bool = False      # keep every block's spectrum on the device
     graphs: bool = False            # capture and replay each launch sequence as a HIP graph
+    quantize: bool = False          # also quantise Welch records in spectrum_format on the device
 
 
 @dataclass
@@ -103,6 +104,8 @@ class DspState:
         """dsp.rs:76-112: returns (state, InputBufferSize)."""
         lib = _lib.load()
         flags = (_lib.FLAG_KEEP_SPECTRA if params.keep_spectra else 0) | (_lib.FLAG_GRAPHS if params.graphs else 0)
+        if params.quantize:
+            flags |= _lib.FLAG_SPECTRUM_U8 if params.spectrum_format == "U8" else _lib.FLAG_SPECTRUM_U16
         fmt = params.input_format
         if params.complex and fmt == _lib.FMT_S16LE:
             fmt = _lib.FMT_CF32LE  # the complex format this library supports
@@ -250,6 +253,20 @@ class DspState:
         if nrec.value:
             _lib.check(lib.spk_spectrum_records(self._ctx, out.ctypes.data_as(ctypes.c_void_p), out.size,
                                                 ctypes.byref(got)), self._ctx)
+        return out
+
+    def spectrum_quantized(self) -> np.ndarray:
+        """Quantised Welch records of the last call: [R, K] uint8 (U8) or
+        big-endian uint16 (U16), the record body of spectrum.rs:128-152."""
+        lib = _lib.load()
+        p, n, bpb = ctypes.c_void_p(), ctypes.c_uint32(), ctypes.c_uint32()
+        _lib.check(lib.spk_spectrum_quantized_device(self._ctx, ctypes.byref(p), ctypes.byref(n), ctypes.byref(bpb)),
+                   self._ctx)
+        dt = np.uint8 if bpb.value == 1 else np.dtype(">u2")
+        out = np.empty((n.value, self.result_bins), dtype=dt)
+        got = ctypes.c_uint32()
+        _lib.check(lib.spk_spectrum_quantized(self._ctx, out.ctypes.data_as(ctypes.c_void_p), out.nbytes,
+                                              ctypes.byref(got)), self._ctx)
         return out
 
     def spectra_device(self):

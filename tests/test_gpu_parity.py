@@ -448,3 +448,38 @@ def test_wideband_256k_sampled_blocks():
            channel_rms_max=max(errs), welch_rel=e_w, records=len(recs_g))
     assert max(errs) < TOL and e_w < 1e-5
     st.close()
+
+
+@pytest.mark.parametrize("fmt", ["U8", "U16"])
+@pytest.mark.parametrize("complex_in", [False, True])
+def test_quantized_spectrum_records(fmt, complex_in):
+    """dB-quantised Welch records (spectrum.rs:128-152) against the oracle's
+    quantiser on the oracle's records: equal up to one LSB where the f32
+    sums round to different sides of a truncation boundary."""
+    rng = np.random.default_rng(41)
+    N, F, A = 8192, 4, 5
+    H = N - N // 4
+    nb = 3 * F
+    if complex_in:
+        x = (rng.standard_normal(nb * H) + 1j * rng.standard_normal(nb * H)).astype(np.complex64) * 0.01
+    else:
+        x = np.clip(np.round(rng.standard_normal(nb * H) * 300), -32768, 32767).astype(np.int16)
+    st, _ = DspState.init(DspParams(complex=complex_in, fs_in=1.0, fft_size=N, ffts_per_buf=F, spectrum_averages=A,
+                                    max_blocks=nb, quantize=True, spectrum_format=fmt))
+    st.add_channel(100, 64)
+    dev = torch.from_numpy(np.ascontiguousarray(x).view(np.int16 if not complex_in else np.float32)).to("cuda")
+    st.process_device(dev.data_ptr(), len(x))
+    st.synchronize()
+    q = st.spectrum_quantized().astype(np.int64)
+    geom = O.Geometry(N, complex_in, F)
+    s = np.float32(1.0) if complex_in else O.input_format_scaling(O.FMT_S16LE)
+    xs = x.astype(np.complex128) if complex_in else x
+    X = O.forward_spectra(O.frame_blocks(xs, geom, nb), geom, s)
+    recs = np.array(O.WelchAccumulator(geom.bins, N, A).accumulate(X))
+    ref = np.stack([O.quantize_spectrum(r, A, fmt.lower()) for r in recs]).astype(np.int64)
+    assert q.shape == ref.shape
+    d = np.abs(q - ref)
+    report(f"quantized_{fmt}_{'complex' if complex_in else 'real'}", records=len(q), max_lsb=int(d.max()),
+           frac_exact=float(np.mean(d == 0)))
+    assert d.max() <= 1 and np.mean(d == 0) > 0.99
+    st.close()
