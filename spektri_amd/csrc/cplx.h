@@ -136,6 +136,7 @@ struct WelchDirectParams {
   int N, K, real_input;
   int nblocks, A;
   const long long* pos;
+  long long* pos_next;
   float* carry;
   float* out;
   const int* hist_src;
@@ -160,11 +161,13 @@ __device__ __forceinline__ cpx bin_at(const cpx* X, int N, int K, int real_input
 __global__ void welch_direct_kernel(WelchDirectParams F) {
   const int gi = (int)blockIdx.y;
   const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const long long pos = *F.pos;
   if (gi == 0) {
     for (int i = k; i < F.hist_words; i += (int)(gridDim.x * blockDim.x)) F.hist_dst[i] = F.hist_src[i];
     if (k < F.ncounters) F.chan_counters[k] = 0;
+    if (k == 0) *F.pos_next = pos + F.nblocks;
   }
-  const int blk0_mod = (int)(*F.pos % F.A);
+  const int blk0_mod = (int)(pos % F.A);
   if (k >= F.K || gi > (blk0_mod + F.nblocks - 1) / F.A) return;
   const int g_lo = gi * F.A - blk0_mod, g_hi = g_lo + F.A;
   const int b_lo = g_lo > 0 ? g_lo : 0;

@@ -828,6 +828,7 @@ struct FinParams {
   int G;               // workgroups of the forward launch
   int nblocks;
   const long long* pos;  // device stream position of the launch's first block
+  long long* pos_next;   // the next call's position slot (pos + nblocks is written there)
   int A, K, M, R;
   float* carry;        // K floats
   float* out;          // records [r][K]
@@ -863,35 +864,52 @@ __device__ __forceinline__ void quantize_bin(unsigned char* q, int qfmt, long lo
 // the group of the launch's first block).  Row y = 0 also saves the input
 // history for the next call.
 __global__ void welch_finalize_kernel(FinParams F) {
+  constexpr int KB = 4;  // bins per thread: more loads in flight per thread
   const int gi = (int)blockIdx.y;
-  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const long long pos = *F.pos;
   if (gi == 0) {
-    for (int i = k; i < F.hist_words; i += (int)(gridDim.x * blockDim.x)) F.hist_dst[i] = F.hist_src[i];
-    if (k < F.ncounters) F.chan_counters[k] = 0;
+    for (int i = tid; i < F.hist_words; i += (int)(gridDim.x * blockDim.x)) F.hist_dst[i] = F.hist_src[i];
+    if (tid < F.ncounters) F.chan_counters[tid] = 0;
+    if (tid == 0) *F.pos_next = pos + F.nblocks;
   }
-  const int blk0_mod = (int)(*F.pos % F.A);
+  const int k0 = KB * tid;
+  const int blk0_mod = (int)(pos % F.A);
   // rows beyond the groups this launch touches (the grid is sized for the worst case)
-  if (k >= F.K || gi > (blk0_mod + F.nblocks - 1) / F.A) return;
-  const int ridx = welch_row_index(k, F.M, F.R);
+  if (k0 >= F.K || gi > (blk0_mod + F.nblocks - 1) / F.A) return;
   const int g_lo = gi * F.A - blk0_mod, g_hi = g_lo + F.A;
   const int b_lo = g_lo > 0 ? g_lo : 0;
   const int b_hi = g_hi < F.nblocks ? g_hi : F.nblocks;
-  float sum = (gi == 0 && blk0_mod != 0) ? F.carry[k] : 0.f;
+  int ridx[KB];
+  float sum[KB];
+#pragma unroll
+  for (int i = 0; i < KB; ++i) {
+    const int k = k0 + i;
+    ridx[i] = k < F.K ? welch_row_index(k, F.M, F.R) : 0;
+    sum[i] = (gi == 0 && blk0_mod != 0 && k < F.K) ? F.carry[k] : 0.f;
+  }
   // workgroup of block b: the w with floor(w nb / G) <= b < floor((w+1) nb / G)
   const unsigned nb = (unsigned)F.nblocks, G = (unsigned)F.G;
   const int w0 = (int)(((unsigned)(b_lo + 1) * G + nb - 1) / nb) - 1;
   const int w1 = (int)(((unsigned)b_hi * G + nb - 1) / nb) - 1;
-#pragma unroll 4
+#pragma unroll 2
   for (int wg = w0; wg <= w1; ++wg) {
     const int lo = (int)((unsigned)wg * nb / G);
     const int slot = gi - (blk0_mod + lo) / F.A;
-    sum += F.wpart[((long long)wg * F.wslots + slot) * F.wstride + ridx];
+    const float* row = F.wpart + ((long long)wg * F.wslots + slot) * F.wstride;
+#pragma unroll
+    for (int i = 0; i < KB; ++i) sum[i] += row[ridx[i]];
   }
-  if (g_hi <= F.nblocks) {
-    F.out[(long long)gi * F.K + k] = sum / (float)F.A;
-    if (F.qfmt) quantize_bin(F.qout, F.qfmt, (long long)gi * F.K + k, sum, F.A);
-  } else {
-    F.carry[k] = sum;
+#pragma unroll
+  for (int i = 0; i < KB; ++i) {
+    const int k = k0 + i;
+    if (k >= F.K) break;
+    if (g_hi <= F.nblocks) {
+      F.out[(long long)gi * F.K + k] = sum[i] / (float)F.A;
+      if (F.qfmt) quantize_bin(F.qout, F.qfmt, (long long)gi * F.K + k, sum[i], F.A);
+    } else {
+      F.carry[k] = sum[i];
+    }
   }
 }
 
@@ -1000,11 +1018,6 @@ __global__ __launch_bounds__(kChanThreads) void chan_adjoint_kernel(ChanAdjParam
     for (int k = 0; k < TPW; ++k) sum += lds_all[k * L + i];
     P.wpart[((long long)c * P.S + split) * L + ((i + L / 2) & (L - 1))] = sum;
   }
-}
-
-// Advance the device stream position after a launch sequence.
-__global__ void advance_kernel(long long* pos, int nblocks) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *pos += nblocks;
 }
 
 // wbar[w_off + j] = sum over splits (in order) of wpart[c][s][j].

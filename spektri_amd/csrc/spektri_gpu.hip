@@ -15,6 +15,7 @@
 #include <map>
 #include <new>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "spektri_gpu.h"
@@ -163,11 +164,12 @@ struct spk_ctx {
   int tune_chunk = 8, tune_rounds = 2, tune_wgs_per_cu = 0;
   BatchPlan batch;
   DevBuf chan_counters;      // kXcdQueues ints
-  DevBuf pos;                // device stream position (blocks consumed), long long
+  DevBuf pos;                // device stream position (blocks consumed), two long long slots
+  int parity = 0;            // slot holding this call's position; the Welch kernel writes the other
   DevBuf qout;               // quantised Welch records (SPK_FLAG_SPECTRUM_U8 / _U16)
   int qfmt = 0;              // 0 none, 1 U8, 2 U16
   bool use_graphs = false;
-  std::map<std::pair<const void*, int>, hipGraphExec_t> graphs;
+  std::map<std::tuple<const void*, int, int>, hipGraphExec_t> graphs;
   long long spec_stride = 0, wstride = 0;
   int wslots = 0, welch_out_cap = 0;
   int gmax[2] = {0, 0};      // persistent workgroups of the unfused / fused forward kernel
@@ -608,7 +610,8 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
     W.real_input = 0;
     W.nblocks = nb;
     W.A = c->A;
-    W.pos = c->pos.as<long long>();
+    W.pos = c->pos.as<long long>() + c->parity;
+    W.pos_next = c->pos.as<long long>() + (c->parity ^ 1);
     W.carry = c->carry.as<float>();
     W.out = c->welch_out.as<float>();
     W.hist_src = reinterpret_cast<const int*>(reinterpret_cast<const char*>(src) +
@@ -623,8 +626,6 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
     hipLaunchKernelGGL(spk::welch_direct_kernel, dim3((c->K + 255) / 256, gmax), dim3(256), 0, s, W);
     HIP_TRY(c, hipGetLastError());
   }
-  hipLaunchKernelGGL(spk::advance_kernel, dim3(1), dim3(64), 0, s, c->pos.as<long long>(), nb);
-  HIP_TRY(c, hipGetLastError());
   if (ev) HIP_TRY(c, hipEventRecord(ev[3], s));
   return SPK_OK;
 }
@@ -671,7 +672,7 @@ spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hi
   P.wstride = c->wstride;
   P.wslots = c->wslots;
   P.A = c->A;
-  P.pos = c->pos.as<long long>();
+  P.pos = c->pos.as<long long>() + c->parity;
   P.wscale = c->wscale;
   P.nblocks = nb;
   P.H = c->H;
@@ -708,7 +709,8 @@ spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hi
     Fp.wslots = c->wslots;
     Fp.G = G;
     Fp.nblocks = nb;
-    Fp.pos = c->pos.as<long long>();
+    Fp.pos = c->pos.as<long long>() + c->parity;
+    Fp.pos_next = c->pos.as<long long>() + (c->parity ^ 1);
     Fp.A = c->A;
     Fp.K = c->K;
     Fp.M = c->M;
@@ -726,11 +728,10 @@ spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hi
     Fp.qfmt = c->qfmt;
     // groups touched: at most (A - 1 + nb - 1) / A + 1; rows past the actual count exit
     const int gmax = (c->A - 1 + nb - 1) / c->A + 1;
-    hipLaunchKernelGGL(spk::welch_finalize_kernel, dim3((c->K + 255) / 256, gmax), dim3(256), 0, s, Fp);
+    hipLaunchKernelGGL(spk::welch_finalize_kernel, dim3((c->K + 4 * 256 - 1) / (4 * 256), gmax), dim3(256), 0, s,
+                       Fp);
     HIP_TRY(c, hipGetLastError());
   }
-  hipLaunchKernelGGL(spk::advance_kernel, dim3(1), dim3(64), 0, s, c->pos.as<long long>(), nb);
-  HIP_TRY(c, hipGetLastError());
   if (ev) HIP_TRY(c, hipEventRecord(ev[3], s));
   return SPK_OK;
 }
@@ -751,7 +752,7 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
   }
   hipStream_t s = c->stream;
   if (c->use_graphs && !c->timing) {
-    const auto key = std::make_pair(src, nb);
+    const auto key = std::make_tuple(src, nb, c->parity);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
       if (c->graphs.size() >= 8) drop_graphs(c);
@@ -794,6 +795,7 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
   const long long s_first = c->blocks_done / c->A;
   c->last_records = (uint32_t)((c->blocks_done + nb) / c->A - s_first);
   c->blocks_done += nb;
+  c->parity ^= 1;
   c->last_blocks = (uint32_t)nb;
   c->have_output = true;
   return SPK_OK;
@@ -863,7 +865,7 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   c->welch_out_cap = c->max_blocks / c->A + 1;
   if (c->chan_counters.alloc(kXcdQueues * sizeof(int)) != hipSuccess ||
       hipMemset(c->chan_counters.p, 0, c->chan_counters.bytes) != hipSuccess ||
-      c->pos.alloc(sizeof(long long)) != hipSuccess || hipMemset(c->pos.p, 0, c->pos.bytes) != hipSuccess)
+      c->pos.alloc(2 * sizeof(long long)) != hipSuccess || hipMemset(c->pos.p, 0, c->pos.bytes) != hipSuccess)
     return cleanup(SPK_ERR_DEVICE);
   c->use_graphs = (cfg->flags & SPK_FLAG_GRAPHS) != 0;
   if ((cfg->flags & SPK_FLAG_SPECTRUM_U8) && (cfg->flags & SPK_FLAG_SPECTRUM_U16)) return cleanup(SPK_ERR_INVALID);
@@ -1091,6 +1093,7 @@ spk_status spk_reset(spk_ctx* c) {
   HIP_TRY(c, hipMemset(c->hist.p, 0, c->hist.bytes));
   HIP_TRY(c, hipMemset(c->carry.p, 0, c->carry.bytes));
   HIP_TRY(c, hipMemset(c->pos.p, 0, c->pos.bytes));
+  c->parity = 0;
   c->blocks_done = 0;
   c->last_blocks = 0;
   c->last_records = 0;
