@@ -171,7 +171,8 @@ def main():
                  "roofline_MSps": round(roofline_msps, 1)},
         "kernels_ms": {k: round(v, 4) for k, v in kern.items()},
         "fused_channels": fused,
-        "graphs": not args.no_graphs,
+        # the library replays graphs only for calls with channel launches
+        "graphs": (not args.no_graphs) and (wl.complex or fused < len(wl.channels)),
     }
     if tune is not None:
         out["channel_stage_tune"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tune.items()}

@@ -58,8 +58,11 @@ typedef struct spk_config {
  * it; channels of L = 256 are then computed inside the forward kernel. */
 #define SPK_FLAG_KEEP_SPECTRA 1u
 /* Capture each distinct (input pointer, block count) launch sequence into a
- * HIP graph once and replay it (no counterpart in the reference).  Ignored
- * while per-kernel timing is enabled. */
+ * HIP graph once and replay it (no counterpart in the reference).  Applies
+ * to calls with channel launches after the forward transform (complex
+ * input, or channels not run inside the forward kernel); a real-input call
+ * whose channels all run inside the forward kernel is two kernels and is
+ * launched directly.  Ignored while per-kernel timing is enabled. */
 #define SPK_FLAG_GRAPHS 2u
 /* SpectrumFormat (spektri/src/dsp/spectrum.rs:29-32): also quantise every
  * Welch record to dB, U8 (0.5 dB per LSB) or U16 (0.05 dB per LSB). */

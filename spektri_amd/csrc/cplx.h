@@ -137,7 +137,10 @@ struct WelchDirectParams {
   int nblocks, A;
   const long long* pos;
   long long* pos_next;
-  float* carry;
+  // open-group sums by bin: the previous call's (read) and this call's
+  // (written), two buffers alternated per call
+  const float* carry_in;
+  float* carry_out;
   float* out;
   const int* hist_src;
   int* hist_dst;
@@ -172,7 +175,7 @@ __global__ void welch_direct_kernel(WelchDirectParams F) {
   const int g_lo = gi * F.A - blk0_mod, g_hi = g_lo + F.A;
   const int b_lo = g_lo > 0 ? g_lo : 0;
   const int b_hi = g_hi < F.nblocks ? g_hi : F.nblocks;
-  float sum = (gi == 0 && blk0_mod != 0) ? F.carry[k] : 0.f;
+  float sum = (gi == 0 && blk0_mod != 0) ? F.carry_in[k] : 0.f;
   for (int b = b_lo; b < b_hi; ++b) {
     const cpx* X = F.spec + (long long)b * F.spec_stride;
     sum += welch_term(bin_at(X, F.N, F.K, F.real_input, k), bin_at(X, F.N, F.K, F.real_input, k - 1),
@@ -182,7 +185,7 @@ __global__ void welch_direct_kernel(WelchDirectParams F) {
     F.out[(long long)gi * F.K + k] = sum / (float)F.A;
     if (F.qfmt) quantize_bin(F.qout, F.qfmt, (long long)gi * F.K + k, sum, F.A);
   } else {
-    F.carry[k] = sum;
+    F.carry_out[k] = sum;
   }
 }
 

@@ -219,6 +219,16 @@ __host__ __device__ inline int welch_row_index(int k, int M, int R) {
   return q * P3 + 64 * (pi >> 5) + hi + (pi & 31);
 }
 
+// Bin held at partial-row position i (the inverse of welch_row_index).
+__host__ __device__ inline int welch_row_bin(int i, int M, int R) {
+  const int P3 = M / R;
+  if (i >= M) return M;
+  const int q = i / P3, t = i % P3;
+  const int pi = 32 * (t >> 6) + (t & 31);
+  if ((t & 63) < 32) return pi + P3 * q;
+  return (pi == 0 ? P3 / 2 : P3 - pi) + P3 * (R - 1 - q);
+}
+
 __device__ __forceinline__ float swap_hi_lo(float v, float& other) {
   // v_permlane32_swap: returns (A, B) with A = {v.lo | v.lo}, B = {v.hi | v.hi}
   // when both operands are v; A's upper half and B's lower half are the
@@ -830,7 +840,12 @@ struct FinParams {
   const long long* pos;  // device stream position of the launch's first block
   long long* pos_next;   // the next call's position slot (pos + nblocks is written there)
   int A, K, M, R;
-  float* carry;        // K floats
+  // Sums of the Welch group left open at the end of the previous call (read)
+  // and of this call (written), in partial-row order, M + 1 floats each.
+  // Two buffers alternated per call: the first group reads one while the
+  // last group writes the other.
+  const float* carry_in;
+  float* carry_out;
   float* out;          // records [r][K]
   // input history for the next call: the last hist_words 32-bit words of
   // this call's input
@@ -864,7 +879,6 @@ __device__ __forceinline__ void quantize_bin(unsigned char* q, int qfmt, long lo
 // the group of the launch's first block).  Row y = 0 also saves the input
 // history for the next call.
 __global__ void welch_finalize_kernel(FinParams F) {
-  constexpr int KB = 4;  // bins per thread: more loads in flight per thread
   const int gi = (int)blockIdx.y;
   const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const long long pos = *F.pos;
@@ -873,20 +887,21 @@ __global__ void welch_finalize_kernel(FinParams F) {
     if (tid < F.ncounters) F.chan_counters[tid] = 0;
     if (tid == 0) *F.pos_next = pos + F.nblocks;
   }
-  const int k0 = KB * tid;
+  // Each thread sums four consecutive partial-row positions (16-byte loads)
+  // over the workgroups of its group and writes the four bins they hold;
+  // position M (the Nyquist bin, M % 4 == 0) is a quad of its own.
+  const int i0 = 4 * tid;
   const int blk0_mod = (int)(pos % F.A);
   // rows beyond the groups this launch touches (the grid is sized for the worst case)
-  if (k0 >= F.K || gi > (blk0_mod + F.nblocks - 1) / F.A) return;
+  if (i0 > F.M || gi > (blk0_mod + F.nblocks - 1) / F.A) return;
+  const bool nyq = i0 == F.M;
   const int g_lo = gi * F.A - blk0_mod, g_hi = g_lo + F.A;
   const int b_lo = g_lo > 0 ? g_lo : 0;
   const int b_hi = g_hi < F.nblocks ? g_hi : F.nblocks;
-  int ridx[KB];
-  float sum[KB];
-#pragma unroll
-  for (int i = 0; i < KB; ++i) {
-    const int k = k0 + i;
-    ridx[i] = k < F.K ? welch_row_index(k, F.M, F.R) : 0;
-    sum[i] = (gi == 0 && blk0_mod != 0 && k < F.K) ? F.carry[k] : 0.f;
+  float4 s = {0.f, 0.f, 0.f, 0.f};
+  if (gi == 0 && blk0_mod != 0) {
+    if (nyq) s.x = F.carry_in[i0];
+    else s = *reinterpret_cast<const float4*>(F.carry_in + i0);
   }
   // workgroup of block b: the w with floor(w nb / G) <= b < floor((w+1) nb / G)
   const unsigned nb = (unsigned)F.nblocks, G = (unsigned)F.G;
@@ -897,19 +912,28 @@ __global__ void welch_finalize_kernel(FinParams F) {
     const int lo = (int)((unsigned)wg * nb / G);
     const int slot = gi - (blk0_mod + lo) / F.A;
     const float* row = F.wpart + ((long long)wg * F.wslots + slot) * F.wstride;
-#pragma unroll
-    for (int i = 0; i < KB; ++i) sum[i] += row[ridx[i]];
-  }
-#pragma unroll
-  for (int i = 0; i < KB; ++i) {
-    const int k = k0 + i;
-    if (k >= F.K) break;
-    if (g_hi <= F.nblocks) {
-      F.out[(long long)gi * F.K + k] = sum[i] / (float)F.A;
-      if (F.qfmt) quantize_bin(F.qout, F.qfmt, (long long)gi * F.K + k, sum[i], F.A);
+    if (nyq) {
+      s.x += row[i0];
     } else {
-      F.carry[k] = sum[i];
+      const float4 v = *reinterpret_cast<const float4*>(row + i0);
+      s.x += v.x;
+      s.y += v.y;
+      s.z += v.z;
+      s.w += v.w;
     }
+  }
+  if (g_hi > F.nblocks) {
+    if (nyq) F.carry_out[i0] = s.x;
+    else *reinterpret_cast<float4*>(F.carry_out + i0) = s;
+    return;
+  }
+  const float sv[4] = {s.x, s.y, s.z, s.w};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (nyq && j > 0) break;
+    const long long o = (long long)gi * F.K + welch_row_bin(i0 + j, F.M, F.R);
+    F.out[o] = sv[j] / (float)F.A;
+    if (F.qfmt) quantize_bin(F.qout, F.qfmt, o, sv[j], F.A);
   }
 }
 

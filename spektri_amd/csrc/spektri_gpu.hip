@@ -612,7 +612,8 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
     W.A = c->A;
     W.pos = c->pos.as<long long>() + c->parity;
     W.pos_next = c->pos.as<long long>() + (c->parity ^ 1);
-    W.carry = c->carry.as<float>();
+    W.carry_in = c->carry.as<float>() + (long long)c->parity * c->spec_stride;
+    W.carry_out = c->carry.as<float>() + (long long)(c->parity ^ 1) * c->spec_stride;
     W.out = c->welch_out.as<float>();
     W.hist_src = reinterpret_cast<const int*>(reinterpret_cast<const char*>(src) +
                                               ((long long)nb * c->H - c->O) * c->bps);
@@ -715,7 +716,8 @@ spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hi
     Fp.K = c->K;
     Fp.M = c->M;
     Fp.R = c->fplan->R;
-    Fp.carry = c->carry.as<float>();
+    Fp.carry_in = c->carry.as<float>() + (long long)c->parity * c->spec_stride;
+    Fp.carry_out = c->carry.as<float>() + (long long)(c->parity ^ 1) * c->spec_stride;
     Fp.out = c->welch_out.as<float>();
     // New history: the last O samples consumed.
     Fp.hist_src = reinterpret_cast<const int*>(reinterpret_cast<const char*>(src) +
@@ -728,8 +730,7 @@ spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hi
     Fp.qfmt = c->qfmt;
     // groups touched: at most (A - 1 + nb - 1) / A + 1; rows past the actual count exit
     const int gmax = (c->A - 1 + nb - 1) / c->A + 1;
-    hipLaunchKernelGGL(spk::welch_finalize_kernel, dim3((c->K + 4 * 256 - 1) / (4 * 256), gmax), dim3(256), 0, s,
-                       Fp);
+    hipLaunchKernelGGL(spk::welch_finalize_kernel, dim3((c->M / 4 + 1 + 255) / 256, gmax), dim3(256), 0, s, Fp);
     HIP_TRY(c, hipGetLastError());
   }
   if (ev) HIP_TRY(c, hipEventRecord(ev[3], s));
@@ -741,9 +742,19 @@ void drop_graphs(spk_ctx* c) {
   c->graphs.clear();
 }
 
+// Whether a call's launch sequence is worth a graph: with channel launches
+// after the forward transform.  A real-input call whose channels all run
+// inside the forward kernel is two launches (forward, finaliser), and
+// replaying it as a graph leaves a longer idle gap between calls than
+// launching the two kernels directly.
+bool graph_pays(const spk_ctx* c) {
+  if (c->complex_in) return true;
+  return c->groups.size() > (c->fused ? 1u : 0u);
+}
+
 // One launch sequence over nb blocks whose samples start at src.  With
 // graphs enabled (and timing off) the sequence for (src, nb) is captured
-// once and replayed.
+// once and replayed when it has channel launches (graph_pays).
 spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
   if (c->chans_dirty) {
     drop_graphs(c);
@@ -751,7 +762,7 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
     if (st != SPK_OK) return st;
   }
   hipStream_t s = c->stream;
-  if (c->use_graphs && !c->timing) {
+  if (c->use_graphs && !c->timing && graph_pays(c)) {
     const auto key = std::make_tuple(src, nb, c->parity);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
@@ -881,7 +892,7 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
         c->staging.alloc((size_t)c->F * c->H * c->bps) != hipSuccess ||
         c->spec.alloc((size_t)c->max_blocks * c->spec_stride * sizeof(cpx)) != hipSuccess ||
         c->cY.alloc((size_t)c->max_blocks * c->N * sizeof(cpx)) != hipSuccess ||
-        c->carry.alloc((size_t)c->K * sizeof(float)) != hipSuccess ||
+        c->carry.alloc((size_t)2 * c->spec_stride * sizeof(float)) != hipSuccess ||
         c->welch_out.alloc((size_t)c->welch_out_cap * c->K * sizeof(float)) != hipSuccess)
       return cleanup(SPK_ERR_DEVICE);
     if (hipMemset(c->hist.p, 0, c->hist.bytes) != hipSuccess ||
@@ -925,7 +936,7 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
       c->wpart.alloc((size_t)gcap * c->wslots * c->wstride * sizeof(float)) != hipSuccess ||
       c->edump.alloc((size_t)gcap * (c->M / c->fplan->R / 64) * spk::kEdgeLanes * c->fplan->R * sizeof(cpx)) !=
           hipSuccess ||
-      c->carry.alloc((size_t)c->K * sizeof(float)) != hipSuccess ||
+      c->carry.alloc((size_t)2 * c->spec_stride * sizeof(float)) != hipSuccess ||
       c->welch_out.alloc((size_t)c->welch_out_cap * c->K * sizeof(float)) != hipSuccess)
     return cleanup(SPK_ERR_DEVICE);
   if (hipMemset(c->hist.p, 0, c->hist.bytes) != hipSuccess || hipMemset(c->carry.p, 0, c->carry.bytes) != hipSuccess)
