@@ -114,6 +114,7 @@ def main():
     ptr, n = x.data_ptr(), wl.nsamples
     for _ in range(args.warmup):
         st.process_device(ptr, n)
+    st.flush()
     torch.cuda.synchronize()
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
@@ -121,6 +122,9 @@ def main():
     t0.record(stream)
     for _ in range(args.steps):
         st.process_device(ptr, n)
+    # the last step's Welch records (the others are summed inside the next
+    # step's forward launch) are inside the timed region
+    st.flush()
     t1.record(stream)
     torch.cuda.synchronize()
     ms_step = t0.elapsed_time(t1) / args.steps

@@ -105,7 +105,11 @@ spk_status spk_process(spk_ctx* ctx, const void* raw_new, size_t nbytes);
 /* Same path on a device-resident stream of nsamples raw samples (4-byte
  * aligned): processes floor(nsamples / (F * 3N/4)) * F blocks, like the
  * reference's main loop over that many bytes (main.rs:171-179), in one
- * launch sequence on the context's stream.  Asynchronous. */
+ * launch sequence on the context's stream.  Asynchronous.  For real input
+ * the call's Welch records are summed by the next call's forward launch
+ * (in the time its early workgroups would idle), or when first asked for:
+ * spk_spectrum_records*, spk_spectrum_quantized*, spk_synchronize and
+ * spk_flush enqueue the sum. */
 spk_status spk_process_device(spk_ctx* ctx, const void* d_raw, uint64_t nsamples);
 
 /* Blocks processed by the last process call. */
@@ -120,7 +124,9 @@ spk_status spk_channel_output_device(spk_ctx* ctx, uint32_t id, const float** dp
 
 /* Welch records completed by the last call (spectrum.rs:113-160): each is K
  * floats acc / A (K = N/2 + 1).  *nrec receives the record count; cap counts
- * floats. */
+ * floats.  The _device form enqueues the records' sum on the context's
+ * stream if it is still pending; the pointer is readable after that stream
+ * (spk_synchronize). */
 spk_status spk_spectrum_records(spk_ctx* ctx, float* out, size_t cap, uint32_t* nrec);
 spk_status spk_spectrum_records_device(spk_ctx* ctx, const float** dptr, uint32_t* nrec);
 
@@ -143,7 +149,13 @@ spk_status spk_reset(spk_ctx* ctx);
 /* Run on the caller's HIP stream (hipStream_t passed as void*); NULL
  * returns to the context's own stream. */
 spk_status spk_set_stream(spk_ctx* ctx, void* stream);
+/* Sum the last call's pending Welch records (as spk_flush), then wait for
+ * the context's stream. */
 spk_status spk_synchronize(spk_ctx* ctx);
+/* Enqueue the sum of the last call's Welch records if still pending (no
+ * counterpart in the reference, whose records are written synchronously);
+ * asynchronous. */
+spk_status spk_flush(spk_ctx* ctx);
 
 /* Per-kernel timing from HIP events recorded on the context's stream around
  * each launch: average ms per call since spk_enable_timing of ms[0] the

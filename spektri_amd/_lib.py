@@ -76,6 +76,7 @@ _SIGS = {
     "spk_reset": (ctypes.c_int, [_P]),
     "spk_set_stream": (ctypes.c_int, [_P, _P]),
     "spk_synchronize": (ctypes.c_int, [_P]),
+    "spk_flush": (ctypes.c_int, [_P]),
     "spk_enable_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "spk_last_timing": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float)]),
     "spk_fused_channels": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32)]),

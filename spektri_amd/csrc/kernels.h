@@ -60,6 +60,40 @@ __host__ __device__ inline int edge_slot(int lane) {
 }
 constexpr int kEdgeNyq = 9;  // wave 0, slot 0: X[M]
 
+// ---------------------------------------------------------------------------
+// Welch records (spektri/src/dsp/spectrum.rs:113-160).  For every group s
+// touched by this launch: sum the partial rows of the workgroups that
+// processed its blocks (plus the carry of a group begun in an earlier
+// launch); a complete group becomes the record sum / A, an incomplete last
+// group goes back to the carry row.
+// ---------------------------------------------------------------------------
+struct FinParams {
+  const float* wpart;
+  long long wstride;
+  int wslots;
+  int G;               // workgroups of the forward launch
+  int nblocks;
+  const long long* pos;  // device stream position of the launch's first block
+  long long* pos_next;   // the next call's position slot (pos + nblocks is written there)
+  int A, K, M, R;
+  // Sums of the Welch group left open at the end of the previous call (read)
+  // and of this call (written), in partial-row order, M + 1 floats each.
+  // Two buffers alternated per call: the first group reads one while the
+  // last group writes the other.
+  const float* carry_in;
+  float* carry_out;
+  float* out;          // records [r][K]
+  // input history for the next call: the last hist_words 32-bit words of
+  // this call's input
+  const int* hist_src;
+  int* hist_dst;
+  int hist_words;
+  int* chan_counters;  // queue counters of chan_batched_kernel, reset here
+  int ncounters;
+  unsigned char* qout;  // quantised records (qfmt 1: K bytes, 2: K big-endian u16) or null
+  int qfmt;
+};
+
 struct FwdParams {
   const int16_t* src;   // stream samples; sample s >= 0 at src[s]
   const int16_t* hist;  // the O samples before src[0] (block 0 reads them, then src[0 .. N-O))
@@ -88,6 +122,21 @@ struct FwdParams {
   const float* weights;
   cpx* fout;
   const cpx* ftw2;      // pass-2 table of the FL-point inverse transform
+  // For the next call, by workgroup 0: its position slot, its input history
+  // (the last O samples of this call) and the batched channel kernel's
+  // queue counters (that kernel runs after this one in the same call).
+  long long* pos_next;
+  const int* hist_src;
+  int* hist_dst;
+  int hist_words;
+  int* chan_counters;
+  int ncounters;
+  // Welch records of the previous call (when fin.nblocks > 0), finished by
+  // the workgroups that are done with their own blocks: chunks of 4 T
+  // partial-row positions of one group, taken through fin_ticket (zero
+  // between launches; the launch's last taker resets it).
+  FinParams fin;
+  int* fin_ticket;
 };
 
 // Fused channel tiles: FL points as 16 x (FL/16), 16 points per thread.
@@ -270,10 +319,81 @@ __device__ __forceinline__ float welch_term(cpx c0, cpx cm, cpx cp) {
   return cr * cr + ci * ci;
 }
 
+// dB quantiser of one accumulated bin, spectrum.rs:128-152 in f32:
+// dB = 10 log10(acc) - 10 log10(n); U8 = clamp(2 dB + 250, 0, 255),
+// U16 = clamp(20 dB + 4000, 0, 4095) big-endian; NaN and -inf clamp to 0,
+// the conversion truncates.
+__device__ __forceinline__ void quantize_bin(unsigned char* q, int qfmt, long long idx, float acc, int n) {
+  const float db = log10f(acc) * 10.0f + log10f((float)n) * -10.0f;
+  if (qfmt == 1) {
+    const float v = fminf(fmaxf(db * 2.0f + 250.0f, 0.0f), 255.0f);
+    q[idx] = (unsigned char)(unsigned)v;
+  } else {
+    const float v = fminf(fmaxf(db * 20.0f + 4000.0f, 0.0f), 4095.0f);
+    const unsigned o = (unsigned)v;
+    q[2 * idx] = (unsigned char)(o >> 8);
+    q[2 * idx + 1] = (unsigned char)(o & 0xFF);
+  }
+}
+
+
+// One thread's share of a Welch group gi of a launch whose first block is
+// at stream position pos: partial-row positions i0 .. i0+3 (or the Nyquist
+// position M alone, M % 4 == 0) summed over the workgroups that processed
+// the group's blocks, plus the carry of a group begun in an earlier call.  A
+// complete group is written as records (bins by welch_row_bin), an
+// incomplete one as the next carry.
+__device__ __forceinline__ void welch_finalize_quad(const FinParams& F, long long pos, int gi, int i0) {
+  const bool nyq = i0 == F.M;
+  const int blk0_mod = (int)(pos % F.A);
+  const int g_lo = gi * F.A - blk0_mod, g_hi = g_lo + F.A;
+  const int b_lo = g_lo > 0 ? g_lo : 0;
+  const int b_hi = g_hi < F.nblocks ? g_hi : F.nblocks;
+  float4 s = {0.f, 0.f, 0.f, 0.f};
+  if (gi == 0 && blk0_mod != 0) {
+    if (nyq) s.x = F.carry_in[i0];
+    else s = *reinterpret_cast<const float4*>(F.carry_in + i0);
+  }
+  // workgroup of block b: the w with floor(w nb / G) <= b < floor((w+1) nb / G)
+  const unsigned nb = (unsigned)F.nblocks, G = (unsigned)F.G;
+  const int w0 = (int)(((unsigned)(b_lo + 1) * G + nb - 1) / nb) - 1;
+  const int w1 = (int)(((unsigned)b_hi * G + nb - 1) / nb) - 1;
+#pragma unroll 2
+  for (int wg = w0; wg <= w1; ++wg) {
+    const int lo = (int)((unsigned)wg * nb / G);
+    const int slot = gi - (blk0_mod + lo) / F.A;
+    const float* row = F.wpart + ((long long)wg * F.wslots + slot) * F.wstride;
+    if (nyq) {
+      s.x += row[i0];
+    } else {
+      const float4 v = *reinterpret_cast<const float4*>(row + i0);
+      s.x += v.x;
+      s.y += v.y;
+      s.z += v.z;
+      s.w += v.w;
+    }
+  }
+  if (g_hi > F.nblocks) {
+    if (nyq) F.carry_out[i0] = s.x;
+    else *reinterpret_cast<float4*>(F.carry_out + i0) = s;
+    return;
+  }
+  const float sv[4] = {s.x, s.y, s.z, s.w};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (nyq && j > 0) break;
+    const long long o = (long long)gi * F.K + welch_row_bin(i0 + j, F.M, F.R);
+    F.out[o] = sv[j] / (float)F.A;
+    if (F.qfmt) quantize_bin(F.qout, F.qfmt, o, sv[j], F.A);
+  }
+}
+
 // Persistent workgroups: workgroup w takes blocks [w nb / G, (w+1) nb / G)
 // in order, so its Welch sums stay in a private partial row (slot = Welch
-// group - first group of the workgroup) that is written with plain stores;
-// welch_finalize_kernel adds the rows of every group.
+// group - first group of the workgroup) that is written with plain stores.
+// The rows of every group are added in the next launch (workgroups that are
+// done with their blocks take chunks of the previous call's groups) or by
+// welch_finalize_kernel when the records are asked for first.
 //
 // After the last pass every lane owns R bins of the real spectrum: lower
 // lanes pi + P3 q, upper lanes ksrc + P3 (R-1-q) (slot q).  A bin's
@@ -292,9 +412,10 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   constexpr int W = C::W;
   constexpr int P3 = C::P3;
   constexpr bool SPLIT = C::SPLIT;
-  __shared__ __align__(16) float lds[C::LDS_FLOATS + C::STASH_FLOATS + C::DUMMY_FLOATS + C::EDGE_PER_THREAD * T];
+  __shared__ __align__(16) float lds[C::LDS_FLOATS + C::STASH_FLOATS + C::DUMMY_FLOATS + C::EDGE_PER_THREAD * T + 1];
   cpx* stash = reinterpret_cast<cpx*>(lds + C::LDS_FLOATS);
   float* eacc = lds + C::LDS_FLOATS + C::STASH_FLOATS + C::DUMMY_FLOATS;  // edge item sums
+  int* ticket_slot = reinterpret_cast<int*>(eacc + C::EDGE_PER_THREAD * T);
 
   constexpr int NI = C::EDGE_PER_THREAD;
   const int G = gridDim.x, wg = blockIdx.x;
@@ -303,6 +424,10 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   const long long block0 = *P.pos;
   const long long grp0 = (block0 + lo) / P.A;
   const float w = P.wscale;
+  if (wg == 0) {
+    if ((int)threadIdx.x < P.ncounters) P.chan_counters[threadIdx.x] = 0;
+    if (threadIdx.x == 0 && P.pos_next) *P.pos_next = block0 + P.nblocks;
+  }
 
   // Raw s16 pairs of the block about to be transformed: point n = b + j M/R1
   // of pass 1 is the pair at block sample 2n.  Rows are addressed as (uniform
@@ -600,6 +725,38 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
       edge_finish();
     }
   }
+
+  // The next call's input history: block 0 (this workgroup's first block)
+  // was the only reader of the current one.
+  if (wg == 0) {
+    for (int i = (int)threadIdx.x; i < P.hist_words; i += T) P.hist_dst[i] = P.hist_src[i];
+  }
+
+  // The previous call's Welch records: its partial rows are complete (they
+  // were written by the previous launch), so the workgroups that finish
+  // their blocks first take them, one chunk (4 T positions of one group) at
+  // a time.  Every workgroup ends with exactly one failed take, so the take
+  // numbered chunks + G - 1 is the launch's last and resets the counter.
+  if (P.fin.nblocks > 0) {
+    // (its position slot is this launch's pos_next, already rewritten)
+    const long long fpos = block0 - P.fin.nblocks;
+    const int ngroups = (int)(fpos % P.fin.A + P.fin.nblocks - 1) / P.fin.A + 1;
+    constexpr int CPG = R / 4;  // chunks per group (M = R T positions + the Nyquist position)
+    const int nchunks = ngroups * CPG;
+    for (;;) {
+      __syncthreads();  // the previous ticket has been read
+      if (threadIdx.x == 0) *ticket_slot = atomicAdd(P.fin_ticket, 1);
+      __syncthreads();
+      const int ticket = *ticket_slot;
+      if (ticket >= nchunks) {
+        if (threadIdx.x == 0 && ticket == nchunks + G - 1) *P.fin_ticket = 0;
+        break;
+      }
+      const int gi = ticket / CPG, c = ticket % CPG;
+      welch_finalize_quad(P.fin, fpos, gi, 4 * (c * T + (int)threadIdx.x));
+      if (c == CPG - 1 && threadIdx.x == 0) welch_finalize_quad(P.fin, fpos, gi, M);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -824,60 +981,10 @@ __global__ __launch_bounds__(kChanThreads) void chan_batched_kernel(BatchParams 
   }
 }
 
-// ---------------------------------------------------------------------------
-// Welch records (spektri/src/dsp/spectrum.rs:113-160).  For every group s
-// touched by this launch: sum the partial rows of the workgroups that
-// processed its blocks (plus the carry of a group begun in an earlier
-// launch); a complete group becomes the record sum / A, an incomplete last
-// group goes back to the carry row.
-// ---------------------------------------------------------------------------
-struct FinParams {
-  const float* wpart;
-  long long wstride;
-  int wslots;
-  int G;               // workgroups of the forward launch
-  int nblocks;
-  const long long* pos;  // device stream position of the launch's first block
-  long long* pos_next;   // the next call's position slot (pos + nblocks is written there)
-  int A, K, M, R;
-  // Sums of the Welch group left open at the end of the previous call (read)
-  // and of this call (written), in partial-row order, M + 1 floats each.
-  // Two buffers alternated per call: the first group reads one while the
-  // last group writes the other.
-  const float* carry_in;
-  float* carry_out;
-  float* out;          // records [r][K]
-  // input history for the next call: the last hist_words 32-bit words of
-  // this call's input
-  const int* hist_src;
-  int* hist_dst;
-  int hist_words;
-  int* chan_counters;  // queue counters of chan_batched_kernel, reset here
-  int ncounters;
-  unsigned char* qout;  // quantised records (qfmt 1: K bytes, 2: K big-endian u16) or null
-  int qfmt;
-};
-
-// dB quantiser of one accumulated bin, spectrum.rs:128-152 in f32:
-// dB = 10 log10(acc) - 10 log10(n); U8 = clamp(2 dB + 250, 0, 255),
-// U16 = clamp(20 dB + 4000, 0, 4095) big-endian; NaN and -inf clamp to 0,
-// the conversion truncates.
-__device__ __forceinline__ void quantize_bin(unsigned char* q, int qfmt, long long idx, float acc, int n) {
-  const float db = log10f(acc) * 10.0f + log10f((float)n) * -10.0f;
-  if (qfmt == 1) {
-    const float v = fminf(fmaxf(db * 2.0f + 250.0f, 0.0f), 255.0f);
-    q[idx] = (unsigned char)(unsigned)v;
-  } else {
-    const float v = fminf(fmaxf(db * 20.0f + 4000.0f, 0.0f), 4095.0f);
-    const unsigned o = (unsigned)v;
-    q[2 * idx] = (unsigned char)(o >> 8);
-    q[2 * idx + 1] = (unsigned char)(o & 0xFF);
-  }
-}
-
-// Grid: x over bins, y over the Welch groups the launch touches (group 0 is
-// the group of the launch's first block).  Row y = 0 also saves the input
-// history for the next call.
+// Grid: x over quads of partial-row positions, y over the Welch groups the
+// launch touches (group 0 is the group of the launch's first block).  Row
+// y = 0 also does the next call's housekeeping the fields ask for (input
+// history, queue counters, position slot).
 __global__ void welch_finalize_kernel(FinParams F) {
   const int gi = (int)blockIdx.y;
   const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -885,56 +992,13 @@ __global__ void welch_finalize_kernel(FinParams F) {
   if (gi == 0) {
     for (int i = tid; i < F.hist_words; i += (int)(gridDim.x * blockDim.x)) F.hist_dst[i] = F.hist_src[i];
     if (tid < F.ncounters) F.chan_counters[tid] = 0;
-    if (tid == 0) *F.pos_next = pos + F.nblocks;
+    if (tid == 0 && F.pos_next) *F.pos_next = pos + F.nblocks;
   }
-  // Each thread sums four consecutive partial-row positions (16-byte loads)
-  // over the workgroups of its group and writes the four bins they hold;
-  // position M (the Nyquist bin, M % 4 == 0) is a quad of its own.
   const int i0 = 4 * tid;
   const int blk0_mod = (int)(pos % F.A);
   // rows beyond the groups this launch touches (the grid is sized for the worst case)
   if (i0 > F.M || gi > (blk0_mod + F.nblocks - 1) / F.A) return;
-  const bool nyq = i0 == F.M;
-  const int g_lo = gi * F.A - blk0_mod, g_hi = g_lo + F.A;
-  const int b_lo = g_lo > 0 ? g_lo : 0;
-  const int b_hi = g_hi < F.nblocks ? g_hi : F.nblocks;
-  float4 s = {0.f, 0.f, 0.f, 0.f};
-  if (gi == 0 && blk0_mod != 0) {
-    if (nyq) s.x = F.carry_in[i0];
-    else s = *reinterpret_cast<const float4*>(F.carry_in + i0);
-  }
-  // workgroup of block b: the w with floor(w nb / G) <= b < floor((w+1) nb / G)
-  const unsigned nb = (unsigned)F.nblocks, G = (unsigned)F.G;
-  const int w0 = (int)(((unsigned)(b_lo + 1) * G + nb - 1) / nb) - 1;
-  const int w1 = (int)(((unsigned)b_hi * G + nb - 1) / nb) - 1;
-#pragma unroll 2
-  for (int wg = w0; wg <= w1; ++wg) {
-    const int lo = (int)((unsigned)wg * nb / G);
-    const int slot = gi - (blk0_mod + lo) / F.A;
-    const float* row = F.wpart + ((long long)wg * F.wslots + slot) * F.wstride;
-    if (nyq) {
-      s.x += row[i0];
-    } else {
-      const float4 v = *reinterpret_cast<const float4*>(row + i0);
-      s.x += v.x;
-      s.y += v.y;
-      s.z += v.z;
-      s.w += v.w;
-    }
-  }
-  if (g_hi > F.nblocks) {
-    if (nyq) F.carry_out[i0] = s.x;
-    else *reinterpret_cast<float4*>(F.carry_out + i0) = s;
-    return;
-  }
-  const float sv[4] = {s.x, s.y, s.z, s.w};
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    if (nyq && j > 0) break;
-    const long long o = (long long)gi * F.K + welch_row_bin(i0 + j, F.M, F.R);
-    F.out[o] = sv[j] / (float)F.A;
-    if (F.qfmt) quantize_bin(F.qout, F.qfmt, o, sv[j], F.A);
-  }
+  welch_finalize_quad(F, pos, gi, i0);
 }
 
 // ---------------------------------------------------------------------------

@@ -169,9 +169,15 @@ struct spk_ctx {
   DevBuf qout;               // quantised Welch records (SPK_FLAG_SPECTRUM_U8 / _U16)
   int qfmt = 0;              // 0 none, 1 U8, 2 U16
   bool use_graphs = false;
-  std::map<std::tuple<const void*, int, int>, hipGraphExec_t> graphs;
+  std::map<std::tuple<const void*, int, int, int>, hipGraphExec_t> graphs;
   long long spec_stride = 0, wstride = 0;
   int wslots = 0, welch_out_cap = 0;
+  long long wpart_half = 0;  // floats per parity of the partial rows (real input)
+  // Real input: the call whose Welch records are not summed yet (its rows
+  // are summed by the next forward launch, or by spk::welch_finalize_kernel
+  // on flush).  pend_nb == 0: none.
+  int pend_nb = 0, pend_G = 0, pend_parity = 0;
+  DevBuf fin_ticket;          // chunk counter of the forward kernel's record phase
   int gmax[2] = {0, 0};      // persistent workgroups of the unfused / fused forward kernel
   bool keep_spec = false;
   bool fused = false;        // channels run inside the forward kernel
@@ -655,6 +661,45 @@ spk_status enqueue_channels(spk_ctx* c, int nb, hipStream_t s, const ChanGroup* 
   return SPK_OK;
 }
 
+// Workgroups of the real forward launch over nb blocks.
+int fwd_grid(const spk_ctx* c, int nb) { return std::min(nb, c->gmax[c->fused ? 1 : 0]); }
+
+// Welch record parameters of the real-input call over nb blocks launched
+// with G workgroups at the given parity (its position slot, partial rows
+// and carry buffers).
+spk::FinParams fin_params(const spk_ctx* c, int nb, int G, int parity) {
+  spk::FinParams F{};
+  F.wpart = c->wpart.as<float>() + parity * c->wpart_half;
+  F.wstride = c->wstride;
+  F.wslots = c->wslots;
+  F.G = G;
+  F.nblocks = nb;
+  F.pos = c->pos.as<long long>() + parity;
+  F.A = c->A;
+  F.K = c->K;
+  F.M = c->M;
+  F.R = c->fplan->R;
+  F.carry_in = c->carry.as<float>() + (long long)parity * c->spec_stride;
+  F.carry_out = c->carry.as<float>() + (long long)(parity ^ 1) * c->spec_stride;
+  F.out = c->welch_out.as<float>();
+  F.qout = c->qout.as<unsigned char>();
+  F.qfmt = c->qfmt;
+  return F;
+}
+
+// Sum the pending call's Welch records now (its partial rows are complete
+// once the launches before this one on the stream have run).
+spk_status flush_records(spk_ctx* c) {
+  if (c->complex_in || c->pend_nb == 0) return SPK_OK;
+  const spk::FinParams F = fin_params(c, c->pend_nb, c->pend_G, c->pend_parity);
+  // groups touched: at most (A - 1 + nb - 1) / A + 1; rows past the actual count exit
+  const int gmax = (c->A - 1 + c->pend_nb - 1) / c->A + 1;
+  hipLaunchKernelGGL(spk::welch_finalize_kernel, dim3((c->M / 4 + 1 + 255) / 256, gmax), dim3(256), 0, c->stream, F);
+  HIP_TRY(c, hipGetLastError());
+  c->pend_nb = 0;
+  return SPK_OK;
+}
+
 spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hipEvent_t* ev) {
   if (c->complex_in) return enqueue_complex(c, src, nb, s, ev);
   spk::FwdParams P{};
@@ -669,7 +714,7 @@ spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hi
   P.tw2 = c->ftw2.as<cpx>();
   P.tw3 = c->ftw3.as<cpx>();
   P.twu = c->ftwu.as<cpx>();
-  P.wpart = c->wpart.as<float>();
+  P.wpart = c->wpart.as<float>() + c->parity * c->wpart_half;
   P.wstride = c->wstride;
   P.wslots = c->wslots;
   P.A = c->A;
@@ -691,8 +736,17 @@ spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hi
     P.fout = c->chan_out.as<cpx>();
     P.ftw2 = fg->tw2.as<cpx>();
   }
+  // Next call's housekeeping (workgroup 0) and the pending call's records.
+  P.pos_next = c->pos.as<long long>() + (c->parity ^ 1);
+  P.hist_src = reinterpret_cast<const int*>(reinterpret_cast<const char*>(src) + ((long long)nb * c->H - c->O) * c->bps);
+  P.hist_dst = c->hist.as<int>();
+  P.hist_words = c->O * c->bps / 4;
+  P.chan_counters = c->chan_counters.as<int>();
+  P.ncounters = kXcdQueues;
+  if (c->pend_nb > 0) P.fin = fin_params(c, c->pend_nb, c->pend_G, c->pend_parity);
+  P.fin_ticket = c->fin_ticket.as<int>();
   const FwdVariant* fv = find_fwd_variant(c->M, c->fused ? kFusedL : 0);
-  const int G = std::min(nb, c->gmax[c->fused ? 1 : 0]);
+  const int G = fwd_grid(c, nb);
   if (ev) HIP_TRY(c, hipEventRecord(ev[0], s));
   HIP_TRY(c, fv->launch(P, G, s));
   if (ev) HIP_TRY(c, hipEventRecord(ev[1], s));
@@ -702,37 +756,7 @@ spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hi
     if (st != SPK_OK) return st;
   }
   if (ev) HIP_TRY(c, hipEventRecord(ev[2], s));
-
-  {
-    spk::FinParams Fp{};
-    Fp.wpart = c->wpart.as<float>();
-    Fp.wstride = c->wstride;
-    Fp.wslots = c->wslots;
-    Fp.G = G;
-    Fp.nblocks = nb;
-    Fp.pos = c->pos.as<long long>() + c->parity;
-    Fp.pos_next = c->pos.as<long long>() + (c->parity ^ 1);
-    Fp.A = c->A;
-    Fp.K = c->K;
-    Fp.M = c->M;
-    Fp.R = c->fplan->R;
-    Fp.carry_in = c->carry.as<float>() + (long long)c->parity * c->spec_stride;
-    Fp.carry_out = c->carry.as<float>() + (long long)(c->parity ^ 1) * c->spec_stride;
-    Fp.out = c->welch_out.as<float>();
-    // New history: the last O samples consumed.
-    Fp.hist_src = reinterpret_cast<const int*>(reinterpret_cast<const char*>(src) +
-                                               ((long long)nb * c->H - c->O) * c->bps);
-    Fp.hist_dst = c->hist.as<int>();
-    Fp.hist_words = c->O * c->bps / 4;
-    Fp.chan_counters = c->chan_counters.as<int>();
-    Fp.ncounters = kXcdQueues;
-    Fp.qout = c->qout.as<unsigned char>();
-    Fp.qfmt = c->qfmt;
-    // groups touched: at most (A - 1 + nb - 1) / A + 1; rows past the actual count exit
-    const int gmax = (c->A - 1 + nb - 1) / c->A + 1;
-    hipLaunchKernelGGL(spk::welch_finalize_kernel, dim3((c->M / 4 + 1 + 255) / 256, gmax), dim3(256), 0, s, Fp);
-    HIP_TRY(c, hipGetLastError());
-  }
+  // (the records of this call are summed by the next forward launch or on flush)
   if (ev) HIP_TRY(c, hipEventRecord(ev[3], s));
   return SPK_OK;
 }
@@ -763,7 +787,7 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
   }
   hipStream_t s = c->stream;
   if (c->use_graphs && !c->timing && graph_pays(c)) {
-    const auto key = std::make_tuple(src, nb, c->parity);
+    const auto key = std::make_tuple(src, nb, c->parity, c->pend_nb);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
       if (c->graphs.size() >= 8) drop_graphs(c);
@@ -802,6 +826,11 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
     }
     spk_status st = enqueue_blocks(c, src, nb, s, ev);
     if (st != SPK_OK) return st;
+  }
+  if (!c->complex_in) {
+    c->pend_nb = nb;
+    c->pend_G = fwd_grid(c, nb);
+    c->pend_parity = c->parity;
   }
   const long long s_first = c->blocks_done / c->A;
   c->last_records = (uint32_t)((c->blocks_done + nb) / c->A - s_first);
@@ -933,13 +962,15 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   if (c->hist.alloc((size_t)c->O * c->bps) != hipSuccess ||
       c->staging.alloc((size_t)c->F * c->H * c->bps) != hipSuccess ||
       c->spec.alloc((size_t)std::max(c->max_blocks, gcap) * c->spec_stride * sizeof(cpx)) != hipSuccess ||
-      c->wpart.alloc((size_t)gcap * c->wslots * c->wstride * sizeof(float)) != hipSuccess ||
+      c->wpart.alloc((size_t)2 * gcap * c->wslots * c->wstride * sizeof(float)) != hipSuccess ||
       c->edump.alloc((size_t)gcap * (c->M / c->fplan->R / 64) * spk::kEdgeLanes * c->fplan->R * sizeof(cpx)) !=
           hipSuccess ||
       c->carry.alloc((size_t)2 * c->spec_stride * sizeof(float)) != hipSuccess ||
       c->welch_out.alloc((size_t)c->welch_out_cap * c->K * sizeof(float)) != hipSuccess)
     return cleanup(SPK_ERR_DEVICE);
-  if (hipMemset(c->hist.p, 0, c->hist.bytes) != hipSuccess || hipMemset(c->carry.p, 0, c->carry.bytes) != hipSuccess)
+  c->wpart_half = (long long)gcap * c->wslots * c->wstride;
+  if (hipMemset(c->hist.p, 0, c->hist.bytes) != hipSuccess || hipMemset(c->carry.p, 0, c->carry.bytes) != hipSuccess ||
+      c->fin_ticket.alloc(sizeof(int)) != hipSuccess || hipMemset(c->fin_ticket.p, 0, sizeof(int)) != hipSuccess)
     return cleanup(SPK_ERR_DEVICE);
 
   const FwdPlan& p = *c->fplan;
@@ -1006,6 +1037,7 @@ spk_status spk_process(spk_ctx* c, const void* raw_new, size_t nbytes) {
   if (nbytes != want) return fail(c, SPK_ERR_INVALID, "buffer must hold exactly F * 3N/4 samples");
   HIP_TRY(c, hipMemcpyAsync(c->staging.p, raw_new, nbytes, hipMemcpyHostToDevice, c->stream));
   spk_status st = run_blocks(c, c->staging.p, c->F);
+  if (st == SPK_OK) st = flush_records(c);
   if (st != SPK_OK) return st;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return SPK_OK;
@@ -1049,6 +1081,10 @@ spk_status spk_channel_output(spk_ctx* c, uint32_t id, float* out, size_t cap, s
 
 spk_status spk_spectrum_records_device(spk_ctx* c, const float** dptr, uint32_t* nrec) {
   if (!c || !dptr || !nrec) return SPK_ERR_INVALID;
+  {
+    spk_status st = flush_records(c);
+    if (st != SPK_OK) return st;
+  }
   *dptr = c->welch_out.as<float>();
   *nrec = c->have_output ? c->last_records : 0;
   return SPK_OK;
@@ -1070,6 +1106,10 @@ spk_status spk_spectrum_records(spk_ctx* c, float* out, size_t cap, uint32_t* nr
 spk_status spk_spectrum_quantized_device(spk_ctx* c, const void** dptr, uint32_t* nrec, uint32_t* bytes_per_bin) {
   if (!c || !dptr || !nrec || !bytes_per_bin) return SPK_ERR_INVALID;
   if (!c->qfmt) return fail(c, SPK_ERR_STATE, "context created without SPK_FLAG_SPECTRUM_U8 / _U16");
+  {
+    spk_status st = flush_records(c);
+    if (st != SPK_OK) return st;
+  }
   *dptr = c->qout.p;
   *nrec = c->have_output ? c->last_records : 0;
   *bytes_per_bin = (uint32_t)c->qfmt;
@@ -1079,6 +1119,10 @@ spk_status spk_spectrum_quantized_device(spk_ctx* c, const void** dptr, uint32_t
 spk_status spk_spectrum_quantized(spk_ctx* c, void* out, size_t cap, uint32_t* nrec) {
   if (!c || !nrec) return SPK_ERR_INVALID;
   if (!c->qfmt) return fail(c, SPK_ERR_STATE, "context created without SPK_FLAG_SPECTRUM_U8 / _U16");
+  {
+    spk_status st = flush_records(c);
+    if (st != SPK_OK) return st;
+  }
   const uint32_t n = c->have_output ? c->last_records : 0;
   const size_t need = (size_t)n * c->K * c->qfmt;
   if (need > cap) return fail(c, SPK_ERR_CAPACITY, "output buffer too small");
@@ -1105,6 +1149,7 @@ spk_status spk_reset(spk_ctx* c) {
   HIP_TRY(c, hipMemset(c->carry.p, 0, c->carry.bytes));
   HIP_TRY(c, hipMemset(c->pos.p, 0, c->pos.bytes));
   c->parity = 0;
+  c->pend_nb = 0;  // dropped with the rest of the stream state
   c->blocks_done = 0;
   c->last_blocks = 0;
   c->last_records = 0;
@@ -1114,6 +1159,10 @@ spk_status spk_reset(spk_ctx* c) {
 
 spk_status spk_set_stream(spk_ctx* c, void* stream) {
   if (!c) return SPK_ERR_INVALID;
+  {
+    spk_status st = flush_records(c);
+    if (st != SPK_OK) return st;
+  }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   c->stream = stream ? reinterpret_cast<hipStream_t>(stream) : c->own_stream;
   drop_graphs(c);
@@ -1122,8 +1171,15 @@ spk_status spk_set_stream(spk_ctx* c, void* stream) {
 
 spk_status spk_synchronize(spk_ctx* c) {
   if (!c) return SPK_ERR_INVALID;
+  spk_status st = flush_records(c);
+  if (st != SPK_OK) return st;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return SPK_OK;
+}
+
+spk_status spk_flush(spk_ctx* c) {
+  if (!c) return SPK_ERR_INVALID;
+  return flush_records(c);
 }
 
 spk_status spk_enable_timing(spk_ctx* c, int on) {

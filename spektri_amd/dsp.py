@@ -164,6 +164,11 @@ class DspState:
     def synchronize(self):
         _lib.check(_lib.load().spk_synchronize(self._ctx), self._ctx)
 
+    def flush(self):
+        """Enqueue the sum of the last call's Welch records if still pending
+        (asynchronous; ``synchronize`` and the record readers do it too)."""
+        _lib.check(_lib.load().spk_flush(self._ctx), self._ctx)
+
     def reset(self):
         _lib.check(_lib.load().spk_reset(self._ctx), self._ctx)
 

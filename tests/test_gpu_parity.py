@@ -166,6 +166,49 @@ def test_streaming_host_buffers_carry_history_and_welch():
     st.close()
 
 
+@pytest.mark.parametrize("N,fused", [(16384, True), (16384, False), (65536, True)])
+def test_back_to_back_calls_defer_welch_records(N, fused):
+    """Device calls enqueued back to back: each call's Welch records are
+    summed inside the next call's forward launch (groups straddling calls,
+    so the carry passes through those sums), the last call's on flush.
+    Channel outputs of every call and the final records equal those of calls
+    synchronised one by one (bit for bit), which match the oracle."""
+    rng = np.random.default_rng(9)
+    F, A, calls = 4, 6, 6
+    K, H = N // 2 + 1, N - N // 4
+    x = np.clip(np.round(rng.standard_normal(calls * F * H) * 4000), -32768, 32767).astype(np.int16)
+    # a channel reaching below bin 0 cannot run inside the forward kernel
+    chans = [(300 if fused else -100, 256, rng.random(256).astype(np.float32)),
+             (K // 2, 256, rng.random(256).astype(np.float32))]
+    dev = torch.from_numpy(x).to("cuda")
+    seg = F * H
+    res = {}
+    for sync_each in (True, False):
+        st, _ = DspState.init(DspParams(complex=False, fs_in=1.0, fft_size=N, ffts_per_buf=F, spectrum_averages=A,
+                                        max_blocks=F))
+        ids = [st.add_channel(first, L, w) for first, L, w in chans]
+        outs, recs = [], []
+        for k in range(calls):
+            st.process_device(dev[k * seg:].data_ptr(), seg)
+            if sync_each:
+                st.synchronize()
+                recs.extend(list(st.spectrum_records()))
+            outs.append([st.channel_output(i) for i in ids])  # waits for the call, leaves its records pending
+        last = st.spectrum_records()
+        res[sync_each] = (outs, recs, last, st.fused_channels())
+        st.close()
+    assert res[True][3] == res[False][3] == (2 if fused else 0)
+    same_out = all(np.array_equal(a, b) for ca, cb in zip(res[True][0], res[False][0]) for a, b in zip(ca, cb))
+    same_last = np.array_equal(res[True][2], res[False][2])
+    X, orecs, oouts = oracle_path(x, N, F, A, chans)
+    e_w = rms(np.array(res[True][1]), orecs)
+    e_c = max(rms(np.concatenate([o[ci] for o in res[False][0]]), oouts[ci]) for ci in range(len(chans)))
+    report(f"back_to_back_{N}_{'fused' if fused else 'unfused'}", calls=calls, identical_to_synced=same_out and same_last,
+           welch_rms=e_w, channel_rms_max=e_c, last_call_records=len(res[False][2]))
+    assert same_out and same_last and len(res[True][1]) == len(orecs)
+    assert e_w < TOL and e_c < TOL
+
+
 def test_reset_restores_fresh_state():
     rng = np.random.default_rng(3)
     N, F, A = 8192, 2, 3
