@@ -240,6 +240,9 @@ int fused_lds_dummy(int M);
 
 // Lay out channel outputs and descriptors, upload weights and tables.
 spk_status rebuild_channels(spk_ctx* c) {
+  // Tables are uploaded with null-stream copies, which do not wait for the
+  // context stream: no launch of an earlier call may still read them.
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
   std::vector<float> wall;
   c->chan_out_off.assign(c->chans.size(), 0);
   long long off = 0;
@@ -453,6 +456,7 @@ std::vector<int> batched_groups(const spk_ctx* c) {
 
 // Cut the (block, channel) tiles of nb blocks into per-XCD item queues.
 spk_status build_batch(spk_ctx* c, int nb) {
+  HIP_TRY(c, hipStreamSynchronize(c->stream));  // as in rebuild_channels
   BatchPlan& B = c->batch;
   std::vector<int> ls = batched_groups(c);
   std::sort(ls.begin(), ls.end(), std::greater<int>());  // big tiles first within a chunk
@@ -942,6 +946,9 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
         upload(c->ctw3, pass_table(p.R3, p.R1 * p.R2, -1)) != hipSuccess || upload(c->cta, ta) != hipSuccess ||
         upload(c->ctb, tb) != hipSuccess || upload(c->ctw64, t64) != hipSuccess)
       return cleanup(SPK_ERR_DEVICE);
+    // the null-stream clears above must be done before work on the
+    // (non-blocking) context stream
+    if (hipDeviceSynchronize() != hipSuccess) return cleanup(SPK_ERR_DEVICE);
     *out = c;
     return SPK_OK;
   }
@@ -985,6 +992,9 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   if (upload(c->ftw2, pass_table(p.R2, p.R1, -1)) != hipSuccess ||
       upload(c->ftw3, pass_table(p.R, p.R1 * p.R2, -1)) != hipSuccess || upload(c->ftwu, twu) != hipSuccess)
     return cleanup(SPK_ERR_DEVICE);
+  // the null-stream clears above must be done before work on the
+  // (non-blocking) context stream
+  if (hipDeviceSynchronize() != hipSuccess) return cleanup(SPK_ERR_DEVICE);
   *out = c;
   return SPK_OK;
 }
@@ -1145,9 +1155,10 @@ spk_status spk_spectra_device(spk_ctx* c, const float** dptr, uint64_t* stride) 
 spk_status spk_reset(spk_ctx* c) {
   if (!c) return SPK_ERR_INVALID;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  HIP_TRY(c, hipMemset(c->hist.p, 0, c->hist.bytes));
-  HIP_TRY(c, hipMemset(c->carry.p, 0, c->carry.bytes));
-  HIP_TRY(c, hipMemset(c->pos.p, 0, c->pos.bytes));
+  // On the context's stream: it does not wait for the null stream.
+  HIP_TRY(c, hipMemsetAsync(c->hist.p, 0, c->hist.bytes, c->stream));
+  HIP_TRY(c, hipMemsetAsync(c->carry.p, 0, c->carry.bytes, c->stream));
+  HIP_TRY(c, hipMemsetAsync(c->pos.p, 0, c->pos.bytes, c->stream));
   c->parity = 0;
   c->pend_nb = 0;  // dropped with the rest of the stream state
   c->blocks_done = 0;
