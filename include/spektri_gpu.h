@@ -36,17 +36,31 @@ typedef enum spk_status {
   SPK_ERR_CAPACITY = 5      /* more blocks / records than the context was sized for */
 } spk_status;
 
-/* DataFormat codes of spektri/src/dsp/data.rs:81-95. */
+/* DataFormat codes of spektri/src/dsp/data.rs:81-95; every one is an input
+ * format (inputformats.rs:12-155).  s16le, f32le and cf32le are read in
+ * place by the transform kernels; the others pass through one conversion
+ * kernel first (into s16, f32 or cf32, with the reference's arithmetic:
+ * u8/cu8 subtract 127.4 in f32, big-endian formats swap bytes). */
+#define SPK_FMT_S8 0x04
 #define SPK_FMT_S16LE 0x08
+#define SPK_FMT_S16BE 0x09
+#define SPK_FMT_F32LE 0x1C
+#define SPK_FMT_F32BE 0x1D
+#define SPK_FMT_U8 0x24
+#define SPK_FMT_CS8 0x44
+#define SPK_FMT_CS16LE 0x48
+#define SPK_FMT_CS16BE 0x49
 #define SPK_FMT_CF32LE 0x5C
+#define SPK_FMT_CF32BE 0x5D
+#define SPK_FMT_CU8 0x64
 
 /* Mirrors DspParams (spektri/src/dsp.rs:24-36) minus fs/fc, which only
  * the frequency<->bin helpers below need, and minus the filter list, which
  * is added with spk_add_channel. */
 typedef struct spk_config {
-  uint32_t fft_size;          /* N: 4096..65536 (s16 real input), 4096..262144 (cf32 complex input) */
-  uint32_t input_format;      /* SPK_FMT_S16LE or SPK_FMT_CF32LE; other DataFormat codes: SPK_ERR_UNSUPPORTED */
-  float scaling;              /* input scaling; 0 = input_format_scaling(fmt) */
+  uint32_t fft_size;          /* N: 4096..65536 (real input), 4096..262144 (complex input) */
+  uint32_t input_format;      /* SPK_FMT_*: real or complex input follows the format (main.rs:35) */
+  float scaling;              /* input scaling; 0 = input_format_scaling(fmt) (inputformats.rs:47-62) */
   uint32_t ffts_per_buf;      /* F: blocks per spk_process call */
   uint32_t spectrum_averages; /* A: blocks per Welch record */
   uint32_t max_blocks;        /* capacity for spk_process_device; 0 = F */
@@ -97,8 +111,9 @@ spk_status spk_add_channel(spk_ctx* ctx, int64_t first_bin, uint32_t bins, const
 spk_status spk_set_channel_weights(spk_ctx* ctx, uint32_t id, const float* weights);
 
 /* DspState::process_real / process_complex (spektri/src/dsp.rs:114-160) on one
- * buffer: exactly F * 3N/4 new raw samples from host memory (nbytes must
- * match).  The N/4-sample history is kept on the device (zeros at start,
+ * buffer: exactly F * 3N/4 new raw samples of the input format from host
+ * memory (nbytes = F * 3N/4 * bytes_per_input_sample, inputformats.rs:12-27).
+ * The N/4-sample history is kept on the device (zeros at start,
  * main.rs:165).  Synchronous. */
 spk_status spk_process(spk_ctx* ctx, const void* raw_new, size_t nbytes);
 

@@ -14,8 +14,11 @@ LIB_PATH = os.path.join(HERE, "libspektri_gpu.so")
 SPK_OK = 0
 STATUS_NAMES = {0: "SPK_OK", 1: "SPK_ERR_INVALID", 2: "SPK_ERR_UNSUPPORTED", 3: "SPK_ERR_DEVICE",
                 4: "SPK_ERR_STATE", 5: "SPK_ERR_CAPACITY"}
-FMT_S16LE = 0x08
-FMT_CF32LE = 0x5C
+# DataFormat codes (spektri/src/dsp/data.rs:81-95); every one is accepted as input.
+FMT_S8, FMT_S16LE, FMT_S16BE, FMT_F32LE, FMT_F32BE, FMT_U8 = 0x04, 0x08, 0x09, 0x1C, 0x1D, 0x24
+FMT_CS8, FMT_CS16LE, FMT_CS16BE, FMT_CF32LE, FMT_CF32BE, FMT_CU8 = 0x44, 0x48, 0x49, 0x5C, 0x5D, 0x64
+REAL_FORMATS = (FMT_S8, FMT_S16LE, FMT_S16BE, FMT_F32LE, FMT_F32BE, FMT_U8)
+COMPLEX_FORMATS = (FMT_CS8, FMT_CS16LE, FMT_CS16BE, FMT_CF32LE, FMT_CF32BE, FMT_CU8)
 FLAG_KEEP_SPECTRA = 1
 FLAG_GRAPHS = 2
 FLAG_SPECTRUM_U8 = 4

@@ -95,8 +95,8 @@ struct FinParams {
 };
 
 struct FwdParams {
-  const int16_t* src;   // stream samples; sample s >= 0 at src[s]
-  const int16_t* hist;  // the O samples before src[0] (block 0 reads them, then src[0 .. N-O))
+  const void* src;      // stream samples (s16, or f32 for the f32 loader); sample s >= 0 at src[s]
+  const void* hist;     // the O samples before src[0] (block 0 reads them, then src[0 .. N-O))
   cpx* spec;            // per-block spectrum rows (written when keep_spec)
   long long spec_stride;
   int keep_spec;
@@ -404,7 +404,7 @@ __device__ __forceinline__ void welch_finalize_quad(const FinParams& F, long lon
 // after a barrier from the edge table.  Channels of the fused stage are
 // gathered by scattering each owned bin into an LDS tile buffer through the
 // bin -> position map.
-template <int M, int R, int R1, int R2, int FL>
+template <int M, int R, int R1, int R2, int FL, bool F32>
 __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   using C = FwdRealCfg<M, R, R1, R2, FL>;
   using FC = FusedCfg<FL>;
@@ -433,15 +433,20 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   // of pass 1 is the pair at block sample 2n.  Rows are addressed as (uniform
   // row pointer) + lane offset.  The next block's pairs are loaded while the
   // current block's channel stage runs.
-  auto load_block = [&](int g, int (&vin)[R]) {
+  // A pair is one 32-bit word of two s16 samples, or (F32) one float2.
+  using Word = std::conditional_t<F32, float2, int>;
+  constexpr int kSampleBytes = F32 ? 4 : 2;
+  auto load_block = [&](int g, Word (&vin)[R]) {
     // Block 0 starts with the O = M/2 history samples (M/4 pairs).  Every
     // point's pair range [T u + j M/R1, + T) lies wholly on one side of that
     // seam (M/4 is a multiple of T), so each point reads from one of two
     // uniform row pointers, chosen at compile time.
     // (typed pointers, not integers: the loads keep the global address space)
-    const int* blk = reinterpret_cast<const int*>(P.src + (g == 0 ? 0 : (long long)g * P.H - P.O));
-    const int* lo_row = g == 0 ? reinterpret_cast<const int*>(P.hist) : blk;
-    const int* hi_row = g == 0 ? reinterpret_cast<const int*>(P.src) : blk;
+    const Word* src = static_cast<const Word*>(P.src);
+    const Word* blk = reinterpret_cast<const Word*>(static_cast<const char*>(P.src) +
+                                                    (g == 0 ? 0 : (long long)g * P.H - P.O) * kSampleBytes);
+    const Word* lo_row = g == 0 ? static_cast<const Word*>(P.hist) : blk;
+    const Word* hi_row = g == 0 ? src : blk;
     const int hi_off = g == 0 ? -(M / 4) : 0;
 #pragma unroll
     for (int e = 0; e < R; ++e) {
@@ -453,7 +458,7 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   // 1024-thread workgroups have no registers to spare for the prefetch
   // (the prefetched words would be spilled, which waits for the loads).
   constexpr bool PREFETCH = T < 1024;
-  int vin[R];
+  Word vin[R];
   if (PREFETCH && lo < hi) load_block(lo, vin);
 
 
@@ -485,8 +490,13 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     cpx x[R];
 #pragma unroll
     for (int e = 0; e < R; ++e) {
-      x[e].x = (float)(int16_t)vin[e] * w;
-      x[e].y = (float)(vin[e] >> 16) * w;
+      if constexpr (F32) {
+        x[e].x = vin[e].x * w;
+        x[e].y = vin[e].y * w;
+      } else {
+        x[e].x = (float)(int16_t)vin[e] * w;
+        x[e].y = (float)(vin[e] >> 16) * w;
+      }
     }
     st_butterflies<M, R, T, R1, 1, -1>(x, t, nullptr);
 

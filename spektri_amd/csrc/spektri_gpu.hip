@@ -21,6 +21,7 @@
 #include "spektri_gpu.h"
 #include "kernels.h"
 #include "cplx.h"
+#include "convert.h"
 
 using spk::cpx;
 
@@ -112,17 +113,37 @@ std::vector<cpx> pass_table(int r, int p, int sign) {
 }
 
 // DataFormat codes of spektri/src/dsp/data.rs:81-95.
-bool is_reference_format(uint32_t fmt) {
-  switch (fmt) {
-    case 0x04: case 0x08: case 0x09: case 0x1C: case 0x1D: case 0x24:
-    case 0x44: case 0x48: case 0x49: case 0x5C: case 0x5D: case 0x64:
-      return true;
-    default:
-      return false;
-  }
+// Per DataFormat: bytes per raw sample (inputformats.rs:12-27), complex
+// flag (:29-44), default scaling (:47-62), and how the transform kernels
+// read it: the conversion pass that runs first (0: none, read in place) and
+// the bytes per sample of what they read (s16 or f32 real, cf32 complex).
+struct FormatInfo {
+  uint32_t code;
+  bool complex_in;
+  int raw_bps;
+  int conv;
+  int canon_bps;
+  float scaling;
+};
+const FormatInfo* format_info(uint32_t fmt) {
+  static const FormatInfo kFormats[] = {
+      {0x04, false, 1, spk::kConvS8ToS16, 2, 1.0f / 127.0f},        // S8
+      {0x08, false, 2, 0, 2, 1.0f / 32767.0f},                      // S16le
+      {0x09, false, 2, spk::kConvS16beToS16, 2, 1.0f / 32767.0f},   // S16be
+      {0x1C, false, 4, 0, 4, 1.0f},                                 // F32le
+      {0x1D, false, 4, spk::kConvF32beToF32, 4, 1.0f},              // F32be
+      {0x24, false, 1, spk::kConvU8ToF32, 4, 1.0f / 127.0f},        // U8
+      {0x44, true, 2, spk::kConvS8ToF32, 8, 1.0f / 127.0f},         // Cs8
+      {0x48, true, 4, spk::kConvS16leToF32, 8, 1.0f / 32767.0f},    // Cs16le
+      {0x49, true, 4, spk::kConvS16beToF32, 8, 1.0f / 32767.0f},    // Cs16be
+      {0x5C, true, 8, 0, 8, 1.0f},                                  // Cf32le
+      {0x5D, true, 8, spk::kConvF32beToF32, 8, 1.0f},               // Cf32be
+      {0x64, true, 2, spk::kConvU8ToF32, 8, 1.0f / 127.0f},         // Cu8
+  };
+  for (const auto& f : kFormats)
+    if (f.code == fmt) return &f;
+  return nullptr;
 }
-
-int bytes_per_sample(uint32_t fmt) { return fmt == SPK_FMT_S16LE ? 2 : (fmt == SPK_FMT_CF32LE ? 8 : 0); }
 
 }  // namespace
 
@@ -140,7 +161,11 @@ constexpr int kXcdQueues = 8;  // MI355X: 8 XCDs; workgroup w is dispatched to X
 
 struct spk_ctx {
   spk_config cfg{};
-  int N = 0, M = 0, K = 0, H = 0, O = 0, F = 0, A = 0, bps = 0;
+  int N = 0, M = 0, K = 0, H = 0, O = 0, F = 0, A = 0;
+  int bps = 0;               // bytes per sample the transform kernels read (s16, f32 or cf32)
+  int raw_bps = 0;           // bytes per sample of the caller's input format
+  int conv_kind = 0;         // spk::ConvKind run before the transform, 0 = none
+  bool f32_in = false;       // real input read as f32 (the forward kernel's f32 loader)
   bool complex_in = false;
   float wscale = 0.f;
   int max_blocks = 0;
@@ -150,6 +175,7 @@ struct spk_ctx {
   hipStream_t stream = nullptr;
 
   DevBuf hist, staging, spec, wpart, carry, edump, welch_out, chan_out, weights;
+  DevBuf conv;               // converted input (formats with a conversion pass)
   DevBuf fscat;              // fused channel stage: per-lane LDS scatter indices
   int fscat_nyq = 0;
   DevBuf ftw2, ftw3, ftwu;
@@ -326,44 +352,45 @@ spk_status rebuild_channels(spk_ctx* c) {
   return SPK_OK;
 }
 
-template <int M, int R, int R1, int R2, int FL>
+template <int M, int R, int R1, int R2, int FL, bool F32>
 hipError_t launch_fwd(const spk::FwdParams& P, int G, hipStream_t s) {
-  hipLaunchKernelGGL((spk::fwd_real_kernel<M, R, R1, R2, FL>), dim3(G), dim3(M / R), 0, s, P);
+  hipLaunchKernelGGL((spk::fwd_real_kernel<M, R, R1, R2, FL, F32>), dim3(G), dim3(M / R), 0, s, P);
   return hipGetLastError();
 }
 
-template <int M, int R, int R1, int R2, int FL>
+template <int M, int R, int R1, int R2, int FL, bool F32>
 int fwd_wgs_per_cu() {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)spk::fwd_real_kernel<M, R, R1, R2, FL>, M / R,
-                                                   0) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)spk::fwd_real_kernel<M, R, R1, R2, FL, F32>,
+                                                   M / R, 0) != hipSuccess)
     return 0;
   return n;
 }
 
 struct FwdVariant {
   int M, R, FL;
+  bool f32;  // f32 loader (real f32 / u8 input) instead of s16 pairs
   hipError_t (*launch)(const spk::FwdParams&, int, hipStream_t);
   int (*occupancy)();
   int lds_dummy;  // LDS cpx index of the fused stage's dummy scatter slot
 };
-#define SPK_FWD_VARIANT(M, R, R1, R2, FL) \
-  {M, R, FL, &launch_fwd<M, R, R1, R2, FL>, &fwd_wgs_per_cu<M, R, R1, R2, FL>, \
+#define SPK_FWD_VARIANT1(M, R, R1, R2, FL, F32) \
+  {M, R, FL, F32, &launch_fwd<M, R, R1, R2, FL, F32>, &fwd_wgs_per_cu<M, R, R1, R2, FL, F32>, \
    spk::FwdRealCfg<M, R, R1, R2, FL>::LDS_DUMMY}
+#define SPK_FWD_VARIANT(M, R, R1, R2)                                               \
+  SPK_FWD_VARIANT1(M, R, R1, R2, 0, false), SPK_FWD_VARIANT1(M, R, R1, R2, 256, false), \
+      SPK_FWD_VARIANT1(M, R, R1, R2, 0, true), SPK_FWD_VARIANT1(M, R, R1, R2, 256, true)
 const FwdVariant kFwdVariants[] = {
-    SPK_FWD_VARIANT(2048, 16, 8, 16, 0),    SPK_FWD_VARIANT(2048, 16, 8, 16, 256),
-    SPK_FWD_VARIANT(4096, 16, 16, 16, 0),   SPK_FWD_VARIANT(4096, 16, 16, 16, 256),
-    SPK_FWD_VARIANT(8192, 32, 16, 16, 0),   SPK_FWD_VARIANT(8192, 32, 16, 16, 256),
-    SPK_FWD_VARIANT(16384, 32, 16, 32, 0),  SPK_FWD_VARIANT(16384, 32, 16, 32, 256),
-    SPK_FWD_VARIANT(32768, 32, 32, 32, 0),  SPK_FWD_VARIANT(32768, 32, 32, 32, 256),
+    SPK_FWD_VARIANT(2048, 16, 8, 16),  SPK_FWD_VARIANT(4096, 16, 16, 16),  SPK_FWD_VARIANT(8192, 32, 16, 16),
+    SPK_FWD_VARIANT(16384, 32, 16, 32), SPK_FWD_VARIANT(32768, 32, 32, 32),
 };
-const FwdVariant* find_fwd_variant(int M, int FL) {
+const FwdVariant* find_fwd_variant(int M, int FL, bool f32) {
   for (const auto& v : kFwdVariants)
-    if (v.M == M && v.FL == FL) return &v;
+    if (v.M == M && v.FL == FL && v.f32 == f32) return &v;
   return nullptr;
 }
 int fused_lds_dummy(int M) {
-  const FwdVariant* v = find_fwd_variant(M, kFusedL);
+  const FwdVariant* v = find_fwd_variant(M, kFusedL, false);
   return v ? v->lds_dummy : 0;
 }
 template <int L, int R, int R1, int R2, int R3>
@@ -580,11 +607,28 @@ hipError_t dispatch_cplx_rows(int N1, const spk::CplxRowParams& P, int nb, hipSt
 
 spk_status enqueue_channels(spk_ctx* c, int nb, hipStream_t s, const ChanGroup* fg);
 
+// Formats without a loader of their own: convert the call's nb * H raw
+// samples into c->conv and return it as the transform's source (the input
+// itself otherwise).  Launch errors surface at the next hipGetLastError.
+const void* enqueue_convert(spk_ctx* c, const void* raw, int nb, hipStream_t s) {
+  if (!c->conv_kind) return raw;
+  spk::ConvParams P{};
+  P.src = static_cast<const uint32_t*>(raw);
+  P.dst = c->conv.p;
+  P.words = (long long)nb * c->H * c->raw_bps / 4;
+  P.kind = c->conv_kind;
+  const long long want = (P.words + 255) / 256;
+  const int grid = (int)std::min<long long>(want, (long long)c->ncu * 8);
+  hipLaunchKernelGGL(spk::convert_kernel, dim3(grid), dim3(256), 0, s, P);
+  return c->conv.p;
+}
+
 // Complex input: row transforms, column transforms (natural-order spectra),
 // Welch records, channel stage, stream-position advance.
 spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, hipEvent_t* ev) {
   const int N1 = c->N / 64;
   if (ev) HIP_TRY(c, hipEventRecord(ev[0], s));
+  src = enqueue_convert(c, src, nb, s);
   spk::CplxRowParams P{};
   P.src = reinterpret_cast<const cpx*>(src);
   P.hist = c->hist.as<cpx>();
@@ -706,9 +750,11 @@ spk_status flush_records(spk_ctx* c) {
 
 spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hipEvent_t* ev) {
   if (c->complex_in) return enqueue_complex(c, src, nb, s, ev);
+  if (ev) HIP_TRY(c, hipEventRecord(ev[0], s));
+  src = enqueue_convert(c, src, nb, s);
   spk::FwdParams P{};
-  P.src = reinterpret_cast<const int16_t*>(src);
-  P.hist = c->hist.as<int16_t>();
+  P.src = src;
+  P.hist = c->hist.p;
   P.spec = c->spec.as<cpx>();
   P.spec_stride = c->spec_stride;
   // Per-block spectrum rows are needed when kept or read by a channel
@@ -749,9 +795,8 @@ spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hi
   P.ncounters = kXcdQueues;
   if (c->pend_nb > 0) P.fin = fin_params(c, c->pend_nb, c->pend_G, c->pend_parity);
   P.fin_ticket = c->fin_ticket.as<int>();
-  const FwdVariant* fv = find_fwd_variant(c->M, c->fused ? kFusedL : 0);
+  const FwdVariant* fv = find_fwd_variant(c->M, c->fused ? kFusedL : 0, c->f32_in);
   const int G = fwd_grid(c, nb);
-  if (ev) HIP_TRY(c, hipEventRecord(ev[0], s));
   HIP_TRY(c, fv->launch(P, G, s));
   if (ev) HIP_TRY(c, hipEventRecord(ev[1], s));
 
@@ -861,15 +906,14 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   if (!c) return SPK_ERR_DEVICE;
   c->cfg = *cfg;
   const uint32_t N = cfg->fft_size;
-  const bool cplx = cfg->input_format == SPK_FMT_CF32LE;
-  if (N == 0 || (N & (N - 1)) != 0 || cfg->ffts_per_buf == 0 || cfg->spectrum_averages == 0 ||
-      !is_reference_format(cfg->input_format)) {
+  const FormatInfo* fi = format_info(cfg->input_format);
+  if (N == 0 || (N & (N - 1)) != 0 || cfg->ffts_per_buf == 0 || cfg->spectrum_averages == 0 || !fi) {
     delete c;
     return SPK_ERR_INVALID;
   }
-  // formats and sizes the reference accepts but this library does not implement
-  if ((cfg->input_format != SPK_FMT_S16LE && !cplx) ||
-      !(cplx ? (N >= 4096 && N <= 262144) : find_fwd_plan((int)N / 2) != nullptr)) {
+  const bool cplx = fi->complex_in;
+  // sizes the reference accepts but this library does not implement
+  if (!(cplx ? (N >= 4096 && N <= 262144) : find_fwd_plan((int)N / 2) != nullptr)) {
     delete c;
     return SPK_ERR_UNSUPPORTED;
   }
@@ -881,9 +925,12 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   c->H = c->N - c->O;
   c->F = (int)cfg->ffts_per_buf;
   c->A = (int)cfg->spectrum_averages;
-  c->bps = bytes_per_sample(cfg->input_format);
-  // input_format_scaling (inputformats.rs:47-62): 1/32767 for s16, 1 for f32
-  const float scaling = cfg->scaling != 0.f ? cfg->scaling : (cplx ? 1.0f : 1.0f / 32767.0f);
+  c->bps = fi->canon_bps;
+  c->raw_bps = fi->raw_bps;
+  c->conv_kind = fi->conv;
+  c->f32_in = !cplx && fi->canon_bps == 4;
+  // input_format_scaling (inputformats.rs:47-62)
+  const float scaling = cfg->scaling != 0.f ? cfg->scaling : fi->scaling;
   c->wscale = scaling / (float)c->N;  // dsp.rs:70-73
   c->max_blocks = (int)(cfg->max_blocks ? cfg->max_blocks : cfg->ffts_per_buf);
   if (c->max_blocks < c->F) c->max_blocks = c->F;
@@ -922,7 +969,8 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
     const ChanPlan& p = *c->cplan;
     const int N1 = c->N / 64;
     if (c->hist.alloc((size_t)c->O * c->bps) != hipSuccess ||
-        c->staging.alloc((size_t)c->F * c->H * c->bps) != hipSuccess ||
+        c->staging.alloc((size_t)c->F * c->H * c->raw_bps) != hipSuccess ||
+        (c->conv_kind && c->conv.alloc((size_t)c->max_blocks * c->H * c->bps) != hipSuccess) ||
         c->spec.alloc((size_t)c->max_blocks * c->spec_stride * sizeof(cpx)) != hipSuccess ||
         c->cY.alloc((size_t)c->max_blocks * c->N * sizeof(cpx)) != hipSuccess ||
         c->carry.alloc((size_t)2 * c->spec_stride * sizeof(float)) != hipSuccess ||
@@ -953,7 +1001,7 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
     return SPK_OK;
   }
   for (int f = 0; f < 2; ++f) {
-    const FwdVariant* v = find_fwd_variant(c->M, f ? kFusedL : 0);
+    const FwdVariant* v = find_fwd_variant(c->M, f ? kFusedL : 0, c->f32_in);
     const int per_cu = v ? v->occupancy() : 0;
     if (per_cu <= 0) return cleanup(SPK_ERR_DEVICE);
     c->gmax[f] = per_cu * ncu;
@@ -967,7 +1015,8 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   c->wslots = (bpw + c->A - 1) / c->A + 1;
   c->wstride = c->spec_stride;
   if (c->hist.alloc((size_t)c->O * c->bps) != hipSuccess ||
-      c->staging.alloc((size_t)c->F * c->H * c->bps) != hipSuccess ||
+      c->staging.alloc((size_t)c->F * c->H * c->raw_bps) != hipSuccess ||
+      (c->conv_kind && c->conv.alloc((size_t)c->max_blocks * c->H * c->bps) != hipSuccess) ||
       c->spec.alloc((size_t)std::max(c->max_blocks, gcap) * c->spec_stride * sizeof(cpx)) != hipSuccess ||
       c->wpart.alloc((size_t)2 * gcap * c->wslots * c->wstride * sizeof(float)) != hipSuccess ||
       c->edump.alloc((size_t)gcap * (c->M / c->fplan->R / 64) * spk::kEdgeLanes * c->fplan->R * sizeof(cpx)) !=
@@ -1043,7 +1092,7 @@ spk_status spk_set_channel_weights(spk_ctx* c, uint32_t id, const float* weights
 
 spk_status spk_process(spk_ctx* c, const void* raw_new, size_t nbytes) {
   if (!c || !raw_new) return SPK_ERR_INVALID;
-  const size_t want = (size_t)c->F * c->H * c->bps;
+  const size_t want = (size_t)c->F * c->H * c->raw_bps;
   if (nbytes != want) return fail(c, SPK_ERR_INVALID, "buffer must hold exactly F * 3N/4 samples");
   HIP_TRY(c, hipMemcpyAsync(c->staging.p, raw_new, nbytes, hipMemcpyHostToDevice, c->stream));
   spk_status st = run_blocks(c, c->staging.p, c->F);

@@ -108,7 +108,10 @@ class DspState:
             flags |= _lib.FLAG_SPECTRUM_U8 if params.spectrum_format == "U8" else _lib.FLAG_SPECTRUM_U16
         fmt = params.input_format
         if params.complex and fmt == _lib.FMT_S16LE:
-            fmt = _lib.FMT_CF32LE  # the complex format this library supports
+            fmt = _lib.FMT_CF32LE  # complex input left at the real default format
+        if (fmt in _lib.COMPLEX_FORMATS) != bool(params.complex):
+            # main.rs:35 derives the complex flag from the format
+            raise ValueError(f"input format 0x{fmt:02X} does not match complex={params.complex}")
         cfg = _lib.SpkConfig(params.fft_size, fmt, params.scaling,
                              params.ffts_per_buf, params.spectrum_averages, params.max_blocks, flags)
         ctx = ctypes.c_void_p()

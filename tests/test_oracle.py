@@ -198,6 +198,36 @@ def test_convert_formats():
     assert O.bytes_per_input_sample(O.FMT_CF32LE) == 8 and O.bytes_per_input_sample(O.FMT_S16LE) == 2
 
 
+def test_convert_every_format_from_literal_bytes():
+    """inputformats.rs:65-155 on hand-written byte strings: byte order, sign,
+    the u8 offset and the complex interleave (re first) of all 12 formats."""
+    off = float(np.float32(0) - np.float32(127.4))  # u8 0 in f32
+    cases = [
+        (O.FMT_S8, b"\x80\xff\x00\x7f", [-128, -1, 0, 127]),
+        (O.FMT_U8, b"\x00\x80\xff", [off, float(np.float32(128) - np.float32(127.4)),
+                                     float(np.float32(255) - np.float32(127.4))]),
+        (O.FMT_S16LE, b"\x01\x80\xfe\xff", [-32767, -2]),
+        (O.FMT_S16BE, b"\x80\x01\xff\xfe", [-32767, -2]),
+        (O.FMT_F32LE, b"\x00\x00\xc0\x3f", [1.5]),
+        (O.FMT_F32BE, b"\x3f\xc0\x00\x00", [1.5]),
+        (O.FMT_CS8, b"\x01\xfe", [1 - 2j]),
+        (O.FMT_CU8, b"\x00\xff", [off + 1j * float(np.float32(255) - np.float32(127.4))]),
+        (O.FMT_CS16LE, b"\x00\x01\x00\xff", [256 - 256j]),
+        (O.FMT_CS16BE, b"\x01\x00\xff\x00", [256 - 256j]),
+        (O.FMT_CF32LE, b"\x00\x00\x80\x3f\x00\x00\x00\xc0", [1 - 2j]),
+        (O.FMT_CF32BE, b"\x3f\x80\x00\x00\xc0\x00\x00\x00", [1 - 2j]),
+    ]
+    for fmt, raw, want in cases:
+        got = O.convert(raw, fmt)
+        np.testing.assert_array_equal(got, np.asarray(want), err_msg=hex(fmt))
+        assert O.is_input_format_complex(fmt) == (fmt & 0x40 != 0)
+        assert len(raw) == len(want) * O.bytes_per_input_sample(fmt)
+    # inputformats.rs:47-62: 1/i8::MAX, 1/i16::MAX, 1 (f32 division)
+    assert O.input_format_scaling(O.FMT_CU8) == np.float32(1) / np.float32(127)
+    assert O.input_format_scaling(O.FMT_CS16BE) == np.float32(1) / np.float32(32767)
+    assert O.input_format_scaling(O.FMT_CF32BE) == np.float32(1)
+
+
 # --- mutation tests: plausible oracle mistakes are caught by the pins --------
 
 def test_pins_catch_wrong_untangle_sign():
