@@ -101,8 +101,12 @@ const char* spk_last_error(const spk_ctx* ctx);
 
 /* Fcfb::add_filter after freq_to_bins_exact (spektri/src/dsp/fcfb.rs:41-64,137-152).
  * first_bin: number of the channel's lowest bin (BinNumbers.first, may be < 0);
- * bins: IFFT size L, a power of two in [8, 8192] and <= N (the reference
- * accepts any multiple of 4; other sizes return SPK_ERR_UNSUPPORTED);
+ * bins: IFFT size L, a multiple of 4 and <= N (as freq_to_bins makes it,
+ * fcfb.rs:225-248): powers of two up to 8192 run in the radix-2^k kernels,
+ * other multiples of 4 up to 4096 in a mixed-radix kernel; larger ones
+ * return SPK_ERR_UNSUPPORTED.  A channel gives 6 * (L / 8) samples per block
+ * (the integer slice buf[L/8 .. L/8*7] of fcfb.rs:176-177: 3L/4 when 8 | L,
+ * none for L = 4);
  * weights: L floats in j order (weights[j] multiplies bin first + j), or NULL
  * for the reference's raised-cosine weights (fcfb.rs:197-203). */
 spk_status spk_add_channel(spk_ctx* ctx, int64_t first_bin, uint32_t bins, const float* weights,
@@ -130,9 +134,9 @@ spk_status spk_process_device(spk_ctx* ctx, const void* d_raw, uint64_t nsamples
 /* Blocks processed by the last process call. */
 spk_status spk_last_blocks(const spk_ctx* ctx, uint32_t* nblocks);
 
-/* FilterDsp::process output of the last call for channel id: nblocks * 3L/4
- * complex samples as interleaved f32 (re, im), i.e. the record body of
- * fcfb.rs:177-187.  cap and *n count floats. */
+/* FilterDsp::process output of the last call for channel id: nblocks *
+ * 6 (L/8) complex samples as interleaved f32 (re, im), i.e. the record body
+ * of fcfb.rs:177-187.  cap and *n count floats. */
 spk_status spk_channel_output(spk_ctx* ctx, uint32_t id, float* out, size_t cap, size_t* n);
 /* Device pointer and float count of the same data (valid until the next call). */
 spk_status spk_channel_output_device(spk_ctx* ctx, uint32_t id, const float** dptr, size_t* n);
@@ -224,7 +228,9 @@ spk_status spk_fused_channels(spk_ctx* ctx, uint32_t* n);
 spk_status spk_set_weights_device(spk_ctx* ctx, const float* weights);
 /* Copy the last call's channel outputs into a packed device array of cap floats. */
 spk_status spk_channels_output_packed(spk_ctx* ctx, float* dst, size_t cap);
-/* Adjoint of weight-multiply, IFFT and discard (FilterDsp::process,
+/* (The packed calls and the adjoint take power-of-two IFFT sizes only;
+ * other sizes return SPK_ERR_UNSUPPORTED.)
+ * Adjoint of weight-multiply, IFFT and discard (FilterDsp::process,
  * fcfb.rs:154-188) for the last process call: ybar holds dLoss/dRe + i
  * dLoss/dIm of every packed output sample; writes wbar (packed weights
  * layout, summed over blocks) and, when xbar is not NULL, the gradient with

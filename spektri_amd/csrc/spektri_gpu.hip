@@ -22,6 +22,7 @@
 #include "kernels.h"
 #include "cplx.h"
 #include "convert.h"
+#include "chan_generic.h"
 
 using spk::cpx;
 
@@ -64,7 +65,40 @@ struct ChanGroup {  // channels sharing one IFFT size
   DevBuf desc;   // spk::ChanDesc[ids.size()]
   DevBuf tw2, tw3;      // inverse-sign pass tables
   DevBuf atw2, atw3;    // forward-sign pass tables (adjoint)
+  // IFFT sizes without a power-of-two plan (spk::chan_generic_kernel):
+  bool generic = false;
+  std::vector<int> radix;  // stage radices, product L
+  DevBuf gtw;              // w_L^e, e = 0..L-1, inverse sign
 };
+
+// Output samples per block of a channel of IFFT size L: the reference keeps
+// buf[L/8 .. L/8*7] with integer division (fcfb.rs:176-177).
+inline long long chan_out_len(int L) { return 6LL * (L / 8); }
+
+// Stage radices of the generic channel transform: the prime factors of L
+// with pairs of 2 merged into 4.
+std::vector<int> generic_radices(int L) {
+  std::vector<int> r;
+  while (L % 4 == 0) {
+    r.push_back(4);
+    L /= 4;
+  }
+  if (L % 2 == 0) {
+    r.push_back(2);
+    L /= 2;
+  }
+  for (int p = 3; L > 1; p += 2) {
+    while (L % p == 0) {
+      r.push_back(p);
+      L /= p;
+    }
+    if ((long long)p * p > L && L > 1) {
+      r.push_back(L);
+      break;
+    }
+  }
+  return r;
+}
 
 struct FwdPlan {
   int M, R, R1, R2;
@@ -278,7 +312,7 @@ spk_status rebuild_channels(spk_ctx* c) {
     w_off[i] = (int)wall.size();
     wall.insert(wall.end(), ch.w.begin(), ch.w.end());
     c->chan_out_off[i] = off;
-    off += (long long)c->max_blocks * (3LL * ch.L / 4);
+    off += (long long)c->max_blocks * chan_out_len(ch.L);
   }
   HIP_TRY(c, upload(c->weights, wall));
   HIP_TRY(c, c->chan_out.alloc((size_t)off * sizeof(cpx)));
@@ -299,6 +333,17 @@ spk_status rebuild_channels(spk_ctx* c) {
       d.push_back(spk::ChanDesc{c->chan_out_off[id], (int)fm, w_off[id]});
     }
     HIP_TRY(c, upload(g.desc, d));
+    g.generic = p == nullptr;
+    if (g.generic) {
+      g.radix = generic_radices(g.L);
+      std::vector<cpx> tw((size_t)g.L);
+      for (int e = 0; e < g.L; ++e) {
+        const double a = 2.0 * M_PI * (double)e / (double)g.L;
+        tw[(size_t)e] = cpx{(float)std::cos(a), (float)std::sin(a)};
+      }
+      HIP_TRY(c, upload(g.gtw, tw));
+      continue;
+    }
     HIP_TRY(c, upload(g.tw2, pass_table(p->R2, p->R1, +1)));
     HIP_TRY(c, upload(g.tw3, pass_table(p->R3, p->R1 * p->R2, +1)));
     HIP_TRY(c, upload(g.atw2, pass_table(p->R2, p->R1, -1)));
@@ -477,8 +522,37 @@ int batch_R2(int L) {
 std::vector<int> batched_groups(const spk_ctx* c) {
   std::vector<int> ls;
   for (const auto& kv : c->groups)
-    if (!(c->fused && kv.first == kFusedL)) ls.push_back(kv.first);
+    if (!(c->fused && kv.first == kFusedL) && !kv.second.generic) ls.push_back(kv.first);
   return ls;
+}
+
+// One launch of spk::chan_generic_kernel for a group without a
+// power-of-two plan.
+hipError_t launch_generic(const spk_ctx* c, const ChanGroup& g, int nb, hipStream_t s) {
+  spk::GenChanParams Q{};
+  Q.spec = c->spec.as<cpx>();
+  Q.spec_stride = c->spec_stride;
+  Q.N = c->N;
+  Q.K = c->K;
+  Q.real_input = c->complex_in ? 0 : 1;
+  Q.weights = c->weights.as<float>();
+  Q.chans = g.desc.as<spk::ChanDesc>();
+  Q.nch = (int)g.ids.size();
+  Q.nblocks = nb;
+  Q.out = c->chan_out.as<cpx>();
+  Q.tw = g.gtw.as<cpx>();
+  Q.L = g.L;
+  Q.nstages = (int)g.radix.size();
+  for (size_t i = 0; i < g.radix.size(); ++i) Q.radix[i] = g.radix[i];
+  int T = 32;
+  while (T < 256 && 2 * T < g.L) T *= 2;  // about two outputs per thread and stage
+  Q.tile_threads = T;
+  const int tpw = spk::kGenThreads / T;
+  const long long tiles = (long long)Q.nch * nb;
+  const unsigned grid = (unsigned)((tiles + tpw - 1) / tpw);
+  const size_t lds = (size_t)tpw * 2 * g.L * sizeof(cpx);
+  hipLaunchKernelGGL(spk::chan_generic_kernel, dim3(grid), dim3(spk::kGenThreads), lds, s, Q);
+  return hipGetLastError();
 }
 
 // Cut the (block, channel) tiles of nb blocks into per-XCD item queues.
@@ -687,10 +761,13 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
 
 // Channel stage for every channel not fused into the forward kernel.
 spk_status enqueue_channels(spk_ctx* c, int nb, hipStream_t s, const ChanGroup* fg) {
+  // sizes without a power-of-two plan have their own kernel in either form
+  for (auto& kv : c->groups)
+    if (kv.second.generic) HIP_TRY(c, launch_generic(c, kv.second, nb, s));
   if (c->chan_mode == SPK_CHAN_BATCHED) return launch_batched(c, nb, s);
   for (auto& kv : c->groups) {
     ChanGroup& g = kv.second;
-    if (&g == fg) continue;
+    if (&g == fg || g.generic) continue;
     spk::ChanParams Q{};
     Q.spec = c->spec.as<cpx>();
     Q.spec_stride = c->spec_stride;
@@ -1069,8 +1146,9 @@ spk_status spk_buffer_size_of(const spk_ctx* c, spk_buffer_size* out) {
 spk_status spk_add_channel(spk_ctx* c, int64_t first_bin, uint32_t bins, const float* weights, uint32_t* id) {
   if (!c) return SPK_ERR_INVALID;
   if (bins == 0 || bins % 4 != 0) return fail(c, SPK_ERR_INVALID, "IFFT size must be a positive multiple of 4");
-  if ((bins & (bins - 1)) != 0 || !find_chan_plan((int)bins) || (int)bins > c->N)
-    return fail(c, SPK_ERR_UNSUPPORTED, "IFFT size must be a power of two in [8, 8192] and <= N");
+  if ((int)bins > c->N || (!find_chan_plan((int)bins) && (int)bins > spk::kGenMaxL))
+    return fail(c, SPK_ERR_UNSUPPORTED,
+                "IFFT size must be <= N and a power of two up to 8192 or another multiple of 4 up to 4096");
   Channel ch;
   ch.first = first_bin;
   ch.L = (int)bins;
@@ -1123,7 +1201,7 @@ spk_status spk_channel_output_device(spk_ctx* c, uint32_t id, const float** dptr
   if (id >= c->chans.size()) return fail(c, SPK_ERR_INVALID, "no such channel");
   if (!c->have_output || c->chans_dirty) return fail(c, SPK_ERR_STATE, "no output yet");
   *dptr = reinterpret_cast<const float*>(c->chan_out.as<cpx>() + c->chan_out_off[id]);
-  *n = (size_t)c->last_blocks * (3 * c->chans[id].L / 4) * 2;
+  *n = (size_t)c->last_blocks * chan_out_len(c->chans[id].L) * 2;
   return SPK_OK;
 }
 
@@ -1279,9 +1357,17 @@ spk_status spk_set_weights_device(spk_ctx* c, const float* w) {
   return SPK_OK;
 }
 
+static bool has_generic_channels(const spk_ctx* c) {
+  for (const auto& ch : c->chans)
+    if (!find_chan_plan(ch.L)) return true;
+  return false;
+}
+
 spk_status spk_channels_output_packed(spk_ctx* c, float* dst, size_t cap) {
   if (!c || !dst) return SPK_ERR_INVALID;
   if (!c->have_output) return fail(c, SPK_ERR_STATE, "no process call yet");
+  if (has_generic_channels(c))
+    return fail(c, SPK_ERR_UNSUPPORTED, "packed outputs need power-of-two IFFT sizes from 8 to 8192");
   const size_t need = 2 * (size_t)c->last_blocks * (packed_weight_floats(c) / 4 * 3);
   if (cap < need) return fail(c, SPK_ERR_CAPACITY, "destination smaller than the packed outputs");
   for (auto& kv : c->groups) {
@@ -1304,6 +1390,8 @@ spk_status spk_channels_adjoint(spk_ctx* c, const float* ybar, float* wbar, floa
   if (!c || !ybar || !wbar) return SPK_ERR_INVALID;
   if (!c->have_output) return fail(c, SPK_ERR_STATE, "no process call yet");
   if (!c->keep_spec) return fail(c, SPK_ERR_STATE, "the adjoint needs a context created with SPK_FLAG_KEEP_SPECTRA");
+  if (has_generic_channels(c))
+    return fail(c, SPK_ERR_UNSUPPORTED, "the adjoint needs power-of-two IFFT sizes from 8 to 8192");
   const int nb = (int)c->last_blocks;
   for (auto& kv : c->groups) {
     ChanGroup& g = kv.second;
@@ -1383,7 +1471,7 @@ spk_status spk_autotune(spk_ctx* c, uint32_t nblocks, spk_tune_result* out) {
   const float per_size = time_it([&]() -> spk_status {
     for (auto& kv : c->groups) {
       ChanGroup& g = kv.second;
-      if (c->fused && kv.first == kFusedL) continue;
+      if ((c->fused && kv.first == kFusedL) || g.generic) continue;  // same kernel in both forms
       spk::ChanParams Q{};
       Q.spec = c->spec.as<cpx>();
       Q.spec_stride = c->spec_stride;

@@ -233,16 +233,20 @@ class DspState:
         return n.value
 
     def channel_output(self, cid: int) -> np.ndarray:
-        """[blocks, 3L/4] complex64 of the last call (fcfb.rs:177-187 record body)."""
+        """[blocks, 6 (L/8)] complex64 of the last call (fcfb.rs:177-187 record
+        body; 3L/4 samples when 8 divides L, the reference's integer slice
+        buf[L/8 .. L/8*7] otherwise)."""
         lib = _lib.load()
         L = dict(self.channels)[cid].bins
         nb = self.last_blocks()
-        n = nb * (3 * L // 4) * 2
+        per = 6 * (L // 8)
+        n = nb * per * 2
         out = np.empty(n, dtype=np.float32)
         got = ctypes.c_size_t()
         _lib.check(lib.spk_channel_output(self._ctx, cid, out.ctypes.data_as(ctypes.c_void_p), n,
                                           ctypes.byref(got)), self._ctx)
-        return out.view(np.complex64).reshape(nb, 3 * L // 4)
+        assert got.value == n, (got.value, n)
+        return out.view(np.complex64).reshape(nb, per)
 
     def channel_output_device(self, cid: int):
         """(device pointer, float count) of the last call's output."""

@@ -131,9 +131,9 @@ def test_raised_cosine_endpoints():
     np.testing.assert_allclose(w, 0.5 - 0.5 * np.cos(2 * np.pi * np.arange(256) / 256), atol=2e-7)
 
 
-@pytest.mark.parametrize("first", [-20, 3, 40])
-def test_channel_brute_force(first):
-    n, L = 64, 16
+@pytest.mark.parametrize("first,L", [(-20, 16), (3, 16), (40, 16), (-6, 12), (27, 20), (50, 28), (1, 4)])
+def test_channel_brute_force(first, L):
+    n = 64
     x = np.random.default_rng(4).standard_normal(n)
     X = np.fft.rfft(x)
     W = np.random.default_rng(5).random(L)
@@ -143,7 +143,8 @@ def test_channel_brute_force(first):
     B = np.array([W[(i + L // 2) % L] * full[(first + (i + L // 2) % L) % n] for i in range(L)])
     ref = brute_dft(B, sign=+1)[L // 8: (L // 8) * 7]
     np.testing.assert_allclose(y, ref, atol=1e-12)
-    assert len(y) == 3 * L // 4
+    # fcfb.rs:176-177 slices buf[L/8 .. L/8*7] with integer division
+    assert len(y) == {16: 12, 12: 6, 20: 12, 28: 18, 4: 0}[L]
 
 
 def test_channel_tone_at_centre_is_constant_and_stitches():
