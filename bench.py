@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--no-graphs", action="store_true", help="launch eagerly instead of replaying HIP graphs")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after timing, copy this many MiB device-to-device three times")
+    ap.add_argument("--host-ingest", action="store_true",
+                    help="also time the stream fed from host memory, buffer by buffer (PCIe-inclusive)")
     return ap.parse_args()
 
 
@@ -73,6 +75,50 @@ def cpu_baseline(wl, x_host, seconds: float):
     sys.path.insert(0, ROOT)
     from oracle import cpu_port
     return cpu_port.time_workload(wl, x_host, seconds)
+
+
+def host_ingest(wl, x_dev):
+    """Input MS/s when the stream arrives in host memory (never ``value``):
+    the same samples fed buffer by buffer through spk_process (synchronous,
+    pageable upload) and spk_process_async (pinned staging, uploads on a
+    copy stream overlapping the previous buffer's kernels), at the
+    workload's buffer size and at a 17x larger one.  Outputs are not read
+    back."""
+    import numpy as np
+    import torch
+    from spektri_amd.dsp import DspParams, DspState
+
+    raw = x_dev.cpu().numpy()
+    res = {}
+    for F in (wl.ffts_per_buf, wl.ffts_per_buf * 17):
+        per = F * wl.hop
+        nbuf = wl.nsamples // per
+        if nbuf == 0:
+            continue
+        for mode in ("sync", "async", "async_pinned"):
+            st, _ = DspState.init(DspParams(complex=False, fs_in=wl.fs, fft_size=wl.fft_size, ffts_per_buf=F,
+                                            spectrum_averages=wl.averages))
+            for first, L, w in wl.channels:
+                st.add_channel(first, L, w)
+            if mode == "async_pinned":
+                # the reader's copy into the library's pinned slot is timed too
+                def feed(chunk):
+                    slot = st.input_buffer().view(np.int16)
+                    np.copyto(slot, chunk)
+                    st.process_async(slot)
+            else:
+                feed = st.process if mode == "sync" else st.process_async
+            feed(raw[:per])  # warm-up (allocations, first launch)
+            st.synchronize()
+            t0 = time.perf_counter()
+            for b in range(nbuf):
+                feed(raw[b * per:(b + 1) * per])
+            st.synchronize()
+            dt = time.perf_counter() - t0
+            st.close()
+            res[f"{mode}_F{F}_MSps"] = round(nbuf * per / dt / 1e6, 1)
+    res["note"] = "host-resident s16 stream, per-buffer calls, PCIe-inclusive; outputs not read back"
+    return res
 
 
 def main():
@@ -193,6 +239,8 @@ def main():
     if tune is not None:
         out["channel_stage_tune"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tune.items()}
     st.close()
+    if args.host_ingest and not wl.complex:
+        out["host_ingest"] = host_ingest(wl, x)
     if args.copy_probe_mib > 0:
         n = args.copy_probe_mib * (1 << 20) // 4
         a = torch.ones(n, dtype=torch.float32, device=dev)

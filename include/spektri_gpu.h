@@ -121,6 +121,23 @@ spk_status spk_set_channel_weights(spk_ctx* ctx, uint32_t id, const float* weigh
  * main.rs:165).  Synchronous. */
 spk_status spk_process(spk_ctx* ctx, const void* raw_new, size_t nbytes);
 
+/* Streaming form of spk_process for a reader loop (main.rs:171-193): copies
+ * the buffer into one of two pinned host slots (the caller may reuse its
+ * buffer on return), uploads it on a copy stream and enqueues the launch
+ * sequence on the context's stream after the upload, then returns without
+ * waiting.  The upload of the next call overlaps this call's kernels, and
+ * the caller's next read overlaps both.  Outputs are read with the usual
+ * readers (which wait for the stream) and belong to the last call: read
+ * them before the next call.  No counterpart in the reference, whose
+ * process_* calls are synchronous. */
+spk_status spk_process_async(spk_ctx* ctx, const void* raw_new, size_t nbytes);
+/* The pinned host slot the next spk_process_async call uploads from (waits
+ * until the slot's previous upload has left it): a reader can read_exact
+ * straight into it (main.rs:176) and pass it back to spk_process_async,
+ * which then skips its host copy.  *nbytes = F * 3N/4 raw samples in bytes.
+ * Valid until the context is destroyed.  No counterpart in the reference. */
+spk_status spk_input_buffer(spk_ctx* ctx, void** host_ptr, size_t* nbytes);
+
 /* Same path on a device-resident stream of nsamples raw samples (4-byte
  * aligned): processes floor(nsamples / (F * 3N/4)) * F blocks, like the
  * reference's main loop over that many bytes (main.rs:171-179), in one

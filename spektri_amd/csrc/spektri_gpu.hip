@@ -15,6 +15,7 @@
 #include <map>
 #include <new>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -210,6 +211,15 @@ struct spk_ctx {
 
   DevBuf hist, staging, spec, wpart, carry, edump, welch_out, chan_out, weights;
   DevBuf conv;               // converted input (formats with a conversion pass)
+  // spk_process_async: two slots of pinned host staging and device staging,
+  // a copy stream for the uploads, and per slot the events "upload done"
+  // (copy stream) and "kernels reading the slot done" (context stream).
+  void* apin[2] = {nullptr, nullptr};
+  DevBuf adev[2];
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t a_up[2] = {nullptr, nullptr}, a_used[2] = {nullptr, nullptr};
+  bool a_busy[2] = {false, false};
+  int aslot = 0;
   DevBuf fscat;              // fused channel stage: per-lane LDS scatter indices
   int fscat_nyq = 0;
   DevBuf ftw2, ftw3, ftwu;
@@ -1128,6 +1138,15 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
 spk_status spk_destroy(spk_ctx* c) {
   if (!c) return SPK_ERR_INVALID;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->copy_stream) {
+    (void)hipStreamSynchronize(c->copy_stream);
+    (void)hipStreamDestroy(c->copy_stream);
+  }
+  for (int i = 0; i < 2; ++i) {
+    if (c->a_up[i]) (void)hipEventDestroy(c->a_up[i]);
+    if (c->a_used[i]) (void)hipEventDestroy(c->a_used[i]);
+    if (c->apin[i]) (void)hipHostFree(c->apin[i]);
+  }
   drop_graphs(c);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -1177,6 +1196,89 @@ spk_status spk_process(spk_ctx* c, const void* raw_new, size_t nbytes) {
   if (st == SPK_OK) st = flush_records(c);
   if (st != SPK_OK) return st;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return SPK_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Host copy into a pinned slot; large buffers are split over a few threads
+// (one thread copies well under the PCIe rate).
+void copy_to_pinned(void* dst, const void* src, size_t n) {
+  const size_t kPerThread = (size_t)4 << 20;
+  unsigned nt = std::min<unsigned>(8u, std::max(1u, std::thread::hardware_concurrency()));
+  nt = (unsigned)std::min<size_t>(nt, n / kPerThread);
+  if (nt <= 1) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  const size_t chunk = (n / nt + 63) / 64 * 64;
+  std::vector<std::thread> th;
+  for (unsigned i = 1; i < nt; ++i) {
+    const size_t lo = std::min(n, i * chunk), hi = std::min(n, lo + chunk);
+    if (lo < hi)
+      th.emplace_back([=] { std::memcpy(static_cast<char*>(dst) + lo, static_cast<const char*>(src) + lo, hi - lo); });
+  }
+  std::memcpy(dst, src, std::min(n, chunk));
+  for (auto& t : th) t.join();
+}
+
+spk_status async_setup(spk_ctx* c) {
+  if (c->copy_stream) return SPK_OK;
+  const size_t want = (size_t)c->F * c->H * c->raw_bps;
+  HIP_TRY(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+  for (int i = 0; i < 2; ++i) {
+    HIP_TRY(c, hipHostMalloc(&c->apin[i], want, hipHostMallocDefault));
+    HIP_TRY(c, c->adev[i].alloc(want));
+    HIP_TRY(c, hipEventCreateWithFlags(&c->a_up[i], hipEventDisableTiming));
+    HIP_TRY(c, hipEventCreateWithFlags(&c->a_used[i], hipEventDisableTiming));
+  }
+  return SPK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+spk_status spk_input_buffer(spk_ctx* c, void** host_ptr, size_t* nbytes) {
+  if (!c || !host_ptr) return SPK_ERR_INVALID;
+  {
+    spk_status st = async_setup(c);
+    if (st != SPK_OK) return st;
+  }
+  const int s = c->aslot;
+  if (c->a_busy[s]) HIP_TRY(c, hipEventSynchronize(c->a_up[s]));  // its last upload has left it
+  *host_ptr = c->apin[s];
+  if (nbytes) *nbytes = (size_t)c->F * c->H * c->raw_bps;
+  return SPK_OK;
+}
+
+spk_status spk_process_async(spk_ctx* c, const void* raw_new, size_t nbytes) {
+  if (!c || !raw_new) return SPK_ERR_INVALID;
+  const size_t want = (size_t)c->F * c->H * c->raw_bps;
+  if (nbytes != want) return fail(c, SPK_ERR_INVALID, "buffer must hold exactly F * 3N/4 samples");
+  {
+    spk_status st = async_setup(c);
+    if (st != SPK_OK) return st;
+  }
+  const int s = c->aslot;
+  // The slot's last upload must have left its pinned memory, and the
+  // kernels of the call before last must be done reading its device copy.
+  if (c->a_busy[s]) {
+    HIP_TRY(c, hipEventSynchronize(c->a_up[s]));
+    HIP_TRY(c, hipStreamWaitEvent(c->copy_stream, c->a_used[s], 0));
+  }
+  // the caller may have filled the slot itself (spk_input_buffer)
+  if (raw_new != c->apin[s]) copy_to_pinned(c->apin[s], raw_new, nbytes);
+  HIP_TRY(c, hipMemcpyAsync(c->adev[s].p, c->apin[s], nbytes, hipMemcpyHostToDevice, c->copy_stream));
+  HIP_TRY(c, hipEventRecord(c->a_up[s], c->copy_stream));
+  HIP_TRY(c, hipStreamWaitEvent(c->stream, c->a_up[s], 0));
+  c->a_busy[s] = true;
+  spk_status st = run_blocks(c, c->adev[s].p, c->F);
+  if (st != SPK_OK) return st;
+  HIP_TRY(c, hipEventRecord(c->a_used[s], c->stream));
+  c->aslot ^= 1;
   return SPK_OK;
 }
 

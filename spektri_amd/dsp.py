@@ -159,6 +159,24 @@ class DspState:
         lib = _lib.load()
         _lib.check(lib.spk_process(self._ctx, buf.ctypes.data_as(ctypes.c_void_p), buf.nbytes), self._ctx)
 
+    def process_async(self, raw) -> None:
+        """One host buffer, staged through pinned memory and uploaded on a
+        copy stream; returns before the GPU is done (read outputs before the
+        next call)."""
+        buf = np.ascontiguousarray(raw).view(np.uint8)
+        lib = _lib.load()
+        _lib.check(lib.spk_process_async(self._ctx, buf.ctypes.data_as(ctypes.c_void_p), buf.nbytes), self._ctx)
+
+    def input_buffer(self) -> np.ndarray:
+        """The pinned slot the next ``process_async`` uploads from, as a uint8
+        array; fill it and pass it to ``process_async`` to skip the host copy."""
+        p, n = ctypes.c_void_p(), ctypes.c_size_t()
+        _lib.check(_lib.load().spk_input_buffer(self._ctx, ctypes.byref(p), ctypes.byref(n)), self._ctx)
+        views = self.__dict__.setdefault("_slot_views", {})
+        if p.value not in views:  # two slots, each viewed once
+            views[p.value] = np.ctypeslib.as_array((ctypes.c_uint8 * n.value).from_address(p.value))
+        return views[p.value]
+
     def process_device(self, dptr: int, nsamples: int) -> None:
         """A device-resident stream (e.g. ``tensor.data_ptr()``), asynchronous."""
         lib = _lib.load()
@@ -288,6 +306,7 @@ class DspState:
         return p.value, stride.value
 
     def close(self):
+        self.__dict__.pop("_slot_views", None)
         if self._ctx:
             _lib.load().spk_destroy(self._ctx)
             self._ctx = None
