@@ -58,13 +58,13 @@ __global__ __launch_bounds__(kChanThreads) void cplx_row_kernel(CplxRowParams P)
     v[e] = cscale(a, P.wscale);
   }
   st_butterflies<N1, R, T, R1, 1, -1>(v, t, nullptr);
-  lds_exchange<R, false>(
+  tile_exchange<T>(  // wave-local when the row's T threads lie in one wave
       v, lds, [&](int e) { return st_out_addr<N1, R, T, R1, 1>(t, e); },
       [&](int e) { return st_in_addr<N1, R, T, R2>(t, e); });
   st_butterflies<N1, R, T, R2, R1, -1>(v, t, P.tw2);
   constexpr int RL = (R3 > 1) ? R3 : R2;
   if constexpr (R3 > 1) {
-    lds_exchange<R, false>(
+    tile_exchange<T>(
         v, lds, [&](int e) { return st_out_addr<N1, R, T, R2, R1>(t, e); },
         [&](int e) { return st_in_addr<N1, R, T, R3>(t, e); });
     st_butterflies<N1, R, T, R3, R1 * R2, -1>(v, t, P.tw3);

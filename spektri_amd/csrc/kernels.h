@@ -715,7 +715,9 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
       for (int e = 0; e < RF; ++e)
         gx[e] = cscale(reinterpret_cast<const cpx*>(tl)[st_in_addr<FL, RF, TF, FC::RF1>(ftt, e)], wv[e]);
       st_butterflies<FL, RF, TF, FC::RF1, 1, +1>(gx, ftt, nullptr);
-      lds_exchange<RF, false>(
+      // the tile's TF = 16 threads lie in one wave and only they touch its
+      // region until the next block's first exchange barrier
+      tile_exchange<TF>(
           gx, tl, [&](int e) { return st_out_addr<FL, RF, TF, FC::RF1, 1>(ftt, e); },
           [&](int e) { return st_in_addr<FL, RF, TF, FC::RF2>(ftt, e); });
       int zero2 = 0;
@@ -823,13 +825,14 @@ __device__ __forceinline__ void chan_tile(const cpx* __restrict__ spec, long lon
     x[e] = cscale(v, W[jj]);
   }
   st_butterflies<L, R, T, R1, 1, +1>(x, t, nullptr);
-  lds_exchange<R, false>(
+  // (a tile of at most 64 threads lies in one wave: wave-local exchanges)
+  tile_exchange<T>(
       x, lds, [&](int e) { return st_out_addr<L, R, T, R1, 1>(t, e); },
       [&](int e) { return st_in_addr<L, R, T, R2>(t, e); });
   st_butterflies<L, R, T, R2, R1, +1>(x, t, tw2);
   constexpr int RL = (R3 > 1) ? R3 : R2;
   if constexpr (R3 > 1) {
-    lds_exchange<R, false>(
+    tile_exchange<T>(
         x, lds, [&](int e) { return st_out_addr<L, R, T, R2, R1>(t, e); },
         [&](int e) { return st_in_addr<L, R, T, R3>(t, e); });
     st_butterflies<L, R, T, R3, R1 * R2, +1>(x, t, tw3);

@@ -123,4 +123,35 @@ __device__ __forceinline__ void lds_exchange(cpx (&x)[R], float* lds, WA waddr, 
   }
 }
 
+// Wave-level ordering point for LDS traffic that stays inside one wave: a
+// wave's LDS instructions are executed in order, so only the compiler has
+// to be kept from moving accesses across it (no s_barrier is issued).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// lds_exchange for a transform whose T threads lie in one wave (T divides
+// 64) and whose LDS region no other wave touches between workgroup
+// barriers: the exchange then needs no workgroup barrier.
+template <int R, typename WA, typename RA>
+__device__ __forceinline__ void lds_exchange_wave(cpx (&x)[R], float* lds, WA waddr, RA raddr) {
+  cpx* l = reinterpret_cast<cpx*>(lds);
+  wave_sync();
+#pragma unroll
+  for (int e = 0; e < R; ++e) l[waddr(e)] = x[e];
+  wave_sync();
+#pragma unroll
+  for (int e = 0; e < R; ++e) x[e] = l[raddr(e)];
+}
+
+// Workgroup-wide exchange, or the wave-local form when every tile of T
+// threads lies inside one wave.
+template <int T, int R, typename WA, typename RA>
+__device__ __forceinline__ void tile_exchange(cpx (&x)[R], float* lds, WA waddr, RA raddr) {
+  if constexpr (T <= 64) lds_exchange_wave<R>(x, lds, waddr, raddr);
+  else lds_exchange<R, false>(x, lds, waddr, raddr);
+}
+
 }  // namespace spk
