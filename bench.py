@@ -41,8 +41,11 @@ HBM_PEAK_GBPS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    # The GPU's clock ramps up over the first ~100 back-to-back launches of
+    # this workload (and again after an idle second); a continuous stream
+    # runs at the ramped clock, which the warm-up reaches.
+    ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--workload", default="rx888_64k")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU work for the baseline sample (0 disables it)")

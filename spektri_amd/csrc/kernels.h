@@ -278,15 +278,6 @@ __host__ __device__ inline int welch_row_bin(int i, int M, int R) {
   return (pi == 0 ? P3 / 2 : P3 - pi) + P3 * (R - 1 - q);
 }
 
-__device__ __forceinline__ float swap_hi_lo(float v, float& other) {
-  // v_permlane32_swap: returns (A, B) with A = {v.lo | v.lo}, B = {v.hi | v.hi}
-  // when both operands are v; A's upper half and B's lower half are the
-  // partner lane's value.
-  auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
-  other = __uint_as_float(r[1]);
-  return __uint_as_float(r[0]);
-}
-
 // DPP wave shifts (gfx9 wave_shr:1 / wave_shl:1): the value of lane l-1
 // (resp. l+1); lane 0 (resp. 63) keeps its own value.
 __device__ __forceinline__ float wave_shr1(float v) {
@@ -581,32 +572,51 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     // W^k = W^kt (table) * W^(P3 q) (constant per slot).
     const cpx wb = twu[kt];
     const cpx wbs = {0.5f * sg * wb.x, 0.5f * sg * wb.y};
+    const cpx wh = {0.5f * wb.x, 0.5f * wb.y};
     const bool lane0 = lane == 0, lane32 = lane == 32;
+    // byte address of the mirror lane for ds_bpermute
+    const int mirror_addr = (lane ^ 32) << 2;
 
     // Untangle; Welch terms into the partial row (not for edge lanes); edge
     // dump; channel scatter into the tile buffers.
     static_for<0, R>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       if constexpr (q % 8 == 0 && q + 8 < R) load_group(std::integral_constant<int, q / 8 + 1>{});
-      cpx A, B;
-      A.x = swap_hi_lo(x[q].x, B.x);
-      A.y = swap_hi_lo(x[q].y, B.y);
+      // V: this lane's value, Pm: its mirror's, fetched through the LDS
+      // crossbar (ds_bpermute issues no VALU work).  A = Z[k] is V on lower
+      // lanes and Pm on upper ones, B = Z[M-k] the other, so with
+      // D = V - Pm = s (A - B) the untangle below needs no per-slot sign.
+      const cpx V = x[q];
+      cpx Pm;
+      Pm.x = __int_as_float(__builtin_amdgcn_ds_bpermute(mirror_addr, __float_as_int(V.x)));
+      Pm.y = __int_as_float(__builtin_amdgcn_ds_bpermute(mirror_addr, __float_as_int(V.y)));
       if (wave == 0) {
         // lane 0 (k = 0): mirror of Z[P3 q] is its own slot (R - q) mod R;
         // lane 32 (k = P3/2): mirror of its slot q is its own slot R-1-q.
         const cpx s0 = stash[(R - q) & (R - 1)];
         const cpx s32 = stash[R + R - 1 - q];
-        B = lane0 ? s0 : B;
-        A = lane32 ? s32 : A;
+        Pm = lane0 ? s0 : (lane32 ? s32 : Pm);
       }
-      const float S1 = A.x + B.x, S2 = A.y + B.y, D1 = A.x - B.x, D2 = A.y - B.y;
+      const float S1 = V.x + Pm.x, S2 = V.y + Pm.y, D1 = V.x - Pm.x, D2 = V.y - Pm.y;
       constexpr float cx = float(ct_cos2pi(-q, 2 * R));
       constexpr float cy = float(ct_sin2pi(-q, 2 * R));
-      const float ux = wbs.x * cx - wbs.y * cy;
-      const float uy = wbs.x * cy + wbs.y * cx;
+      // u = (s/2) W^k (s = +1 lower, -1 upper lanes); ua = Re u, ub = s Im u
+      // (slot constants of 0 and 1 are folded: x * 0 is not 0 in IEEE
+      // arithmetic, so the compiler keeps such products)
+      float ua, ub;
+      if constexpr (q == 0) {
+        ua = wbs.x;
+        ub = wh.y;
+      } else if constexpr (2 * q == R) {
+        ua = wbs.y;  // cx = 0, cy = -1
+        ub = -wh.x;
+      } else {
+        ua = wbs.x * cx - wbs.y * cy;
+        ub = wh.x * cy + wh.y * cx;
+      }
       cpx X;
-      X.x = 0.5f * S1 + (ux * S2 + uy * D1);
-      X.y = sg * (0.5f * D2 + (uy * S2 - ux * D1));
+      X.x = 0.5f * S1 + (ua * S2 + ub * D1);
+      X.y = 0.5f * D2 + (ub * S2 - ua * D1);
       if (P.keep_spec) at32(spec, (unsigned)(kb + q * kstep)) = X;
       if (es >= 0)
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, X), edr, evoff + q * (int)sizeof(cpx), 0, 0);
