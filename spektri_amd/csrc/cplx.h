@@ -151,41 +151,57 @@ struct WelchDirectParams {
   int qfmt;
 };
 
-__device__ __forceinline__ cpx bin_at(const cpx* X, int N, int K, int real_input, int i) {
-  int m = i % N;
-  if (m < 0) m += N;
-  if (real_input && m >= K) {
-    const cpx v = X[N - m];
-    return {v.x, -v.y};
-  }
-  return X[m];
-}
-
+// Complex input (K = N, a multiple of 1024): one thread per 4 consecutive
+// bins k0..k0+3, read as two 16-byte loads per block; the outer neighbours
+// X[k0 - 1] and X[k0 + 4] come from the adjacent lanes by DPP, and lanes 0
+// and 63 of each wave load theirs (modulo N: get_bin's wrap, fftutil.rs:8-22).
+// Grid: x = K / 1024 workgroups of 256 threads, y = Welch groups.
 __global__ void welch_direct_kernel(WelchDirectParams F) {
   const int gi = (int)blockIdx.y;
-  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int k0 = 4 * tid;
   const long long pos = *F.pos;
   if (gi == 0) {
-    for (int i = k; i < F.hist_words; i += (int)(gridDim.x * blockDim.x)) F.hist_dst[i] = F.hist_src[i];
-    if (k < F.ncounters) F.chan_counters[k] = 0;
-    if (k == 0) *F.pos_next = pos + F.nblocks;
+    for (int i = tid; i < F.hist_words; i += (int)(gridDim.x * blockDim.x)) F.hist_dst[i] = F.hist_src[i];
+    if (tid < F.ncounters) F.chan_counters[tid] = 0;
+    if (tid == 0) *F.pos_next = pos + F.nblocks;
   }
   const int blk0_mod = (int)(pos % F.A);
-  if (k >= F.K || gi > (blk0_mod + F.nblocks - 1) / F.A) return;
+  // whole workgroups leave together (the DPP below needs every lane)
+  if (gi > (blk0_mod + F.nblocks - 1) / F.A) return;
   const int g_lo = gi * F.A - blk0_mod, g_hi = g_lo + F.A;
   const int b_lo = g_lo > 0 ? g_lo : 0;
   const int b_hi = g_hi < F.nblocks ? g_hi : F.nblocks;
-  float sum = (gi == 0 && blk0_mod != 0) ? F.carry_in[k] : 0.f;
+  const int lane = (int)(threadIdx.x & 63);
+  const int kl = (k0 - 1) & (F.N - 1), kr = (k0 + 4) & (F.N - 1);
+  float4 s = {0.f, 0.f, 0.f, 0.f};
+  if (gi == 0 && blk0_mod != 0) s = *reinterpret_cast<const float4*>(F.carry_in + k0);
   for (int b = b_lo; b < b_hi; ++b) {
     const cpx* X = F.spec + (long long)b * F.spec_stride;
-    sum += welch_term(bin_at(X, F.N, F.K, F.real_input, k), bin_at(X, F.N, F.K, F.real_input, k - 1),
-                      bin_at(X, F.N, F.K, F.real_input, k + 1));
+    const float4 u = *reinterpret_cast<const float4*>(X + k0);      // X[k0], X[k0+1]
+    const float4 v = *reinterpret_cast<const float4*>(X + k0 + 2);  // X[k0+2], X[k0+3]
+    cpx L = {wave_shr1(v.z), wave_shr1(v.w)};  // lane - 1's X[k0 - 1]
+    cpx Rn = {wave_shl1(u.x), wave_shl1(u.y)}; // lane + 1's X[k0 + 4]
+    if (lane == 0) L = X[kl];
+    if (lane == 63) Rn = X[kr];
+    const cpx x0 = {u.x, u.y}, x1 = {u.z, u.w}, x2 = {v.x, v.y}, x3 = {v.z, v.w};
+    s.x += welch_term(x0, L, x1);
+    s.y += welch_term(x1, x0, x2);
+    s.z += welch_term(x2, x1, x3);
+    s.w += welch_term(x3, x2, Rn);
   }
   if (g_hi <= F.nblocks) {
-    F.out[(long long)gi * F.K + k] = sum / (float)F.A;
-    if (F.qfmt) quantize_bin(F.qout, F.qfmt, (long long)gi * F.K + k, sum, F.A);
+    const long long o = (long long)gi * F.K + k0;
+    const float inv = (float)F.A;
+    *reinterpret_cast<float4*>(F.out + o) = make_float4(s.x / inv, s.y / inv, s.z / inv, s.w / inv);
+    if (F.qfmt) {
+      quantize_bin(F.qout, F.qfmt, o, s.x, F.A);
+      quantize_bin(F.qout, F.qfmt, o + 1, s.y, F.A);
+      quantize_bin(F.qout, F.qfmt, o + 2, s.z, F.A);
+      quantize_bin(F.qout, F.qfmt, o + 3, s.w, F.A);
+    }
   } else {
-    F.carry_out[k] = sum;
+    *reinterpret_cast<float4*>(F.carry_out + k0) = s;
   }
 }
 

@@ -762,7 +762,8 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
     W.qout = c->qout.as<unsigned char>();
     W.qfmt = c->qfmt;
     const int gmax = (c->A - 1 + nb - 1) / c->A + 1;
-    hipLaunchKernelGGL(spk::welch_direct_kernel, dim3((c->K + 255) / 256, gmax), dim3(256), 0, s, W);
+    // four bins per thread; K = N is a multiple of 1024
+    hipLaunchKernelGGL(spk::welch_direct_kernel, dim3(c->K / 1024, gmax), dim3(256), 0, s, W);
     HIP_TRY(c, hipGetLastError());
   }
   if (ev) HIP_TRY(c, hipEventRecord(ev[3], s));
