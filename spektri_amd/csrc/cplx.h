@@ -90,17 +90,21 @@ struct CplxColParams {
   const cpx* tw64;  // W64^m, m < 64 (forward sign)
 };
 
-// 64 columns k1 per 256-thread workgroup, 4 threads per column: thread
-// (c, s) takes rows n2 = s + 4 j, j < 16, runs the 16-point DFT over j,
-// twiddles by W64^{s q}, and after an LDS exchange each thread finishes 4
-// of the 16 q values with 4-point DFTs over s: X[k1 + N1 (q + 16 p)].
-// Grid: nblocks * N1 / 64.
+// 64 columns per 256-thread workgroup, 4 threads per column: column
+// gc = 64 x + c is column k1 = gc mod N1 of block g = gc / N1 (so for
+// N1 < 64 a workgroup spans several blocks).  Thread (c, s) takes rows
+// n2 = s + 4 j, j < 16, runs the 16-point DFT over j, twiddles by
+// W64^{s q}, and after an LDS exchange each thread finishes 4 of the 16 q
+// values with 4-point DFTs over s: X[k1 + N1 (q + 16 p)].
+// Grid: ceil(nblocks * N1 / 64).
 __global__ __launch_bounds__(kChanThreads) void cplx_col_kernel(CplxColParams P) {
   __shared__ cpx lds[64 * 16 * 4 + 64];  // [c][q][s], padded per column
-  const int tiles = P.N1 / 64;
-  const int g = (int)blockIdx.x / tiles, k1_0 = ((int)blockIdx.x % tiles) * 64;
   const int c = threadIdx.x & 63, s = threadIdx.x >> 6;
-  const cpx* y = P.Y + (long long)g * 64 * P.N1 + k1_0 + c;
+  const long long gc = (long long)blockIdx.x * 64 + c;
+  const bool active = gc < (long long)P.nblocks * P.N1;  // columns past the end compute column 0, store nothing
+  const long long gcc = active ? gc : 0;
+  const int g = (int)(gcc / P.N1), k1 = (int)(gcc % P.N1);
+  const cpx* y = P.Y + (long long)g * 64 * P.N1 + k1;
   cpx z[16];
 #pragma unroll
   for (int j = 0; j < 16; ++j) z[j] = y[(long long)(s + 4 * j) * P.N1];
@@ -111,7 +115,7 @@ __global__ __launch_bounds__(kChanThreads) void cplx_col_kernel(CplxColParams P)
 #pragma unroll
   for (int q = 0; q < 16; ++q) lds[la(c, q, s)] = z[q];
   __syncthreads();
-  cpx* out = P.spec + (long long)g * P.spec_stride + k1_0 + c;
+  cpx* out = P.spec + (long long)g * P.spec_stride + k1;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int q = 4 * s + i;
@@ -119,8 +123,10 @@ __global__ __launch_bounds__(kChanThreads) void cplx_col_kernel(CplxColParams P)
 #pragma unroll
     for (int ss = 0; ss < 4; ++ss) b[ss] = lds[la(c, q, ss)];
     dft<4, -1>(b);
+    if (active) {
 #pragma unroll
-    for (int p = 0; p < 4; ++p) out[(long long)(q + 16 * p) * P.N1] = b[p];
+      for (int p = 0; p < 4; ++p) out[(long long)(q + 16 * p) * P.N1] = b[p];
+    }
   }
 }
 

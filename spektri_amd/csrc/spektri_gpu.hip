@@ -104,8 +104,9 @@ std::vector<int> generic_radices(int L) {
 struct FwdPlan {
   int M, R, R1, R2;
 };
-const FwdPlan kFwdPlans[] = {
-    {2048, 16, 8, 16}, {4096, 16, 16, 16}, {8192, 32, 16, 16}, {16384, 32, 16, 32}, {32768, 32, 32, 32}};
+const FwdPlan kFwdPlans[] = {{512, 8, 8, 8},     {1024, 16, 8, 8},     {2048, 16, 8, 16},
+                             {4096, 16, 16, 16}, {8192, 32, 16, 16}, {16384, 32, 16, 32},
+                             {32768, 32, 32, 32}};
 
 struct ChanPlan {
   int L, R, R1, R2, R3;
@@ -436,8 +437,9 @@ struct FwdVariant {
   SPK_FWD_VARIANT1(M, R, R1, R2, 0, false), SPK_FWD_VARIANT1(M, R, R1, R2, 256, false), \
       SPK_FWD_VARIANT1(M, R, R1, R2, 0, true), SPK_FWD_VARIANT1(M, R, R1, R2, 256, true)
 const FwdVariant kFwdVariants[] = {
-    SPK_FWD_VARIANT(2048, 16, 8, 16),  SPK_FWD_VARIANT(4096, 16, 16, 16),  SPK_FWD_VARIANT(8192, 32, 16, 16),
-    SPK_FWD_VARIANT(16384, 32, 16, 32), SPK_FWD_VARIANT(32768, 32, 32, 32),
+    SPK_FWD_VARIANT(512, 8, 8, 8),      SPK_FWD_VARIANT(1024, 16, 8, 8),    SPK_FWD_VARIANT(2048, 16, 8, 16),
+    SPK_FWD_VARIANT(4096, 16, 16, 16),  SPK_FWD_VARIANT(8192, 32, 16, 16),  SPK_FWD_VARIANT(16384, 32, 16, 32),
+    SPK_FWD_VARIANT(32768, 32, 32, 32),
 };
 const FwdVariant* find_fwd_variant(int M, int FL, bool f32) {
   for (const auto& v : kFwdVariants)
@@ -677,6 +679,8 @@ void launch_cplx_rows(const spk::CplxRowParams& P, int nb, hipStream_t s) {
 
 hipError_t dispatch_cplx_rows(int N1, const spk::CplxRowParams& P, int nb, hipStream_t s) {
   switch (N1) {
+    case 16: launch_cplx_rows<16, 4, 4, 4, 1>(P, nb, s); break;
+    case 32: launch_cplx_rows<32, 8, 8, 4, 1>(P, nb, s); break;
     case 64: launch_cplx_rows<64, 8, 8, 8, 1>(P, nb, s); break;
     case 128: launch_cplx_rows<128, 16, 16, 8, 1>(P, nb, s); break;
     case 256: launch_cplx_rows<256, 16, 16, 16, 1>(P, nb, s); break;
@@ -733,7 +737,9 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
   Q.spec = c->spec.as<cpx>();
   Q.spec_stride = c->spec_stride;
   Q.tw64 = c->ctw64.as<cpx>();
-  hipLaunchKernelGGL(spk::cplx_col_kernel, dim3(nb * (N1 / 64)), dim3(spk::kChanThreads), 0, s, Q);
+  // 64 (block, column) pairs per workgroup
+  hipLaunchKernelGGL(spk::cplx_col_kernel, dim3((unsigned)(((long long)nb * N1 + 63) / 64)), dim3(spk::kChanThreads),
+                     0, s, Q);
   HIP_TRY(c, hipGetLastError());
   if (ev) HIP_TRY(c, hipEventRecord(ev[1], s));
   spk_status st = enqueue_channels(c, nb, s, nullptr);
@@ -995,13 +1001,14 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   c->cfg = *cfg;
   const uint32_t N = cfg->fft_size;
   const FormatInfo* fi = format_info(cfg->input_format);
-  if (N == 0 || (N & (N - 1)) != 0 || cfg->ffts_per_buf == 0 || cfg->spectrum_averages == 0 || !fi) {
+  if (N < 4 || cfg->ffts_per_buf == 0 || cfg->spectrum_averages == 0 || !fi) {
     delete c;
     return SPK_ERR_INVALID;
   }
   const bool cplx = fi->complex_in;
-  // sizes the reference accepts but this library does not implement
-  if (!(cplx ? (N >= 4096 && N <= 262144) : find_fwd_plan((int)N / 2) != nullptr)) {
+  // sizes the reference accepts (rustfft plans any N) but this library does
+  // not implement: N not a power of two, or outside the planned range
+  if ((N & (N - 1)) != 0 || !(cplx ? (N >= 1024 && N <= 262144) : find_fwd_plan((int)N / 2) != nullptr)) {
     delete c;
     return SPK_ERR_UNSUPPORTED;
   }

@@ -33,8 +33,10 @@ def test_null_and_invalid_arguments_return_codes():
     assert lib.spk_destroy(None) == 1
     assert lib.spk_last_error(None) == b"null context"
     out = ctypes.c_void_p()
-    cfg = _lib.SpkConfig(1000, _lib.FMT_S16LE, 0.0, 8, 16, 0)  # N not a power of two
+    cfg = _lib.SpkConfig(0, _lib.FMT_S16LE, 0.0, 8, 16, 0)  # no FFT size
     assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 1
+    cfg = _lib.SpkConfig(1000, _lib.FMT_S16LE, 0.0, 8, 16, 0)  # valid for rustfft, not planned here
+    assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 2
     cfg = _lib.SpkConfig(8192, 0x99, 0.0, 8, 16, 0)  # not a DataFormat code
     assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 1
     cfg = _lib.SpkConfig(1 << 17, _lib.FMT_U8, 0.0, 8, 16, 0)  # real N above 2^16: valid for the reference

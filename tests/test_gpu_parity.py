@@ -105,6 +105,26 @@ def test_fused_channels_without_kept_spectra(N):
         st.close()
 
 
+@pytest.mark.parametrize("N", [1024, 2048])
+def test_small_fft_fused_channels(N):
+    """The reference's own test scripts run N = 1024 and 2048
+    (test/test_spectrum.sh, test/test_combined.sh): one-wave forward
+    workgroups, with L = 256 channels computed inside the forward kernel."""
+    rng = np.random.default_rng(N + 7)
+    F, A = 8, 7
+    H = 3 * N // 4
+    x = np.clip(np.round(rng.standard_normal(6 * F * H) * 3000
+                         + 9000 * np.sin(0.05 * np.arange(6 * F * H))), -32768, 32767).astype(np.int16)
+    K = N // 2 + 1
+    chans = [(0, 256, rng.random(256).astype(np.float32)), (K - 256, 256, rng.random(256).astype(np.float32))]
+    st, ids = run_gpu(x, N, F, A, chans, max_blocks=6 * F, keep_spectra=False)
+    assert st.fused_channels() == len(chans)
+    nb = st.last_blocks()
+    X, recs, outs = oracle_path(x, N, F, A, chans)
+    compare(f"small_fused_N{N}", st, ids, chans, X, recs, outs, nb, K, spectra=False)
+    st.close()
+
+
 def test_plain_8k_full_size():
     wl = W.plain_8k()
     x = W.make_signal_np(wl)
@@ -119,7 +139,7 @@ def test_plain_8k_full_size():
     st.close()
 
 
-@pytest.mark.parametrize("N", [4096, 8192, 16384, 32768, 65536])
+@pytest.mark.parametrize("N", [1024, 2048, 4096, 8192, 16384, 32768, 65536])
 def test_every_fft_size_and_ifft_size(N):
     rng = np.random.default_rng(N)
     F, A = 4, 5
@@ -414,7 +434,7 @@ def run_complex(x_c64: np.ndarray, N, F, A, channels, batched=False, graphs=Fals
     return st, ids
 
 
-@pytest.mark.parametrize("N", [4096, 16384, 65536, 262144])
+@pytest.mark.parametrize("N", [1024, 2048, 4096, 16384, 65536, 262144])
 def test_complex_input_every_size(N):
     """cf32 input: spectra, Welch records (A straddling the call) and channels
     of every IFFT size, including channels wrapping around the spectrum edge."""
