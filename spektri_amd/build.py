@@ -25,6 +25,8 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["spektri_gpu.hip"]
+HOST_DIR = os.path.join(HERE, "host")
+HOST_BIN = os.path.join(HOST_DIR, "spektri-gpu")
 
 
 def build_lib(verbose: bool = True) -> str:
@@ -45,6 +47,22 @@ def build_lib(verbose: bool = True) -> str:
     return LIB
 
 
+def build_host(verbose: bool = True) -> str:
+    """The spektri-gpu command-line host (host/main.cpp), linked against the
+    library with a run path to it."""
+    tmp = HOST_BIN + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(HOST_DIR, "main.cpp"), "-L", HERE, "-lspektri_gpu", "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
+    if verbose:
+        print("+", " ".join(cmd), flush=True)
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"host build failed ({proc.returncode}):\n" + proc.stdout + proc.stderr)
+    os.replace(tmp, HOST_BIN)
+    return HOST_BIN
+
+
 if __name__ == "__main__":
     build_lib()
+    build_host()
     sys.exit(0)

@@ -1,0 +1,77 @@
+"""End to end through the spektri-gpu command-line host: a raw s16le file on
+standard input, spectrum records on standard output and channel records in
+the files named by the filters' file= parameters, compared with the FP64
+oracle.  Record layout: 24 bytes of metadata (data.rs:110-132), then the
+quantised bins (spectrum.rs:128-152) or F blocks of 6 (L/8) complex f32
+samples (fcfb.rs:75-89,176-187)."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import spektri_oracle as O
+
+from .gpu_util import report, rms
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "spektri_amd", "host", "spektri-gpu")
+
+
+def records(buf: bytes, body: int):
+    assert len(buf) % (24 + body) == 0, (len(buf), body)
+    out = []
+    for o in range(0, len(buf), 24 + body):
+        seq, secs, nanos, rsv = struct.unpack_from("<QQII", buf, o)
+        out.append((seq, secs, nanos, rsv, buf[o + 24:o + 24 + body]))
+    return out
+
+
+def test_command_line_host_end_to_end(tmp_path):
+    N, F, A, fs = 8192, 8, 5, 8.192e6  # bin spacing 1000 Hz
+    H = 3 * N // 4
+    nbuf = 4
+    rng = np.random.default_rng(77)
+    n = nbuf * F * H + 1000  # a trailing partial buffer is dropped (main.rs:176)
+    t = np.arange(n)
+    x = np.clip(np.round(8000 * np.cos(2 * np.pi * 1.0005e6 / fs * t) + 3000 * np.cos(2 * np.pi * 2.0e6 / fs * t)
+                         + rng.standard_normal(n) * 200), -32768, 32767).astype(np.int16)
+    raw = tmp_path / "in.s16"
+    x.tofile(raw)
+    ch0, ch1 = tmp_path / "ch0.cf32", tmp_path / "ch1.cf32"
+    args = [BIN, "--inputformat=s16le", f"--samplerate={fs:.0f}", "--centerfreq=0", f"--fftsize={N}",
+            f"--fftbuf={F}", f"--averages={A}", "--spectrumformat=u8", "--filters",
+            f"fs=256000:fc=1000000:file={ch0}", f"fs=48000:fc=2000000:file={ch1}", "fs=31000:fc=0"]
+    with open(raw, "rb") as f:
+        r = subprocess.run(args, stdin=f, capture_output=True, timeout=120)
+    assert r.returncode == 0, r.stderr.decode()
+    assert b"Invalid filter parameters" in r.stderr  # 31 kHz is not a multiple of 4 bins
+
+    geom = O.Geometry(N, False, F)
+    nb = nbuf * F
+    X = O.forward_spectra(O.frame_blocks(x.astype(np.float64), geom, nb), geom, O.input_format_scaling(O.FMT_S16LE))
+    K = N // 2 + 1
+    # spectrum records on standard output
+    recs = np.array(O.WelchAccumulator(K, N, A).accumulate(X))
+    srec = records(r.stdout, K)
+    assert [s[0] for s in srec] == list(range(len(recs))) == list(range(nb // A))
+    q = np.stack([np.frombuffer(s[4], np.uint8) for s in srec]).astype(np.int64)
+    ref = np.stack([O.quantize_spectrum(rr, A, "u8") for rr in recs]).astype(np.int64)
+    d = np.abs(q - ref)
+    # channel records in the files
+    errs = {}
+    for path, (L, first) in ((ch0, (256, 1000 - 128)), (ch1, (48, 2000 - 24))):
+        per = 6 * (L // 8)
+        crec = records(path.read_bytes(), F * per * 8)
+        assert [c[0] for c in crec] == list(range(nbuf))
+        assert all(c[3] == 0 and c[1] > 1_600_000_000 for c in crec)  # reserved word, wall-clock seconds
+        y = np.concatenate([np.frombuffer(c[4], "<c8") for c in crec]).reshape(nb, per)
+        yr = O.channel_process(X, N, first, O.raised_cosine_weights(L).astype(np.float64))
+        errs[f"L{L}_rms"] = rms(y, yr)
+    report("host_cli_end_to_end", spectrum_records=len(srec), spectrum_max_lsb=int(d.max()),
+           spectrum_frac_exact=float(np.mean(d == 0)), **errs)
+    assert d.max() <= 1 and np.mean(d == 0) > 0.99
+    assert max(errs.values()) < 1e-5, errs
