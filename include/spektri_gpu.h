@@ -111,6 +111,11 @@ const char* spk_last_error(const spk_ctx* ctx);
  * for the reference's raised-cosine weights (fcfb.rs:197-203). */
 spk_status spk_add_channel(spk_ctx* ctx, int64_t first_bin, uint32_t bins, const float* weights,
                            uint32_t* id);
+/* Remove a channel (Fcfb::process drops filters that are done,
+ * fcfb.rs:96-99; the planned subscription-driven channels need it at run
+ * time).  Its id is not reused; the next process call lays the channel set
+ * out again. */
+spk_status spk_remove_channel(spk_ctx* ctx, uint32_t id);
 /* Replace a channel's weights (same L). */
 spk_status spk_set_channel_weights(spk_ctx* ctx, uint32_t id, const float* weights);
 
@@ -234,7 +239,7 @@ spk_status spk_fused_channels(spk_ctx* ctx, uint32_t* n);
 
 /* ---- Differentiable channel weights (no counterpart in the reference; the
  * reference's weights are fixed raised cosines, fcfb.rs:197-203).  Packed
- * layouts follow channel id order: weights L floats per channel; channel
+ * layouts follow channel id order over the channels not removed: weights L floats per channel; channel
  * outputs and their gradients nblocks * 3L/4 complex per channel; gathered-
  * bin gradients nblocks * L complex per channel (nblocks = blocks of the
  * last process call).  All pointers are device pointers; the calls are

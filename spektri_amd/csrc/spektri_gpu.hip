@@ -57,6 +57,7 @@ struct DevBuf {
 struct Channel {
   int64_t first;
   int L;
+  bool removed = false;  // spk_remove_channel: keeps its id, has no output
   std::vector<float> w;
 };
 
@@ -255,6 +256,7 @@ struct spk_ctx {
 
   std::vector<Channel> chans;
   std::vector<long long> chan_out_off;  // cpx offsets per channel (capacity layout)
+  std::vector<int> chan_w_off;          // float offsets of the live channels' weights
   std::map<int, ChanGroup> groups;
   bool chans_dirty = true;
 
@@ -315,11 +317,13 @@ spk_status rebuild_channels(spk_ctx* c) {
   // context stream: no launch of an earlier call may still read them.
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   std::vector<float> wall;
-  c->chan_out_off.assign(c->chans.size(), 0);
+  c->chan_out_off.assign(c->chans.size(), -1);
+  c->chan_w_off.assign(c->chans.size(), -1);
   long long off = 0;
-  std::vector<int> w_off(c->chans.size());
+  std::vector<int>& w_off = c->chan_w_off;
   for (size_t i = 0; i < c->chans.size(); ++i) {
     const Channel& ch = c->chans[i];
+    if (ch.removed) continue;
     w_off[i] = (int)wall.size();
     wall.insert(wall.end(), ch.w.begin(), ch.w.end());
     c->chan_out_off[i] = off;
@@ -329,6 +333,7 @@ spk_status rebuild_channels(spk_ctx* c) {
   HIP_TRY(c, c->chan_out.alloc((size_t)off * sizeof(cpx)));
   c->groups.clear();
   for (size_t i = 0; i < c->chans.size(); ++i) {
+    if (c->chans[i].removed) continue;
     ChanGroup& g = c->groups[c->chans[i].L];
     g.L = c->chans[i].L;
     g.ids.push_back((uint32_t)i);
@@ -589,9 +594,7 @@ spk_status build_batch(spk_ctx* c, int nb) {
       const Channel& ch = c->chans[id];
       long long fm = ch.first % c->N;
       if (fm < 0) fm += c->N;
-      long long w_off = 0;
-      for (uint32_t j = 0; j < id; ++j) w_off += c->chans[j].L;
-      descs.push_back(spk::ChanDesc{c->chan_out_off[id], (int)fm, (int)w_off});
+      descs.push_back(spk::ChanDesc{c->chan_out_off[id], (int)fm, c->chan_w_off[id]});
     }
     const int L = ls[k], R1 = batch_R1(L), R2 = batch_R2(L), R = L / batch_T(L);
     const int R3 = L / (R1 * R2);
@@ -1186,9 +1189,18 @@ spk_status spk_add_channel(spk_ctx* c, int64_t first_bin, uint32_t bins, const f
   return SPK_OK;
 }
 
+spk_status spk_remove_channel(spk_ctx* c, uint32_t id) {
+  if (!c) return SPK_ERR_INVALID;
+  if (id >= c->chans.size() || c->chans[id].removed) return fail(c, SPK_ERR_INVALID, "no such channel");
+  c->chans[id].removed = true;
+  c->chans[id].w.clear();
+  c->chans_dirty = true;
+  return SPK_OK;
+}
+
 spk_status spk_set_channel_weights(spk_ctx* c, uint32_t id, const float* weights) {
   if (!c || !weights) return SPK_ERR_INVALID;
-  if (id >= c->chans.size()) return fail(c, SPK_ERR_INVALID, "no such channel");
+  if (id >= c->chans.size() || c->chans[id].removed) return fail(c, SPK_ERR_INVALID, "no such channel");
   Channel& ch = c->chans[id];
   ch.w.assign(weights, weights + ch.L);
   c->chans_dirty = true;
@@ -1308,7 +1320,7 @@ spk_status spk_last_blocks(const spk_ctx* c, uint32_t* nblocks) {
 
 spk_status spk_channel_output_device(spk_ctx* c, uint32_t id, const float** dptr, size_t* n) {
   if (!c || !dptr || !n) return SPK_ERR_INVALID;
-  if (id >= c->chans.size()) return fail(c, SPK_ERR_INVALID, "no such channel");
+  if (id >= c->chans.size() || c->chans[id].removed) return fail(c, SPK_ERR_INVALID, "no such channel");
   if (!c->have_output || c->chans_dirty) return fail(c, SPK_ERR_STATE, "no output yet");
   *dptr = reinterpret_cast<const float*>(c->chan_out.as<cpx>() + c->chan_out_off[id]);
   *n = (size_t)c->last_blocks * chan_out_len(c->chans[id].L) * 2;
@@ -1442,7 +1454,8 @@ spk_status spk_enable_timing(spk_ctx* c, int on) {
 
 size_t packed_weight_floats(const spk_ctx* c) {
   size_t n = 0;
-  for (const auto& ch : c->chans) n += (size_t)ch.L;
+  for (const auto& ch : c->chans)
+    if (!ch.removed) n += (size_t)ch.L;
   return n;
 }
 
@@ -1461,6 +1474,7 @@ spk_status spk_set_weights_device(spk_ctx* c, const float* w) {
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   size_t off = 0;
   for (auto& ch : c->chans) {
+    if (ch.removed) continue;
     ch.w.assign(hw.begin() + off, hw.begin() + off + ch.L);
     off += (size_t)ch.L;
   }
@@ -1469,7 +1483,7 @@ spk_status spk_set_weights_device(spk_ctx* c, const float* w) {
 
 static bool has_generic_channels(const spk_ctx* c) {
   for (const auto& ch : c->chans)
-    if (!find_chan_plan(ch.L)) return true;
+    if (!ch.removed && !find_chan_plan(ch.L)) return true;
   return false;
 }
 

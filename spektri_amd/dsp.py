@@ -147,6 +147,11 @@ class DspState:
         self.channels.append((cid.value, BinNumbers(bins, first)))
         return cid.value
 
+    def remove_channel(self, cid: int):
+        """Drop a channel (its id is not reused)."""
+        _lib.check(_lib.load().spk_remove_channel(self._ctx, cid), self._ctx)
+        self.channels = [(i, b) for i, b in self.channels if i != cid]
+
     def set_weights(self, cid: int, weights):
         lib = _lib.load()
         w = np.ascontiguousarray(weights, dtype=np.float32)
