@@ -129,7 +129,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
         print(f"bench.py runs on one GPU only (asked for {max(args.gpus, world)}): the FCFB + Welch "
-              "path is measured per GPU; see the sharding note in DESIGN.md.", file=sys.stderr)
+              "path is measured per GPU (it would shard over blocks).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -39,7 +39,7 @@ REPORT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
 
 
 def report(key: str, **vals):
-    """Append per-config error figures to gpurun_out/parity_report.json."""
+    """Append per-config error figures to the JSON parity report (REPORT)."""
     os.makedirs(os.path.dirname(REPORT), exist_ok=True)
     data = {}
     if os.path.exists(REPORT):
