@@ -58,7 +58,7 @@ typedef enum spk_status {
  * the frequency<->bin helpers below need, and minus the filter list, which
  * is added with spk_add_channel. */
 typedef struct spk_config {
-  uint32_t fft_size;          /* N, a power of two: 1024..65536 (real input), 1024..262144 (complex input) */
+  uint32_t fft_size;          /* N, a power of two: 1024..524288 (real input), 1024..262144 (complex input) */
   uint32_t input_format;      /* SPK_FMT_*: real or complex input follows the format (main.rs:35) */
   float scaling;              /* input scaling; 0 = input_format_scaling(fmt) (inputformats.rs:47-62) */
   uint32_t ffts_per_buf;      /* F: blocks per spk_process call */

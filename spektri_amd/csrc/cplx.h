@@ -157,6 +157,75 @@ struct WelchDirectParams {
   int qfmt;
 };
 
+// Real input with N above the forward kernel's range goes through the
+// complex pipeline: the complex FFT Z of the M = N/2 packed pairs
+// z[n] = x[2n] + i x[2n+1] sits in the spectrum rows, and this kernel
+// untangles it in place into the real spectrum X[0..M]:
+//   X[k] = E + W^k O,  X[M-k] = conj(E - W^k O),
+//   E = (Z[k] + conj Z[M-k]) / 2,  O = -i (Z[k] - conj Z[M-k]) / 2,
+// W = exp(-2 pi i / N) (multifft.rs:57-95 untangles a different packing,
+// two blocks per transform, into the same X).  Thread k < M/2 + 1 of block
+// y writes bins k and M - k.  Grid: x over k, y over blocks.
+struct UntangleParams {
+  cpx* spec;
+  long long spec_stride;
+  int M;
+  const cpx* tw;  // W^k, k = 0..M/2
+};
+
+__global__ void untangle_kernel(UntangleParams P) {
+  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k > P.M / 2) return;
+  cpx* Z = P.spec + (long long)blockIdx.y * P.spec_stride;
+  const int km = (P.M - k) & (P.M - 1);  // k = 0: its own mirror
+  const cpx A = Z[k], B = Z[km];
+  const cpx E = {0.5f * (A.x + B.x), 0.5f * (A.y - B.y)};
+  const cpx O = {0.5f * (A.y + B.y), -0.5f * (A.x - B.x)};
+  const cpx WO = cmul(P.tw[k], O);
+  Z[k] = {E.x + WO.x, E.y + WO.y};
+  const cpx Xm = {E.x - WO.x, WO.y - E.y};
+  if (k == 0) Z[P.M] = Xm;           // Nyquist bin
+  else if (2 * k != P.M) Z[km] = Xm;  // (k = M/2 is its own mirror)
+}
+
+// Welch records of real input from the kept spectra (spectrum.rs:95-116):
+// one thread per bin, neighbours by get_bin (fftutil.rs:8-22: beyond the
+// stored half, conj X[N - m]).  Grid: x over bins, y over groups.
+__global__ void welch_direct_real_kernel(WelchDirectParams F) {
+  const int gi = (int)blockIdx.y;
+  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const long long pos = *F.pos;
+  if (gi == 0) {
+    for (int i = k; i < F.hist_words; i += (int)(gridDim.x * blockDim.x)) F.hist_dst[i] = F.hist_src[i];
+    if (k < F.ncounters) F.chan_counters[k] = 0;
+    if (k == 0) *F.pos_next = pos + F.nblocks;
+  }
+  const int blk0_mod = (int)(pos % F.A);
+  if (k >= F.K || gi > (blk0_mod + F.nblocks - 1) / F.A) return;
+  const int g_lo = gi * F.A - blk0_mod, g_hi = g_lo + F.A;
+  const int b_lo = g_lo > 0 ? g_lo : 0;
+  const int b_hi = g_hi < F.nblocks ? g_hi : F.nblocks;
+  auto bin = [&](const cpx* X, int i) {
+    const int m = i & (F.N - 1);
+    if (m >= F.K) {
+      const cpx v = X[F.N - m];
+      return cpx{v.x, -v.y};
+    }
+    return X[m];
+  };
+  float sum = (gi == 0 && blk0_mod != 0) ? F.carry_in[k] : 0.f;
+  for (int b = b_lo; b < b_hi; ++b) {
+    const cpx* X = F.spec + (long long)b * F.spec_stride;
+    sum += welch_term(X[k], bin(X, k - 1), bin(X, k + 1));
+  }
+  if (g_hi <= F.nblocks) {
+    F.out[(long long)gi * F.K + k] = sum / (float)F.A;
+    if (F.qfmt) quantize_bin(F.qout, F.qfmt, (long long)gi * F.K + k, sum, F.A);
+  } else {
+    F.carry_out[k] = sum;
+  }
+}
+
 // Complex input (K = N, a multiple of 1024): one thread per 4 consecutive
 // bins k0..k0+3, read as two 16-byte loads per block; the outer neighbours
 // X[k0 - 1] and X[k0 + 4] come from the adjacent lanes by DPP, and lanes 0

@@ -204,6 +204,11 @@ struct spk_ctx {
   int conv_kind = 0;         // spk::ConvKind run before the transform, 0 = none
   bool f32_in = false;       // real input read as f32 (the forward kernel's f32 loader)
   bool complex_in = false;
+  // The two-kernel complex pipeline (row and column transforms of an
+  // Nc-point FFT) runs complex input (Nc = N) and real input above the
+  // forward kernel's range (Nc = N/2 packed pairs, then an untangle).
+  bool two_step = false;
+  int Nc = 0;
   float wscale = 0.f;
   int max_blocks = 0;
   const FwdPlan* fplan = nullptr;
@@ -228,6 +233,7 @@ struct spk_ctx {
   DevBuf eitems;             // Welch items of the edge bins (spk::edge_items)
   // complex input: row-transform scratch and tables (N = 64 N1)
   DevBuf cY, ctw2, ctw3, cta, ctb, ctw64;
+  DevBuf ctu;                // real two-step input: untangle twiddles W_N^k, k <= N/4
   const ChanPlan* cplan = nullptr;
   DevBuf adj_part;           // adjoint per-split weight sums
   int ncu = 0;
@@ -368,7 +374,7 @@ spk_status rebuild_channels(spk_ctx* c) {
   // The forward kernel runs the L = 256 channels itself when one 16-thread
   // tile per channel fits in its workgroup.
   const auto it = c->groups.find(kFusedL);
-  c->fused = !c->complex_in && it != c->groups.end() && (int)it->second.ids.size() * 16 <= c->M / c->fplan->R;
+  c->fused = !c->two_step && it != c->groups.end() && (int)it->second.ids.size() * 16 <= c->M / c->fplan->R;
   if (c->fused) {
     // The fused stage takes channels inside bins [0, K) with no bin shared
     // (get_bin mirrors and overlapping channels go to the channel kernel).
@@ -717,14 +723,15 @@ const void* enqueue_convert(spk_ctx* c, const void* raw, int nb, hipStream_t s) 
 // Complex input: row transforms, column transforms (natural-order spectra),
 // Welch records, channel stage, stream-position advance.
 spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, hipEvent_t* ev) {
-  const int N1 = c->N / 64;
+  const int N1 = c->Nc / 64;
+  const int pk = c->complex_in ? 1 : 2;  // real samples per complex sample of the pipeline
   if (ev) HIP_TRY(c, hipEventRecord(ev[0], s));
   src = enqueue_convert(c, src, nb, s);
   spk::CplxRowParams P{};
   P.src = reinterpret_cast<const cpx*>(src);
   P.hist = c->hist.as<cpx>();
-  P.H = c->H;
-  P.O = c->O;
+  P.H = c->H / pk;
+  P.O = c->O / pk;
   P.wscale = c->wscale;
   P.nblocks = nb;
   P.Y = c->cY.as<cpx>();
@@ -744,6 +751,16 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
   hipLaunchKernelGGL(spk::cplx_col_kernel, dim3((unsigned)(((long long)nb * N1 + 63) / 64)), dim3(spk::kChanThreads),
                      0, s, Q);
   HIP_TRY(c, hipGetLastError());
+  if (!c->complex_in) {
+    spk::UntangleParams U{};
+    U.spec = c->spec.as<cpx>();
+    U.spec_stride = c->spec_stride;
+    U.M = c->M;
+    U.tw = c->ctu.as<cpx>();
+    hipLaunchKernelGGL(spk::untangle_kernel, dim3((unsigned)((c->M / 2 + 1 + 255) / 256), (unsigned)nb), dim3(256), 0,
+                       s, U);
+    HIP_TRY(c, hipGetLastError());
+  }
   if (ev) HIP_TRY(c, hipEventRecord(ev[1], s));
   spk_status st = enqueue_channels(c, nb, s, nullptr);
   if (st != SPK_OK) return st;
@@ -754,7 +771,7 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
     W.spec_stride = c->spec_stride;
     W.N = c->N;
     W.K = c->K;
-    W.real_input = 0;
+    W.real_input = c->complex_in ? 0 : 1;
     W.nblocks = nb;
     W.A = c->A;
     W.pos = c->pos.as<long long>() + c->parity;
@@ -771,8 +788,10 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
     W.qout = c->qout.as<unsigned char>();
     W.qfmt = c->qfmt;
     const int gmax = (c->A - 1 + nb - 1) / c->A + 1;
-    // four bins per thread; K = N is a multiple of 1024
-    hipLaunchKernelGGL(spk::welch_direct_kernel, dim3(c->K / 1024, gmax), dim3(256), 0, s, W);
+    if (c->complex_in)  // four bins per thread; K = N is a multiple of 1024
+      hipLaunchKernelGGL(spk::welch_direct_kernel, dim3(c->K / 1024, gmax), dim3(256), 0, s, W);
+    else
+      hipLaunchKernelGGL(spk::welch_direct_real_kernel, dim3((c->K + 255) / 256, gmax), dim3(256), 0, s, W);
     HIP_TRY(c, hipGetLastError());
   }
   if (ev) HIP_TRY(c, hipEventRecord(ev[3], s));
@@ -835,7 +854,7 @@ spk::FinParams fin_params(const spk_ctx* c, int nb, int G, int parity) {
 // Sum the pending call's Welch records now (its partial rows are complete
 // once the launches before this one on the stream have run).
 spk_status flush_records(spk_ctx* c) {
-  if (c->complex_in || c->pend_nb == 0) return SPK_OK;
+  if (c->two_step || c->pend_nb == 0) return SPK_OK;
   const spk::FinParams F = fin_params(c, c->pend_nb, c->pend_G, c->pend_parity);
   // groups touched: at most (A - 1 + nb - 1) / A + 1; rows past the actual count exit
   const int gmax = (c->A - 1 + c->pend_nb - 1) / c->A + 1;
@@ -846,7 +865,7 @@ spk_status flush_records(spk_ctx* c) {
 }
 
 spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hipEvent_t* ev) {
-  if (c->complex_in) return enqueue_complex(c, src, nb, s, ev);
+  if (c->two_step) return enqueue_complex(c, src, nb, s, ev);
   if (ev) HIP_TRY(c, hipEventRecord(ev[0], s));
   src = enqueue_convert(c, src, nb, s);
   spk::FwdParams P{};
@@ -918,7 +937,7 @@ void drop_graphs(spk_ctx* c) {
 // replaying it as a graph leaves a longer idle gap between calls than
 // launching the two kernels directly.
 bool graph_pays(const spk_ctx* c) {
-  if (c->complex_in) return true;
+  if (c->two_step) return true;
   return c->groups.size() > (c->fused ? 1u : 0u);
 }
 
@@ -973,7 +992,7 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
     spk_status st = enqueue_blocks(c, src, nb, s, ev);
     if (st != SPK_OK) return st;
   }
-  if (!c->complex_in) {
+  if (!c->two_step) {
     c->pend_nb = nb;
     c->pend_G = fwd_grid(c, nb);
     c->pend_parity = c->parity;
@@ -1009,9 +1028,12 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
     return SPK_ERR_INVALID;
   }
   const bool cplx = fi->complex_in;
+  // real N beyond the forward kernel's plans: the complex pipeline on N/2 pairs
+  const bool big_real = !cplx && find_fwd_plan((int)N / 2) == nullptr && N > 65536;
   // sizes the reference accepts (rustfft plans any N) but this library does
-  // not implement: N not a power of two, or outside the planned range
-  if ((N & (N - 1)) != 0 || !(cplx ? (N >= 1024 && N <= 262144) : find_fwd_plan((int)N / 2) != nullptr)) {
+  // not implement: N not a power of two, or outside the planned ranges
+  if ((N & (N - 1)) != 0 ||
+      !(cplx ? (N >= 1024 && N <= 262144) : (big_real ? N <= 524288 : find_fwd_plan((int)N / 2) != nullptr))) {
     delete c;
     return SPK_ERR_UNSUPPORTED;
   }
@@ -1027,13 +1049,27 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   c->raw_bps = fi->raw_bps;
   c->conv_kind = fi->conv;
   c->f32_in = !cplx && fi->canon_bps == 4;
+  c->two_step = cplx || big_real;
+  c->Nc = cplx ? c->N : c->N / 2;
+  if (big_real) {
+    // the row transforms read f32 pairs as complex samples
+    c->bps = 4;
+    switch (cfg->input_format) {
+      case SPK_FMT_S8: c->conv_kind = spk::kConvS8ToF32; break;
+      case SPK_FMT_S16LE: c->conv_kind = spk::kConvS16leToF32; break;
+      case SPK_FMT_S16BE: c->conv_kind = spk::kConvS16beToF32; break;
+      case SPK_FMT_U8: c->conv_kind = spk::kConvU8ToF32; break;
+      case SPK_FMT_F32BE: c->conv_kind = spk::kConvF32beToF32; break;
+      default: c->conv_kind = 0; break;  // f32le
+    }
+  }
   // input_format_scaling (inputformats.rs:47-62)
   const float scaling = cfg->scaling != 0.f ? cfg->scaling : fi->scaling;
   c->wscale = scaling / (float)c->N;  // dsp.rs:70-73
   c->max_blocks = (int)(cfg->max_blocks ? cfg->max_blocks : cfg->ffts_per_buf);
   if (c->max_blocks < c->F) c->max_blocks = c->F;
-  c->fplan = cplx ? nullptr : find_fwd_plan(c->M);
-  c->cplan = cplx ? find_chan_plan(c->N / 64) : nullptr;
+  c->fplan = c->two_step ? nullptr : find_fwd_plan(c->M);
+  c->cplan = c->two_step ? find_chan_plan(c->Nc / 64) : nullptr;
 
   auto cleanup = [&](spk_status s) {
     spk_destroy(c);
@@ -1042,8 +1078,9 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) return cleanup(SPK_ERR_DEVICE);
   c->stream = c->own_stream;
 
-  // complex input always keeps the spectra (the channel stage reads them)
-  c->keep_spec = c->complex_in || (cfg->flags & SPK_FLAG_KEEP_SPECTRA) != 0;
+  // the two-step pipeline always keeps the spectra (Welch and the channel
+  // stage read them)
+  c->keep_spec = c->two_step || (cfg->flags & SPK_FLAG_KEEP_SPECTRA) != 0;
   // Persistent forward grid: resident workgroups per CU times CUs.
   int dev = 0, ncu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
@@ -1061,16 +1098,16 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   c->qfmt = (cfg->flags & SPK_FLAG_SPECTRUM_U8) ? 1 : ((cfg->flags & SPK_FLAG_SPECTRUM_U16) ? 2 : 0);
   if (c->qfmt && c->qout.alloc((size_t)c->welch_out_cap * c->K * c->qfmt) != hipSuccess)
     return cleanup(SPK_ERR_DEVICE);
-  if (c->complex_in) {
+  if (c->two_step) {
     // every spectrum row is kept: the channel stage always reads them
     if ((long long)(c->max_blocks + c->A) * 64 >= (1LL << 31)) return cleanup(SPK_ERR_UNSUPPORTED);
     const ChanPlan& p = *c->cplan;
-    const int N1 = c->N / 64;
+    const int N1 = c->Nc / 64;
     if (c->hist.alloc((size_t)c->O * c->bps) != hipSuccess ||
         c->staging.alloc((size_t)c->F * c->H * c->raw_bps) != hipSuccess ||
         (c->conv_kind && c->conv.alloc((size_t)c->max_blocks * c->H * c->bps) != hipSuccess) ||
         c->spec.alloc((size_t)c->max_blocks * c->spec_stride * sizeof(cpx)) != hipSuccess ||
-        c->cY.alloc((size_t)c->max_blocks * c->N * sizeof(cpx)) != hipSuccess ||
+        c->cY.alloc((size_t)c->max_blocks * c->Nc * sizeof(cpx)) != hipSuccess ||
         c->carry.alloc((size_t)2 * c->spec_stride * sizeof(float)) != hipSuccess ||
         c->welch_out.alloc((size_t)c->welch_out_cap * c->K * sizeof(float)) != hipSuccess)
       return cleanup(SPK_ERR_DEVICE);
@@ -1084,8 +1121,13 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
     };
     std::vector<cpx> ta(64 * 64), tb((size_t)64 * std::max(1, N1 / 64)), t64(64);
     for (int n2 = 0; n2 < 64; ++n2) {
-      for (int a = 0; a < 64; ++a) ta[(size_t)n2 * 64 + a] = wn((long long)n2 * a, c->N);
-      for (int b = 0; b < std::max(1, N1 / 64); ++b) tb[(size_t)n2 * std::max(1, N1 / 64) + b] = wn(64LL * n2 * b, c->N);
+      for (int a = 0; a < 64; ++a) ta[(size_t)n2 * 64 + a] = wn((long long)n2 * a, c->Nc);
+      for (int b = 0; b < std::max(1, N1 / 64); ++b) tb[(size_t)n2 * std::max(1, N1 / 64) + b] = wn(64LL * n2 * b, c->Nc);
+    }
+    if (!c->complex_in) {
+      std::vector<cpx> tu((size_t)c->M / 2 + 1);
+      for (size_t k = 0; k < tu.size(); ++k) tu[k] = wn((long long)k, c->N);
+      if (upload(c->ctu, tu) != hipSuccess) return cleanup(SPK_ERR_DEVICE);
     }
     for (int m = 0; m < 64; ++m) t64[m] = wn(m, 64);
     if (upload(c->ctw2, pass_table(p.R2, p.R1, -1)) != hipSuccess ||

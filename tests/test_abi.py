@@ -39,7 +39,7 @@ def test_null_and_invalid_arguments_return_codes():
     assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 2
     cfg = _lib.SpkConfig(8192, 0x99, 0.0, 8, 16, 0)  # not a DataFormat code
     assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 1
-    cfg = _lib.SpkConfig(1 << 17, _lib.FMT_U8, 0.0, 8, 16, 0)  # real N above 2^16: valid for the reference
+    cfg = _lib.SpkConfig(1 << 20, _lib.FMT_U8, 0.0, 8, 16, 0)  # real N above 2^19: valid for the reference
     assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 2
     cfg = _lib.SpkConfig(1 << 20, _lib.FMT_CF32LE, 0.0, 8, 16, 0)  # complex N above 2^18
     assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 2

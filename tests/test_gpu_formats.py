@@ -22,8 +22,7 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
-N, F, A = 8192, 4, 3
-H = N - N // 4
+F, A = 4, 3
 
 FORMATS = {
     "s8": O.FMT_S8, "s16le": O.FMT_S16LE, "s16be": O.FMT_S16BE, "f32le": O.FMT_F32LE,
@@ -52,7 +51,7 @@ def raw_stream(fmt: int, nsamples: int, rng) -> np.ndarray:
     return np.clip(np.round(top * v), -top - 1, top).astype(dt)
 
 
-def channels_for(cplx: bool, rng, only256=False):
+def channels_for(N, cplx: bool, rng, only256=False):
     K = N if cplx else N // 2 + 1
     if only256:
         return [(int(f) // 4 * 4, 256, rng.random(256).astype(np.float32)) for f in (40, 1000, 2400, K - 300)]
@@ -61,7 +60,7 @@ def channels_for(cplx: bool, rng, only256=False):
             (K - 600, 1024, rng.random(1024).astype(np.float32))]  # spans Nyquist (real) / the top bins
 
 
-def oracle_blocks(raw, fmt, first, nb):
+def oracle_blocks(N, raw, fmt, first, nb):
     cplx = O.is_input_format_complex(fmt)
     geom = O.Geometry(N, cplx, F)
     x = O.convert(raw, fmt)
@@ -69,7 +68,7 @@ def oracle_blocks(raw, fmt, first, nb):
     return X
 
 
-def check_call(tag, st, ids, chans, X, acc, keep_spectra):
+def check_call(N, tag, st, ids, chans, X, acc, keep_spectra):
     nb = X.shape[0]
     assert st.last_blocks() == nb
     vals = {}
@@ -87,12 +86,18 @@ def check_call(tag, st, ids, chans, X, acc, keep_spectra):
     assert all(v < TOL for v in vals.values()), vals
 
 
-@pytest.mark.parametrize("name", list(FORMATS))
-def test_input_format(name):
+CASES = [(name, 8192) for name in FORMATS] + [("s16le", 131072), ("u8", 131072), ("f32be", 262144)]
+
+
+@pytest.mark.parametrize("name,N", CASES)
+def test_input_format(name, N):
+    """N = 8192 for every format; real N above 65536 (the complex pipeline
+    on packed pairs) for a converted and an in-place format."""
     fmt = FORMATS[name]
+    H = N - N // 4
     cplx = O.is_input_format_complex(fmt)
     rng = np.random.default_rng(fmt)
-    chans = channels_for(cplx, rng)
+    chans = channels_for(N, cplx, rng)
     per_buf = F * H
     raw = raw_stream(fmt, 5 * per_buf, rng)
     bps = O.bytes_per_input_sample(fmt)
@@ -106,20 +111,20 @@ def test_input_format(name):
     acc = O.WelchAccumulator(N if cplx else N // 2 + 1, N, A)
     for call in range(2):
         st.process(raw_bytes[call * per_buf * bps:(call + 1) * per_buf * bps])
-        X = oracle_blocks(raw, fmt, call * F, F)
-        check_call(f"format_{name}_host_call{call}", st, ids, chans, X, acc, True)
+        X = oracle_blocks(N, raw, fmt, call * F, F)
+        check_call(N, f"format_{name}_N{N}_host_call{call}", st, ids, chans, X, acc, True)
 
     # then three buffers from device memory, continuing the same stream
     dev = torch.from_numpy(raw_bytes[2 * per_buf * bps:].copy()).to("cuda")
     st.process_device(dev.data_ptr(), 3 * per_buf)
     st.synchronize()
-    X = oracle_blocks(raw, fmt, 2 * F, 3 * F)
-    check_call(f"format_{name}_device", st, ids, chans, X, acc, True)
+    X = oracle_blocks(N, raw, fmt, 2 * F, 3 * F)
+    check_call(N, f"format_{name}_N{N}_device", st, ids, chans, X, acc, True)
     st.close()
 
-    if not cplx:
+    if not cplx and N <= 65536:
         # L = 256 channels only: the forward kernel runs them (no spectrum rows)
-        chans = channels_for(False, rng, only256=True)
+        chans = channels_for(N, False, rng, only256=True)
         st, _ = DspState.init(DspParams(complex=False, fs_in=1.0, fft_size=N, ffts_per_buf=F, spectrum_averages=A,
                                         input_format=fmt, max_blocks=5 * F))
         ids = [st.add_channel(f, L, w) for f, L, w in chans]
@@ -128,10 +133,10 @@ def test_input_format(name):
         st.synchronize()
         assert st.fused_channels() == len(chans)
         acc = O.WelchAccumulator(N // 2 + 1, N, A)
-        check_call(f"format_{name}_fused", st, ids, chans, oracle_blocks(raw, fmt, 0, 5 * F), acc, False)
+        check_call(N, f"format_{name}_fused", st, ids, chans, oracle_blocks(N, raw, fmt, 0, 5 * F), acc, False)
         st.close()
 
 
 def test_format_flag_mismatch_rejected():
     with pytest.raises(ValueError):
-        DspState.init(DspParams(complex=False, fs_in=1.0, fft_size=N, input_format=_lib.FMT_CS16LE))
+        DspState.init(DspParams(complex=False, fs_in=1.0, fft_size=8192, input_format=_lib.FMT_CS16LE))

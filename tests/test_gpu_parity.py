@@ -139,8 +139,10 @@ def test_plain_8k_full_size():
     st.close()
 
 
-@pytest.mark.parametrize("N", [1024, 2048, 4096, 8192, 16384, 32768, 65536])
+@pytest.mark.parametrize("N", [1024, 2048, 4096, 8192, 16384, 32768, 65536, 131072, 262144, 524288])
 def test_every_fft_size_and_ifft_size(N):
+    """N up to 65536 runs in the forward kernel, larger N through the complex
+    pipeline on N/2 packed pairs and an untangle."""
     rng = np.random.default_rng(N)
     F, A = 4, 5
     H = 3 * N // 4
