@@ -214,6 +214,7 @@ def main():
     dominant = "fwd" if all_fused or kern["forward_ms"] >= chan_ms else "chan"
     roof = {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBPS,
             "kernel": (("cplx_row_kernel + cplx_col_kernel" if wl.complex else
+                        "cplx_row_kernel + cplx_col_kernel + untangle_kernel" if wl.fft_size > 65536 else
                         f"fwd_real_kernel<M={wl.fft_size // 2}, FL={256 if all_fused else 0}>") if dominant == "fwd"
                        else "channel stage (" + ("batched" if tune and tune["mode"] == "batched" else "per-size") + ")"),
             "achieved": round(fwd_gbps if dominant == "fwd" else chan_gbps, 1),

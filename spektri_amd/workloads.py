@@ -133,6 +133,20 @@ def rx888_64k(nsamples: int | None = None) -> Workload:
     return wl
 
 
+def rx888_256k(nsamples: int | None = None) -> Workload:
+    """rx888_64k's signal and channel centres at N = 2^18 (500 Hz bins, the
+    two-step path for real input): 2^26 samples, F = 2 (340 blocks), Welch
+    over 8 blocks, 64 channels of 1024 bins (the same 512 kHz as L = 256 at
+    N = 2^16)."""
+    wl = rx888_64k(nsamples)
+    N = 262144
+    w = Workload("rx888_256k", wl.seed, N, wl.fs, 2, 8, wl.nsamples)
+    centres = round4(np.linspace(4000, 128000, 64))
+    w.channels = [(int(c) - 512, 1024, raised_cosine_transition(1024, 0.2)) for c in centres]
+    w.tones, w.sweep_amp, w.noise_sigma = wl.tones, wl.sweep_amp, wl.noise_sigma
+    return w
+
+
 def uneven_64k(nsamples: int | None = None) -> Workload:
     """Batched uneven channel set: as rx888_64k (signal, N, Welch over 32
     blocks) at F = 4 (1364 blocks), with 512 channels of IFFT size drawn
@@ -253,7 +267,8 @@ def design_16k(nsamples: int | None = None) -> Workload:
     return wl
 
 
-WORKLOADS = {"plain_8k": plain_8k, "rx888_64k": rx888_64k, "uneven_64k": uneven_64k, "wideband_256k": wideband_256k,
+WORKLOADS = {"plain_8k": plain_8k, "rx888_64k": rx888_64k, "rx888_256k": rx888_256k, "uneven_64k": uneven_64k,
+             "wideband_256k": wideband_256k,
              "design_16k": design_16k}
 
 
