@@ -37,10 +37,10 @@ typedef enum spk_status {
 } spk_status;
 
 /* DataFormat codes of spektri/src/dsp/data.rs:81-95; every one is an input
- * format (inputformats.rs:12-155).  s16le, f32le and cf32le are read in
- * place by the transform kernels; the others pass through one conversion
- * kernel first (into s16, f32 or cf32, with the reference's arithmetic:
- * u8/cu8 subtract 127.4 in f32, big-endian formats swap bytes). */
+ * format (inputformats.rs:12-155).  s16le, f32le, cs16le and cf32le are
+ * read in place by the transform kernels; the others pass through one
+ * conversion kernel first (into s16, f32 or cf32, with the reference's
+ * arithmetic: u8/cu8 subtract 127.4 in f32, big-endian formats swap bytes). */
 #define SPK_FMT_S8 0x04
 #define SPK_FMT_S16LE 0x08
 #define SPK_FMT_S16BE 0x09

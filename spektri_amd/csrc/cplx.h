@@ -15,8 +15,8 @@
 namespace spk {
 
 struct CplxRowParams {
-  const cpx* src;   // stream samples; sample s >= 0 at src[s]
-  const cpx* hist;  // the O samples before src[0]
+  const void* src;   // stream samples (cf32, or s16 pairs for the S16 loader); sample s >= 0 at src[s]
+  const void* hist;  // the O samples before src[0], same form
   int H, O;
   float wscale;
   int nblocks;
@@ -30,7 +30,9 @@ struct CplxRowParams {
 // Workgroup w serves XCD w mod 8; block g goes to XCD g mod 8, so the 64
 // row transforms of one block (which read interleaved samples of the same
 // cache lines) share one L2.  Grid: 8 * ceil(nblocks / 8) * (64 / TPW).
-template <int N1, int R, int R1, int R2, int R3>
+// S16: a complex sample is a 32-bit pair of s16 (cs16le input, and real
+// s16le input read as x[2n] + i x[2n+1]), converted in the load.
+template <int N1, int R, int R1, int R2, int R3, bool S16>
 __global__ __launch_bounds__(kChanThreads) void cplx_row_kernel(CplxRowParams P) {
   constexpr int T = N1 / R;
   static_assert(R1 * R2 * R3 == N1, "radix plan must cover N1");
@@ -54,7 +56,13 @@ __global__ __launch_bounds__(kChanThreads) void cplx_row_kernel(CplxRowParams P)
   for (int e = 0; e < R; ++e) {
     const int n1 = st_in_index<N1, R, T, R1>(t, e);
     const long long si = s0 + 64LL * n1 + n2;
-    const cpx a = si < 0 ? P.hist[P.O + si] : P.src[si];
+    cpx a;
+    if constexpr (S16) {
+      const int w = si < 0 ? static_cast<const int*>(P.hist)[P.O + si] : static_cast<const int*>(P.src)[si];
+      a = {(float)(int16_t)w, (float)(w >> 16)};
+    } else {
+      a = si < 0 ? static_cast<const cpx*>(P.hist)[P.O + si] : static_cast<const cpx*>(P.src)[si];
+    }
     v[e] = cscale(a, P.wscale);
   }
   st_butterflies<N1, R, T, R1, 1, -1>(v, t, nullptr);
@@ -188,23 +196,31 @@ __global__ void untangle_kernel(UntangleParams P) {
   else if (2 * k != P.M) Z[km] = Xm;  // (k = M/2 is its own mirror)
 }
 
-// Welch records of real input from the kept spectra (spectrum.rs:95-116):
-// one thread per bin, neighbours by get_bin (fftutil.rs:8-22: beyond the
-// stored half, conj X[N - m]).  Grid: x over bins, y over groups.
+// Welch records of real input from the kept spectra (spectrum.rs:95-116),
+// K = N/2 + 1 bins: one thread per four bins k0..k0+3 (two 16-byte loads
+// per block; the row stride leaves room past the Nyquist bin), the outer
+// neighbours X[k0 - 1] and X[k0 + 4] from the adjacent lanes by DPP; lanes 0
+// and 63 of a wave, and the quads at either end of the spectrum, load them
+// with get_bin's mirror rule (fftutil.rs:8-22: beyond the stored half,
+// conj X[N - m]).  Grid: x over quads (whole waves), y over groups.
 __global__ void welch_direct_real_kernel(WelchDirectParams F) {
   const int gi = (int)blockIdx.y;
-  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int k0 = 4 * tid;
   const long long pos = *F.pos;
   if (gi == 0) {
-    for (int i = k; i < F.hist_words; i += (int)(gridDim.x * blockDim.x)) F.hist_dst[i] = F.hist_src[i];
-    if (k < F.ncounters) F.chan_counters[k] = 0;
-    if (k == 0) *F.pos_next = pos + F.nblocks;
+    for (int i = tid; i < F.hist_words; i += (int)(gridDim.x * blockDim.x)) F.hist_dst[i] = F.hist_src[i];
+    if (tid < F.ncounters) F.chan_counters[tid] = 0;
+    if (tid == 0) *F.pos_next = pos + F.nblocks;
   }
   const int blk0_mod = (int)(pos % F.A);
-  if (k >= F.K || gi > (blk0_mod + F.nblocks - 1) / F.A) return;
+  if (gi > (blk0_mod + F.nblocks - 1) / F.A) return;  // whole workgroup
   const int g_lo = gi * F.A - blk0_mod, g_hi = g_lo + F.A;
   const int b_lo = g_lo > 0 ? g_lo : 0;
   const int b_hi = g_hi < F.nblocks ? g_hi : F.nblocks;
+  const int lane = (int)(threadIdx.x & 63);
+  const bool own = k0 < F.K;  // quads past the spectrum compute (DPP partners) but store nothing
+  const int kq = own ? k0 : 0;
   auto bin = [&](const cpx* X, int i) {
     const int m = i & (F.N - 1);
     if (m >= F.K) {
@@ -213,16 +229,39 @@ __global__ void welch_direct_real_kernel(WelchDirectParams F) {
     }
     return X[m];
   };
-  float sum = (gi == 0 && blk0_mod != 0) ? F.carry_in[k] : 0.f;
+  // neighbours that the lanes next door do not hold
+  const bool load_l = lane == 0 || kq == 0;
+  const bool load_r = lane == 63 || kq + 4 >= F.K;
+  float4 s = {0.f, 0.f, 0.f, 0.f};
+  if (gi == 0 && blk0_mod != 0 && own) s = *reinterpret_cast<const float4*>(F.carry_in + kq);
   for (int b = b_lo; b < b_hi; ++b) {
     const cpx* X = F.spec + (long long)b * F.spec_stride;
-    sum += welch_term(X[k], bin(X, k - 1), bin(X, k + 1));
+    const float4 u = *reinterpret_cast<const float4*>(X + kq);      // X[k0], X[k0+1]
+    const float4 v = *reinterpret_cast<const float4*>(X + kq + 2);  // X[k0+2], X[k0+3]
+    cpx L = {wave_shr1(v.z), wave_shr1(v.w)};
+    cpx Rn = {wave_shl1(u.x), wave_shl1(u.y)};
+    if (load_l) L = bin(X, kq - 1);
+    if (load_r) Rn = bin(X, kq + 4);
+    cpx x0 = {u.x, u.y}, x1 = {u.z, u.w}, x2 = {v.x, v.y}, x3 = {v.z, v.w};
+    // bins past K - 1 (the last quad) are neighbours by get_bin, not stored values
+    if (kq + 1 >= F.K) x1 = bin(X, kq + 1);
+    if (kq + 2 >= F.K) x2 = bin(X, kq + 2);
+    if (kq + 3 >= F.K) x3 = bin(X, kq + 3);
+    s.x += welch_term(x0, L, x1);
+    s.y += welch_term(x1, x0, x2);
+    s.z += welch_term(x2, x1, x3);
+    s.w += welch_term(x3, x2, Rn);
   }
+  if (!own) return;
+  const float sv[4] = {s.x, s.y, s.z, s.w};
   if (g_hi <= F.nblocks) {
-    F.out[(long long)gi * F.K + k] = sum / (float)F.A;
-    if (F.qfmt) quantize_bin(F.qout, F.qfmt, (long long)gi * F.K + k, sum, F.A);
+    const long long o = (long long)gi * F.K + kq;
+    for (int j = 0; j < 4 && kq + j < F.K; ++j) {
+      F.out[o + j] = sv[j] / (float)F.A;
+      if (F.qfmt) quantize_bin(F.qout, F.qfmt, o + j, sv[j], F.A);
+    }
   } else {
-    F.carry_out[k] = sum;
+    for (int j = 0; j < 4 && kq + j < F.K; ++j) F.carry_out[kq + j] = sv[j];
   }
 }
 

@@ -209,6 +209,7 @@ struct spk_ctx {
   // forward kernel's range (Nc = N/2 packed pairs, then an untangle).
   bool two_step = false;
   int Nc = 0;
+  bool s16_pairs = false;    // two-step path reading s16 pairs (cs16le, or real s16le)
   float wscale = 0.f;
   int max_blocks = 0;
   const FwdPlan* fplan = nullptr;
@@ -680,23 +681,26 @@ spk_status launch_batched(spk_ctx* c, int nb, hipStream_t s) {
 // Kernels only (no memset or copy nodes), so it can be captured into a
 // graph; every per-call quantity the kernels need is in device memory.
 template <int N1, int R, int R1, int R2, int R3>
-void launch_cplx_rows(const spk::CplxRowParams& P, int nb, hipStream_t s) {
+void launch_cplx_rows(const spk::CplxRowParams& P, int nb, bool s16, hipStream_t s) {
   constexpr int TPW = spk::kChanThreads / (N1 / R);
   const int grid = 8 * ((nb + 7) / 8) * (64 / TPW);
-  hipLaunchKernelGGL((spk::cplx_row_kernel<N1, R, R1, R2, R3>), dim3(grid), dim3(spk::kChanThreads), 0, s, P);
+  if (s16)
+    hipLaunchKernelGGL((spk::cplx_row_kernel<N1, R, R1, R2, R3, true>), dim3(grid), dim3(spk::kChanThreads), 0, s, P);
+  else
+    hipLaunchKernelGGL((spk::cplx_row_kernel<N1, R, R1, R2, R3, false>), dim3(grid), dim3(spk::kChanThreads), 0, s, P);
 }
 
-hipError_t dispatch_cplx_rows(int N1, const spk::CplxRowParams& P, int nb, hipStream_t s) {
+hipError_t dispatch_cplx_rows(int N1, const spk::CplxRowParams& P, int nb, bool s16, hipStream_t s) {
   switch (N1) {
-    case 16: launch_cplx_rows<16, 4, 4, 4, 1>(P, nb, s); break;
-    case 32: launch_cplx_rows<32, 8, 8, 4, 1>(P, nb, s); break;
-    case 64: launch_cplx_rows<64, 8, 8, 8, 1>(P, nb, s); break;
-    case 128: launch_cplx_rows<128, 16, 16, 8, 1>(P, nb, s); break;
-    case 256: launch_cplx_rows<256, 16, 16, 16, 1>(P, nb, s); break;
-    case 512: launch_cplx_rows<512, 32, 32, 16, 1>(P, nb, s); break;
-    case 1024: launch_cplx_rows<1024, 32, 32, 32, 1>(P, nb, s); break;
-    case 2048: launch_cplx_rows<2048, 32, 32, 32, 2>(P, nb, s); break;
-    case 4096: launch_cplx_rows<4096, 16, 16, 16, 16>(P, nb, s); break;
+    case 16: launch_cplx_rows<16, 4, 4, 4, 1>(P, nb, s16, s); break;
+    case 32: launch_cplx_rows<32, 8, 8, 4, 1>(P, nb, s16, s); break;
+    case 64: launch_cplx_rows<64, 8, 8, 8, 1>(P, nb, s16, s); break;
+    case 128: launch_cplx_rows<128, 16, 16, 8, 1>(P, nb, s16, s); break;
+    case 256: launch_cplx_rows<256, 16, 16, 16, 1>(P, nb, s16, s); break;
+    case 512: launch_cplx_rows<512, 32, 32, 16, 1>(P, nb, s16, s); break;
+    case 1024: launch_cplx_rows<1024, 32, 32, 32, 1>(P, nb, s16, s); break;
+    case 2048: launch_cplx_rows<2048, 32, 32, 32, 2>(P, nb, s16, s); break;
+    case 4096: launch_cplx_rows<4096, 16, 16, 16, 16>(P, nb, s16, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -728,8 +732,8 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
   if (ev) HIP_TRY(c, hipEventRecord(ev[0], s));
   src = enqueue_convert(c, src, nb, s);
   spk::CplxRowParams P{};
-  P.src = reinterpret_cast<const cpx*>(src);
-  P.hist = c->hist.as<cpx>();
+  P.src = src;
+  P.hist = c->hist.p;
   P.H = c->H / pk;
   P.O = c->O / pk;
   P.wscale = c->wscale;
@@ -739,7 +743,7 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
   P.tw3 = c->ctw3.as<cpx>();
   P.ta = c->cta.as<cpx>();
   P.tb = c->ctb.as<cpx>();
-  HIP_TRY(c, dispatch_cplx_rows(N1, P, nb, s));
+  HIP_TRY(c, dispatch_cplx_rows(N1, P, nb, c->s16_pairs, s));
   spk::CplxColParams Q{};
   Q.Y = c->cY.as<cpx>();
   Q.N1 = N1;
@@ -791,7 +795,7 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
     if (c->complex_in)  // four bins per thread; K = N is a multiple of 1024
       hipLaunchKernelGGL(spk::welch_direct_kernel, dim3(c->K / 1024, gmax), dim3(256), 0, s, W);
     else
-      hipLaunchKernelGGL(spk::welch_direct_real_kernel, dim3((c->K + 255) / 256, gmax), dim3(256), 0, s, W);
+      hipLaunchKernelGGL(spk::welch_direct_real_kernel, dim3((c->K + 1023) / 1024, gmax), dim3(256), 0, s, W);
     HIP_TRY(c, hipGetLastError());
   }
   if (ev) HIP_TRY(c, hipEventRecord(ev[3], s));
@@ -1051,12 +1055,23 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   c->f32_in = !cplx && fi->canon_bps == 4;
   c->two_step = cplx || big_real;
   c->Nc = cplx ? c->N : c->N / 2;
+  if (cfg->input_format == SPK_FMT_CS16LE) {
+    // the row transforms convert s16 pairs as they load them
+    c->conv_kind = 0;
+    c->bps = 4;
+    c->s16_pairs = true;
+  }
   if (big_real) {
-    // the row transforms read f32 pairs as complex samples
+    // the row transforms read pairs of samples as complex samples: s16 pairs
+    // for s16le, f32 pairs for every other format
     c->bps = 4;
     switch (cfg->input_format) {
       case SPK_FMT_S8: c->conv_kind = spk::kConvS8ToF32; break;
-      case SPK_FMT_S16LE: c->conv_kind = spk::kConvS16leToF32; break;
+      case SPK_FMT_S16LE:
+        c->conv_kind = 0;
+        c->bps = 2;
+        c->s16_pairs = true;
+        break;
       case SPK_FMT_S16BE: c->conv_kind = spk::kConvS16beToF32; break;
       case SPK_FMT_U8: c->conv_kind = spk::kConvU8ToF32; break;
       case SPK_FMT_F32BE: c->conv_kind = spk::kConvF32beToF32; break;
