@@ -1,10 +1,10 @@
 // Input formats without a loader of their own in the transform kernels
 // (spektri/src/inputformats.rs:65-155): one elementwise pass turns the raw
 // stream into the form a loader reads -- s16 (real s8, s16be), f32 (real u8,
-// f32be) or cf32 (every complex format but cf32le) -- with the reference's
+// f32be) or cf32 (cs8, cu8, cs16be, cf32be) -- with the reference's
 // per-sample arithmetic: integers convert exactly, u8/cu8 subtract 127.4 in
 // f32 (inputformats.rs:70,108-109), big-endian formats swap bytes.  s16le,
-// f32le and cf32le are read in place and never pass through here.
+// f32le, cs16le and cf32le are read in place and never pass through here.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -20,7 +20,6 @@ enum ConvKind : int {
   kConvU8ToF32 = 3,     // u8, cu8: 4 numbers -> 4 f32 (x - 127.4)
   kConvS8ToF32 = 4,     // cs8: 4 numbers -> 4 f32
   kConvF32beToF32 = 5,  // f32be, cf32be: 1 number -> 1 f32
-  kConvS16leToF32 = 6,  // cs16le: 2 numbers -> 2 f32
   kConvS16beToF32 = 7,  // cs16be: 2 numbers -> 2 f32
 };
 
@@ -65,9 +64,6 @@ __global__ void convert_kernel(ConvParams P) {
         break;
       case kConvF32beToF32:
         static_cast<uint32_t*>(P.dst)[i] = __builtin_bswap32(v);
-        break;
-      case kConvS16leToF32:
-        static_cast<float2*>(P.dst)[i] = make_float2((float)(int16_t)(v & 0xFFFF), (float)(int16_t)(v >> 16));
         break;
       case kConvS16beToF32: {
         const uint32_t s = bswap16x2(v);

@@ -171,7 +171,7 @@ const FormatInfo* format_info(uint32_t fmt) {
       {0x1D, false, 4, spk::kConvF32beToF32, 4, 1.0f},              // F32be
       {0x24, false, 1, spk::kConvU8ToF32, 4, 1.0f / 127.0f},        // U8
       {0x44, true, 2, spk::kConvS8ToF32, 8, 1.0f / 127.0f},         // Cs8
-      {0x48, true, 4, spk::kConvS16leToF32, 8, 1.0f / 32767.0f},    // Cs16le
+      {0x48, true, 4, 0, 4, 1.0f / 32767.0f},                      // Cs16le (s16 pairs, read in place)
       {0x49, true, 4, spk::kConvS16beToF32, 8, 1.0f / 32767.0f},    // Cs16be
       {0x5C, true, 8, 0, 8, 1.0f},                                  // Cf32le
       {0x5D, true, 8, spk::kConvF32beToF32, 8, 1.0f},               // Cf32be
@@ -1055,12 +1055,8 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   c->f32_in = !cplx && fi->canon_bps == 4;
   c->two_step = cplx || big_real;
   c->Nc = cplx ? c->N : c->N / 2;
-  if (cfg->input_format == SPK_FMT_CS16LE) {
-    // the row transforms convert s16 pairs as they load them
-    c->conv_kind = 0;
-    c->bps = 4;
-    c->s16_pairs = true;
-  }
+  // the row transforms convert s16 pairs as they load them
+  if (cfg->input_format == SPK_FMT_CS16LE) c->s16_pairs = true;
   if (big_real) {
     // the row transforms read pairs of samples as complex samples: s16 pairs
     // for s16le, f32 pairs for every other format
