@@ -88,6 +88,13 @@ def freq_to_bins_exact(fs_in, fc_in, fft_size, fs_out, fc_out):
     return BinNumbers(b.value, f.value) if st == _lib.SPK_OK else None
 
 
+def nearest_freq(fs_in, fc_in, fft_size, fs_out, fc_out):
+    """Fcfb::nearest_freq (fcfb.rs:108-115): the nearest (fs_out, fc_out)
+    that add_filter accepts, or None."""
+    bn = freq_to_bins(fs_in, fc_in, fft_size, fs_out, fc_out)
+    return None if bn is None else bins_to_freq(fs_in, fc_in, fft_size, bn)
+
+
 class DspState:
     """GPU-backed DspState.  Use ``DspState.init(params)``."""
 

@@ -74,3 +74,19 @@ def test_freq_to_bins_reference_known_answer():
     fso, fco = ctypes.c_double(), ctypes.c_double()
     assert lib.spk_bins_to_freq(128.0e6, 0.0, 16384, 64, 6400, ctypes.byref(fso), ctypes.byref(fco)) == 0
     assert (fso.value, fco.value) == (500000.0, 50.250e6)
+
+
+def test_nearest_freq_is_accepted_by_add_filter():
+    """Fcfb::nearest_freq (fcfb.rs:108-115): the rounded (fs, fc) round-trips
+    through freq_to_bins_exact, so add_filter accepts it."""
+    from spektri_amd import dsp
+    fs, N = 131.072e6, 65536
+    rng = np.random.default_rng(3)
+    for _ in range(100):
+        fso, fco = float(rng.uniform(8e3, 2e6)), float(rng.uniform(-60e6, 60e6))
+        nf = dsp.nearest_freq(fs, 0.0, N, fso, fco)
+        assert nf is not None
+        bn = dsp.freq_to_bins_exact(fs, 0.0, N, *nf)
+        assert bn is not None and bn.bins % 4 == 0
+        assert abs(nf[0] - fso) <= 2 * fs / N + 1e-6 and abs(nf[1] - fco) <= 2 * fs / N + 1e-6
+    assert dsp.nearest_freq(fs, 0.0, N, 1.0, 0.0) is None  # rounds to 0 bins
