@@ -180,6 +180,16 @@ spk_status spk_spectrum_quantized(spk_ctx* ctx, void* out, size_t cap, uint32_t*
 spk_status spk_spectrum_quantized_device(spk_ctx* ctx, const void** dptr, uint32_t* nrec,
                                          uint32_t* bytes_per_bin);
 
+/* The same quantised records as the reference sends them
+ * (spectrum.rs:120-153): each is 24 bytes of metadata (data.rs:110-132:
+ * u64 seq = first_seq + record index, u64 secs, u32 nanos, u32 0, all
+ * little-endian) followed by the body, packed on the device and copied out
+ * in one transfer, so the host only writes or sends them.  *nbytes
+ * receives the packed size (also when cap is too small); cap counts bytes.
+ * No counterpart in the reference, whose host serialises each record. */
+spk_status spk_spectrum_records_packed(spk_ctx* ctx, uint64_t first_seq, uint64_t secs, uint32_t nanos, void* out,
+                                       size_t cap, uint32_t* nrec, size_t* nbytes);
+
 /* Forward spectra of the last call: nblocks rows of K complex f32 with a
  * row stride of *stride complex values (the reference's fft_result_buf).
  * Valid only for contexts created with SPK_FLAG_KEEP_SPECTRA. */

@@ -400,7 +400,6 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   using C = FwdRealCfg<M, R, R1, R2, FL>;
   using FC = FusedCfg<FL>;
   constexpr int T = C::T;
-  constexpr int W = C::W;
   constexpr int P3 = C::P3;
   constexpr bool SPLIT = C::SPLIT;
   __shared__ __align__(16) float lds[C::LDS_FLOATS + C::STASH_FLOATS + C::DUMMY_FLOATS + C::EDGE_PER_THREAD * T + 1];
@@ -1149,6 +1148,41 @@ __global__ void adjoint_reduce_kernel(AdjReduceParams P) {
 
 // Copy channel outputs from the capacity layout (out_off) to the packed
 // layout (nblocks * 3 w_off / 4).  Grid y = channel.
+// Welch records as the reference sends them (spectrum.rs:120-153): per
+// record 24 bytes of metadata (data.rs:110-132: u64 seq, u64 seconds, u32
+// nanoseconds, u32 reserved, little-endian) followed by the quantised body.
+// One thread per output byte.
+struct RecordPackParams {
+  const unsigned char* body;  // nrec bodies of body_bytes each
+  unsigned char* out;         // nrec records of 24 + body_bytes
+  long long body_bytes;
+  int nrec;
+  unsigned long long first_seq, secs;
+  unsigned nanos;
+};
+
+__global__ void pack_records_kernel(RecordPackParams P) {
+  const long long rec = 24 + P.body_bytes;
+  const long long total = rec * P.nrec;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long r = i / rec, o = i % rec;
+    unsigned char b;
+    if (o >= 24) {
+      b = P.body[r * P.body_bytes + (o - 24)];
+    } else if (o < 8) {
+      b = (unsigned char)((P.first_seq + (unsigned long long)r) >> (8 * o));
+    } else if (o < 16) {
+      b = (unsigned char)(P.secs >> (8 * (o - 8)));
+    } else if (o < 20) {
+      b = (unsigned char)(P.nanos >> (8 * (o - 16)));
+    } else {
+      b = 0;
+    }
+    P.out[i] = b;
+  }
+}
+
 struct PackParams {
   const cpx* src;
   const ChanDesc* chans;

@@ -246,6 +246,7 @@ struct spk_ctx {
   DevBuf pos;                // device stream position (blocks consumed), two long long slots
   int parity = 0;            // slot holding this call's position; the Welch kernel writes the other
   DevBuf qout;               // quantised Welch records (SPK_FLAG_SPECTRUM_U8 / _U16)
+  DevBuf qpack;              // the same records with their metadata, packed (spk_spectrum_records_packed)
   int qfmt = 0;              // 0 none, 1 U8, 2 U16
   bool use_graphs = false;
   std::map<std::tuple<const void*, int, int, int>, hipGraphExec_t> graphs;
@@ -1444,6 +1445,39 @@ spk_status spk_spectrum_quantized(spk_ctx* c, void* out, size_t cap, uint32_t* n
     HIP_TRY(c, hipStreamSynchronize(c->stream));
   }
   *nrec = n;
+  return SPK_OK;
+}
+
+spk_status spk_spectrum_records_packed(spk_ctx* c, uint64_t first_seq, uint64_t secs, uint32_t nanos, void* out,
+                                       size_t cap, uint32_t* nrec, size_t* nbytes) {
+  if (!c || !nrec || !nbytes) return SPK_ERR_INVALID;
+  if (!c->qfmt) return fail(c, SPK_ERR_STATE, "context created without SPK_FLAG_SPECTRUM_U8 / _U16");
+  {
+    spk_status st = flush_records(c);
+    if (st != SPK_OK) return st;
+  }
+  const uint32_t n = c->have_output ? c->last_records : 0;
+  const long long body = (long long)c->K * c->qfmt;
+  const size_t need = (size_t)n * (size_t)(24 + body);
+  *nrec = n;
+  *nbytes = need;
+  if (need > cap) return fail(c, SPK_ERR_CAPACITY, "output buffer too small");
+  if (!need) return SPK_OK;
+  if (!out) return SPK_ERR_INVALID;
+  if (c->qpack.bytes < need) HIP_TRY(c, c->qpack.alloc((size_t)c->welch_out_cap * (size_t)(24 + body)));
+  spk::RecordPackParams P{};
+  P.body = c->qout.as<unsigned char>();
+  P.out = c->qpack.as<unsigned char>();
+  P.body_bytes = body;
+  P.nrec = (int)n;
+  P.first_seq = first_seq;
+  P.secs = secs;
+  P.nanos = nanos;
+  const unsigned grid = (unsigned)std::min<long long>(((long long)need + 255) / 256, (long long)c->ncu * 8);
+  hipLaunchKernelGGL(spk::pack_records_kernel, dim3(grid), dim3(256), 0, c->stream, P);
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, hipMemcpyAsync(out, c->qpack.p, need, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
   return SPK_OK;
 }
 

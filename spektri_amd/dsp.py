@@ -311,6 +311,22 @@ class DspState:
                                               ctypes.byref(got)), self._ctx)
         return out
 
+    def spectrum_records_packed(self, first_seq: int, secs: int, nanos: int) -> bytes:
+        """The last call's quantised records with their 24-byte metadata, as
+        the reference sends them (spectrum.rs:120-153), packed on the GPU."""
+        lib = _lib.load()
+        n, nb = ctypes.c_uint32(), ctypes.c_size_t()
+        st = lib.spk_spectrum_records_packed(self._ctx, first_seq, secs, nanos, None, 0, ctypes.byref(n),
+                                             ctypes.byref(nb))
+        if nb.value == 0:
+            _lib.check(st, self._ctx)
+            return b""
+        buf = np.empty(nb.value, dtype=np.uint8)
+        _lib.check(lib.spk_spectrum_records_packed(self._ctx, first_seq, secs, nanos,
+                                                   buf.ctypes.data_as(ctypes.c_void_p), buf.nbytes,
+                                                   ctypes.byref(n), ctypes.byref(nb)), self._ctx)
+        return buf.tobytes()
+
     def spectra_device(self):
         """(device pointer, row stride in complex values) of the last call's spectra."""
         p, stride = ctypes.c_void_p(), ctypes.c_uint64()

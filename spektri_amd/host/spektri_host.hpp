@@ -202,12 +202,13 @@ class Output {
   ~Output() {
     if (file_) fclose(file_);
   }
-  void write(const std::vector<uint8_t>& buf) {
+  void write(const uint8_t* data, size_t n) {
     if (file_) {
-      fwrite(buf.data(), 1, buf.size(), file_);
+      fwrite(data, 1, n, file_);
       fflush(file_);
     }
   }
+  void write(const std::vector<uint8_t>& buf) { write(buf.data(), buf.size()); }
   const std::vector<uint8_t>& topic() const { return topic_; }
 
  private:
@@ -298,20 +299,18 @@ class DspState {
       rec.insert(rec.end(), p, p + n * sizeof(float));  // f32 re, im pairs, little-endian host
       f.output.write(rec);
     }
+    // Spectrum records arrive packed with their metadata from the GPU.
     uint32_t nrec = 0;
-    const void* qdev = nullptr;
-    uint32_t bpb = 0;
-    check(spk_spectrum_quantized_device(ctx_, &qdev, &nrec, &bpb));
-    const size_t K = params_.complex ? params_.fft_size : params_.fft_size / 2 + 1;
-    spec_.resize((size_t)nrec * K * bpb);
-    if (nrec) check(spk_spectrum_quantized(ctx_, spec_.data(), spec_.size(), &nrec));
-    for (uint32_t r = 0; r < nrec; ++r) {
-      rec.clear();
-      serialize_metadata(rec, m, spectrum_seq_++);
-      rec.insert(rec.end(), spec_.begin() + (ptrdiff_t)((size_t)r * K * bpb),
-                 spec_.begin() + (ptrdiff_t)((size_t)(r + 1) * K * bpb));
-      spectrum_out_->write(rec);
-    }
+    size_t nbytes = 0;
+    const uint64_t secs = (uint64_t)m.systemtime.tv_sec;
+    const uint32_t nanos = (uint32_t)m.systemtime.tv_nsec;
+    check_or_capacity(spk_spectrum_records_packed(ctx_, spectrum_seq_, secs, nanos, nullptr, 0, &nrec, &nbytes));
+    if (!nrec) return;
+    spec_.resize(nbytes);
+    check(spk_spectrum_records_packed(ctx_, spectrum_seq_, secs, nanos, spec_.data(), spec_.size(), &nrec, &nbytes));
+    spectrum_seq_ += nrec;
+    const size_t each = nbytes / nrec;
+    for (uint32_t r = 0; r < nrec; ++r) spectrum_out_->write(spec_.data() + (size_t)r * each, each);
   }
 
   size_t filter_count() const { return filters_.size(); }
@@ -328,6 +327,10 @@ class DspState {
 
   void check(spk_status st) {
     if (st != SPK_OK) throw std::runtime_error(std::string("spektri GPU call failed: ") + spk_last_error(ctx_));
+  }
+  // A size query (cap 0) reports SPK_ERR_CAPACITY when there is anything to copy.
+  void check_or_capacity(spk_status st) {
+    if (st != SPK_ERR_CAPACITY) check(st);
   }
 
   spk_ctx* ctx_ = nullptr;

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from oracle import spektri_oracle as O
+from spektri_amd import wire
 from spektri_amd import workloads as W
 from spektri_amd.dsp import DspParams, DspState
 
@@ -547,4 +548,32 @@ def test_quantized_spectrum_records(fmt, complex_in):
     report(f"quantized_{fmt}_{'complex' if complex_in else 'real'}", records=len(q), max_lsb=int(d.max()),
            frac_exact=float(np.mean(d == 0)))
     assert d.max() <= 1 and np.mean(d == 0) > 0.99
+    st.close()
+
+
+@pytest.mark.parametrize("fmt", ["U8", "U16"])
+def test_spectrum_records_packed(fmt):
+    """Records packed on the GPU with their metadata (spectrum.rs:120-153,
+    data.rs:110-132) equal, byte for byte, the host serialisation of the
+    same quantised bodies; the sequence numbers continue across calls."""
+    rng = np.random.default_rng(43)
+    N, F, A = 8192, 4, 3
+    H = N - N // 4
+    st, _ = DspState.init(DspParams(complex=False, fs_in=1.0, fft_size=N, ffts_per_buf=F, spectrum_averages=A,
+                                    max_blocks=2 * F, quantize=True, spectrum_format=fmt))
+    seq, secs, nanos = 0xFFFFFFF0, 1_700_000_123, 987_654_321
+    total = 0
+    for call, nb in enumerate((2 * F, F)):
+        x = np.clip(np.round(rng.standard_normal(nb * H) * 300), -32768, 32767).astype(np.int16)
+        dev = torch.from_numpy(x).to("cuda")
+        st.process_device(dev.data_ptr(), len(x))
+        st.synchronize()
+        q = st.spectrum_quantized()
+        got = st.spectrum_records_packed(seq, secs + call, nanos)
+        want = b"".join(wire.spectrum_record(wire.serialize_metadata(seq + r, secs + call, nanos), q[r])
+                        for r in range(q.shape[0]))
+        assert len(q) > 0 and got == want, (call, len(got), len(want))
+        seq += len(q)
+        total += len(q)
+    report(f"records_packed_{fmt}", records=total, bytes_equal=True)
     st.close()
