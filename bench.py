@@ -80,6 +80,27 @@ def cpu_baseline(wl, x_host, seconds: float):
     return cpu_port.time_workload(wl, x_host, seconds)
 
 
+def cpu_fft_bound(wl, x_host, seconds: float = 3.0):
+    """Sanity bound for any CPU path: torch.fft.rfft (CPU, all of torch's
+    threads) of the workload's N-point blocks alone, no windowing, Welch or
+    channels.  Input MS/s counts H new samples per block, like ``value``."""
+    import numpy as np
+    import torch
+
+    N, H = wl.fft_size, wl.hop
+    nb = min(wl.nblocks, max(8, (64 << 20) // (N * 4)))
+    frames = np.lib.stride_tricks.as_strided(x_host, shape=(nb, N), strides=(H * x_host.strides[0], x_host.strides[0]))
+    xb = torch.from_numpy(frames.astype(np.float32))
+    torch.fft.rfft(xb)
+    reps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        torch.fft.rfft(xb)
+        reps += 1
+    dt = time.perf_counter() - t0
+    return {"value": round(reps * nb * H / dt / 1e6, 1), "unit": "MS/s", "threads": torch.get_num_threads(),
+            "sample": f"{reps} x {nb} blocks of {N}-point torch.fft.rfft in {dt:.2f} s; forward FFT only"}
+
+
 def host_ingest(wl, x_dev):
     """Input MS/s when the stream arrives in host memory (never ``value``):
     the same samples fed buffer by buffer through spk_process (synchronous,
@@ -261,6 +282,7 @@ def main():
         x_host = x.cpu().numpy()
         del x
         out["cpu_baseline"] = cpu_baseline(wl, x_host, args.cpu_seconds)
+        out["cpu_fft_only_bound"] = cpu_fft_bound(wl, x_host)
         if out["cpu_baseline"].get("value"):
             out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out), flush=True)
