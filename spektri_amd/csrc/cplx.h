@@ -165,7 +165,7 @@ struct WelchDirectParams {
   int qfmt;
 };
 
-// Real input with N above the forward kernel's range goes through the
+// Real input outside the forward kernel's plans goes through the
 // complex pipeline: the complex FFT Z of the M = N/2 packed pairs
 // z[n] = x[2n] + i x[2n+1] sits in the spectrum rows, and this kernel
 // untangles it in place into the real spectrum X[0..M]:
@@ -185,7 +185,7 @@ __global__ void untangle_kernel(UntangleParams P) {
   const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (k > P.M / 2) return;
   cpx* Z = P.spec + (long long)blockIdx.y * P.spec_stride;
-  const int km = (P.M - k) & (P.M - 1);  // k = 0: its own mirror
+  const int km = k == 0 ? 0 : P.M - k;  // k = 0: its own mirror
   const cpx A = Z[k], B = Z[km];
   const cpx E = {0.5f * (A.x + B.x), 0.5f * (A.y - B.y)};
   const cpx O = {0.5f * (A.y + B.y), -0.5f * (A.x - B.x)};
@@ -196,8 +196,9 @@ __global__ void untangle_kernel(UntangleParams P) {
   else if (2 * k != P.M) Z[km] = Xm;  // (k = M/2 is its own mirror)
 }
 
-// Welch records of real input from the kept spectra (spectrum.rs:95-116),
-// K = N/2 + 1 bins: one thread per four bins k0..k0+3 (two 16-byte loads
+// Welch records from the kept spectra (spectrum.rs:95-116) for real input,
+// K = N/2 + 1 bins, and for complex input of any N (K = N: get_bin wraps
+// without a mirror): one thread per four bins k0..k0+3 (two 16-byte loads
 // per block; the row stride leaves room past the Nyquist bin), the outer
 // neighbours X[k0 - 1] and X[k0 + 4] from the adjacent lanes by DPP; lanes 0
 // and 63 of a wave, and the quads at either end of the spectrum, load them
@@ -222,7 +223,7 @@ __global__ void welch_direct_real_kernel(WelchDirectParams F) {
   const bool own = k0 < F.K;  // quads past the spectrum compute (DPP partners) but store nothing
   const int kq = own ? k0 : 0;
   auto bin = [&](const cpx* X, int i) {
-    const int m = i & (F.N - 1);
+    const int m = i < 0 ? i + F.N : (i >= F.N ? i - F.N : i);  // -1 <= i < N + 4
     if (m >= F.K) {
       const cpx v = X[F.N - m];
       return cpx{v.x, -v.y};

@@ -24,6 +24,7 @@
 #include "cplx.h"
 #include "convert.h"
 #include "chan_generic.h"
+#include "gfft.h"
 
 using spk::cpx;
 
@@ -126,6 +127,32 @@ const ChanPlan* find_chan_plan(int L) {
     if (p.L == L) return &p;
   return nullptr;
 }
+// Pass radices of the mixed-radix forward transform of Nc points: eights,
+// then a four or a two, then odd primes up to 31 (the instantiated
+// radices); empty when Nc has a larger prime factor.
+std::vector<int> gfft_radices(int Nc) {
+  std::vector<int> r;
+  while (Nc % 8 == 0 && Nc > 1) {
+    r.push_back(8);
+    Nc /= 8;
+  }
+  if (Nc % 4 == 0) {
+    r.push_back(4);
+    Nc /= 4;
+  }
+  if (Nc % 2 == 0) {
+    r.push_back(2);
+    Nc /= 2;
+  }
+  for (int p : {3, 5, 7, 11, 13, 17, 19, 23, 29, 31})
+    while (Nc % p == 0) {
+      r.push_back(p);
+      Nc /= p;
+    }
+  if (Nc != 1) r.clear();
+  return r;
+}
+
 const FwdPlan* find_fwd_plan(int M) {
   for (const auto& p : kFwdPlans)
     if (p.M == M) return &p;
@@ -236,6 +263,11 @@ struct spk_ctx {
   DevBuf cY, ctw2, ctw3, cta, ctb, ctw64;
   DevBuf ctu;                // real two-step input: untangle twiddles W_N^k, k <= N/4
   const ChanPlan* cplan = nullptr;
+  // sizes without a two-step plan: mixed-radix passes (gfft.h) over the
+  // Nc-point transform, radices in pass order, twiddles W_Nc^m
+  bool gfft = false;
+  std::vector<int> gfft_radices;
+  DevBuf gtw;
   DevBuf adj_part;           // adjoint per-split weight sums
   int ncu = 0;
   // channel stage launch form and batched tuning (chunk blocks, rounds per item, workgroups per CU)
@@ -725,6 +757,67 @@ const void* enqueue_convert(spk_ctx* c, const void* raw, int nb, hipStream_t s) 
   return c->conv.p;
 }
 
+template <int R>
+void launch_gfft_pass(const spk::GfftPassParams& P, int load, dim3 grid, hipStream_t s) {
+  switch (load) {
+    case 0: hipLaunchKernelGGL((spk::gfft_pass_kernel<R, 0>), grid, dim3(spk::kGfftThreads), 0, s, P); break;
+    case 1: hipLaunchKernelGGL((spk::gfft_pass_kernel<R, 1>), grid, dim3(spk::kGfftThreads), 0, s, P); break;
+    default: hipLaunchKernelGGL((spk::gfft_pass_kernel<R, 2>), grid, dim3(spk::kGfftThreads), 0, s, P); break;
+  }
+}
+
+hipError_t dispatch_gfft_pass(int R, const spk::GfftPassParams& P, int load, dim3 grid, hipStream_t s) {
+  switch (R) {
+    case 2: launch_gfft_pass<2>(P, load, grid, s); break;
+    case 3: launch_gfft_pass<3>(P, load, grid, s); break;
+    case 4: launch_gfft_pass<4>(P, load, grid, s); break;
+    case 5: launch_gfft_pass<5>(P, load, grid, s); break;
+    case 7: launch_gfft_pass<7>(P, load, grid, s); break;
+    case 8: launch_gfft_pass<8>(P, load, grid, s); break;
+    case 11: launch_gfft_pass<11>(P, load, grid, s); break;
+    case 13: launch_gfft_pass<13>(P, load, grid, s); break;
+    case 17: launch_gfft_pass<17>(P, load, grid, s); break;
+    case 19: launch_gfft_pass<19>(P, load, grid, s); break;
+    case 23: launch_gfft_pass<23>(P, load, grid, s); break;
+    case 29: launch_gfft_pass<29>(P, load, grid, s); break;
+    case 31: launch_gfft_pass<31>(P, load, grid, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// Mixed-radix forward transform of nb blocks into the spectrum rows: the
+// first pass frames and windows the input, passes alternate between the
+// scratch rows (cY) and the spectrum rows so that the last writes the
+// spectrum rows.
+spk_status enqueue_gfft(spk_ctx* c, const void* src, int nb, hipStream_t s) {
+  const int pk = c->complex_in ? 1 : 2;
+  const int np = (int)c->gfft_radices.size();
+  spk::GfftPassParams P{};
+  P.src = src;
+  P.hist = c->hist.p;
+  P.Hc = c->H / pk;
+  P.Oc = c->O / pk;
+  P.wscale = c->wscale;
+  P.tw = c->gtw.as<cpx>();
+  P.Nc = c->Nc;
+  P.nblocks = nb;
+  int Ns = 1;
+  for (int i = 0; i < np; ++i) {
+    const int R = c->gfft_radices[i];
+    const bool to_spec = ((np - 1 - i) % 2) == 0;
+    P.out = to_spec ? c->spec.as<cpx>() : c->cY.as<cpx>();
+    P.out_stride = to_spec ? c->spec_stride : c->Nc;
+    P.Ns = Ns;
+    const dim3 grid((unsigned)((c->Nc / R + spk::kGfftThreads - 1) / spk::kGfftThreads), (unsigned)nb);
+    HIP_TRY(c, dispatch_gfft_pass(R, P, i == 0 ? (c->s16_pairs ? 1 : 2) : 0, grid, s));
+    P.in = P.out;
+    P.in_stride = P.out_stride;
+    Ns *= R;
+  }
+  return SPK_OK;
+}
+
 // Complex input: row transforms, column transforms (natural-order spectra),
 // Welch records, channel stage, stream-position advance.
 spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, hipEvent_t* ev) {
@@ -732,6 +825,10 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
   const int pk = c->complex_in ? 1 : 2;  // real samples per complex sample of the pipeline
   if (ev) HIP_TRY(c, hipEventRecord(ev[0], s));
   src = enqueue_convert(c, src, nb, s);
+  if (c->gfft) {
+    spk_status st = enqueue_gfft(c, src, nb, s);
+    if (st != SPK_OK) return st;
+  } else {
   spk::CplxRowParams P{};
   P.src = src;
   P.hist = c->hist.p;
@@ -756,6 +853,7 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
   hipLaunchKernelGGL(spk::cplx_col_kernel, dim3((unsigned)(((long long)nb * N1 + 63) / 64)), dim3(spk::kChanThreads),
                      0, s, Q);
   HIP_TRY(c, hipGetLastError());
+  }
   if (!c->complex_in) {
     spk::UntangleParams U{};
     U.spec = c->spec.as<cpx>();
@@ -793,9 +891,9 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
     W.qout = c->qout.as<unsigned char>();
     W.qfmt = c->qfmt;
     const int gmax = (c->A - 1 + nb - 1) / c->A + 1;
-    if (c->complex_in)  // four bins per thread; K = N is a multiple of 1024
+    if (c->complex_in && !c->gfft)  // four bins per thread; K = N is a multiple of 1024
       hipLaunchKernelGGL(spk::welch_direct_kernel, dim3(c->K / 1024, gmax), dim3(256), 0, s, W);
-    else
+    else  // any K (complex input: K = N, get_bin wraps without a mirror)
       hipLaunchKernelGGL(spk::welch_direct_real_kernel, dim3((c->K + 1023) / 1024, gmax), dim3(256), 0, s, W);
     HIP_TRY(c, hipGetLastError());
   }
@@ -1033,14 +1131,22 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
     return SPK_ERR_INVALID;
   }
   const bool cplx = fi->complex_in;
+  const bool pow2 = (N & (N - 1)) == 0;
   // real N beyond the forward kernel's plans: the complex pipeline on N/2 pairs
-  const bool big_real = !cplx && find_fwd_plan((int)N / 2) == nullptr && N > 65536;
-  // sizes the reference accepts (rustfft plans any N) but this library does
-  // not implement: N not a power of two, or outside the planned ranges
-  if ((N & (N - 1)) != 0 ||
-      !(cplx ? (N >= 1024 && N <= 262144) : (big_real ? N <= 524288 : find_fwd_plan((int)N / 2) != nullptr))) {
-    delete c;
-    return SPK_ERR_UNSUPPORTED;
+  const bool big_real = !cplx && pow2 && find_fwd_plan((int)N / 2) == nullptr && N > 65536 && N <= 524288;
+  // every other size the reference accepts (rustfft plans any N) goes
+  // through the mixed-radix passes: N a multiple of 8 (whole sample pairs
+  // in the hop and the overlap), at most 2^20, and the transform length
+  // without a prime factor above 31
+  const bool planned = pow2 && (cplx ? (N >= 1024 && N <= 262144) : (big_real || find_fwd_plan((int)N / 2) != nullptr));
+  if (!planned) {
+    const int Nc = cplx ? (int)N : (int)N / 2;
+    if (N % 8 != 0 || N < 16 || N > (1u << 20) || gfft_radices(Nc).empty()) {
+      delete c;
+      return SPK_ERR_UNSUPPORTED;
+    }
+    c->gfft = true;
+    c->gfft_radices = gfft_radices(Nc);
   }
   c->N = (int)N;
   c->M = c->N / 2;
@@ -1054,11 +1160,11 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   c->raw_bps = fi->raw_bps;
   c->conv_kind = fi->conv;
   c->f32_in = !cplx && fi->canon_bps == 4;
-  c->two_step = cplx || big_real;
+  c->two_step = cplx || big_real || c->gfft;
   c->Nc = cplx ? c->N : c->N / 2;
   // the row transforms convert s16 pairs as they load them
   if (cfg->input_format == SPK_FMT_CS16LE) c->s16_pairs = true;
-  if (big_real) {
+  if (big_real || (c->gfft && !cplx)) {
     // the row transforms read pairs of samples as complex samples: s16 pairs
     // for s16le, f32 pairs for every other format
     c->bps = 4;
@@ -1081,7 +1187,7 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   c->max_blocks = (int)(cfg->max_blocks ? cfg->max_blocks : cfg->ffts_per_buf);
   if (c->max_blocks < c->F) c->max_blocks = c->F;
   c->fplan = c->two_step ? nullptr : find_fwd_plan(c->M);
-  c->cplan = c->two_step ? find_chan_plan(c->Nc / 64) : nullptr;
+  c->cplan = c->two_step && !c->gfft ? find_chan_plan(c->Nc / 64) : nullptr;
 
   auto cleanup = [&](spk_status s) {
     spk_destroy(c);
@@ -1113,8 +1219,8 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   if (c->two_step) {
     // every spectrum row is kept: the channel stage always reads them
     if ((long long)(c->max_blocks + c->A) * 64 >= (1LL << 31)) return cleanup(SPK_ERR_UNSUPPORTED);
-    const ChanPlan& p = *c->cplan;
-    const int N1 = c->Nc / 64;
+    // the untangle and mixed-radix launches take one grid row per block
+    if ((!c->complex_in || c->gfft) && c->max_blocks > 65535) return cleanup(SPK_ERR_UNSUPPORTED);
     if (c->hist.alloc((size_t)c->O * c->bps) != hipSuccess ||
         c->staging.alloc((size_t)c->F * c->H * c->raw_bps) != hipSuccess ||
         (c->conv_kind && c->conv.alloc((size_t)c->max_blocks * c->H * c->bps) != hipSuccess) ||
@@ -1131,15 +1237,25 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
       const double a = 2.0 * M_PI * (double)(m % n) / (double)n;
       return cpx{(float)std::cos(a), (float)(-std::sin(a))};
     };
-    std::vector<cpx> ta(64 * 64), tb((size_t)64 * std::max(1, N1 / 64)), t64(64);
-    for (int n2 = 0; n2 < 64; ++n2) {
-      for (int a = 0; a < 64; ++a) ta[(size_t)n2 * 64 + a] = wn((long long)n2 * a, c->Nc);
-      for (int b = 0; b < std::max(1, N1 / 64); ++b) tb[(size_t)n2 * std::max(1, N1 / 64) + b] = wn(64LL * n2 * b, c->Nc);
-    }
     if (!c->complex_in) {
       std::vector<cpx> tu((size_t)c->M / 2 + 1);
       for (size_t k = 0; k < tu.size(); ++k) tu[k] = wn((long long)k, c->N);
       if (upload(c->ctu, tu) != hipSuccess) return cleanup(SPK_ERR_DEVICE);
+    }
+    if (c->gfft) {
+      std::vector<cpx> g((size_t)c->Nc);
+      for (int m = 0; m < c->Nc; ++m) g[(size_t)m] = wn(m, c->Nc);
+      if (upload(c->gtw, g) != hipSuccess) return cleanup(SPK_ERR_DEVICE);
+      if (hipDeviceSynchronize() != hipSuccess) return cleanup(SPK_ERR_DEVICE);
+      *out = c;
+      return SPK_OK;
+    }
+    const ChanPlan& p = *c->cplan;
+    const int N1 = c->Nc / 64;
+    std::vector<cpx> ta(64 * 64), tb((size_t)64 * std::max(1, N1 / 64)), t64(64);
+    for (int n2 = 0; n2 < 64; ++n2) {
+      for (int a = 0; a < 64; ++a) ta[(size_t)n2 * 64 + a] = wn((long long)n2 * a, c->Nc);
+      for (int b = 0; b < std::max(1, N1 / 64); ++b) tb[(size_t)n2 * std::max(1, N1 / 64) + b] = wn(64LL * n2 * b, c->Nc);
     }
     for (int m = 0; m < 64; ++m) t64[m] = wn(m, 64);
     if (upload(c->ctw2, pass_table(p.R2, p.R1, -1)) != hipSuccess ||

@@ -35,14 +35,15 @@ def test_null_and_invalid_arguments_return_codes():
     out = ctypes.c_void_p()
     cfg = _lib.SpkConfig(0, _lib.FMT_S16LE, 0.0, 8, 16, 0)  # no FFT size
     assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 1
-    cfg = _lib.SpkConfig(1000, _lib.FMT_S16LE, 0.0, 8, 16, 0)  # valid for rustfft, not planned here
-    assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 2
+    # sizes rustfft plans but this library does not: N not a multiple of 8,
+    # a transform length with a prime factor above 31 (real N = 296: 148 =
+    # 4 * 37 pairs; complex N = 8 * 37), N above 2^20
+    for N, fmt in [(1004, _lib.FMT_S16LE), (1000 + 4, _lib.FMT_CF32LE), (296, _lib.FMT_S16LE),
+                   (8 * 37, _lib.FMT_CF32LE), (1 << 21, _lib.FMT_U8), (1 << 21, _lib.FMT_CF32LE)]:
+        cfg = _lib.SpkConfig(N, fmt, 0.0, 8, 16, 0)
+        assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 2, (N, fmt)
     cfg = _lib.SpkConfig(8192, 0x99, 0.0, 8, 16, 0)  # not a DataFormat code
     assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 1
-    cfg = _lib.SpkConfig(1 << 20, _lib.FMT_U8, 0.0, 8, 16, 0)  # real N above 2^19: valid for the reference
-    assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 2
-    cfg = _lib.SpkConfig(1 << 20, _lib.FMT_CF32LE, 0.0, 8, 16, 0)  # complex N above 2^18
-    assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 2
 
 
 @pytest.mark.parametrize("fs,N", [(131.072e6, 65536), (128e6, 16384), (8.192e6, 8192)])
