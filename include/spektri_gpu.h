@@ -194,6 +194,18 @@ spk_status spk_spectrum_quantized_device(spk_ctx* ctx, const void** dptr, uint32
 spk_status spk_spectrum_records_packed(spk_ctx* ctx, uint64_t first_seq, uint64_t secs, uint32_t nanos, void* out,
                                        size_t cap, uint32_t* nrec, size_t* nbytes);
 
+/* Every live channel's record of the last call, as the reference writes
+ * them (fcfb.rs:75-89): in ascending id order, 24 bytes of metadata
+ * (data.rs:110-132: u64 seq, u64 secs, u32 nanos, u32 0, little-endian,
+ * the same for every channel of a buffer) followed by the channel's
+ * nblocks * 6 (L/8) samples as f32 re, im (little-endian), so record i is
+ * 24 + 8 * nblocks * 6 (L_i / 8) bytes.  Packed on the device and copied
+ * out in one transfer.  *nbytes receives the packed size (also when cap is
+ * too small).  No counterpart in the reference, whose host serialises each
+ * record. */
+spk_status spk_channel_records_packed(spk_ctx* ctx, uint64_t seq, uint64_t secs, uint32_t nanos, void* out,
+                                      size_t cap, uint32_t* nrec, size_t* nbytes);
+
 /* Forward spectra of the last call: nblocks rows of K complex f32 with a
  * row stride of *stride complex values (the reference's fft_result_buf).
  * Valid only for contexts created with SPK_FLAG_KEEP_SPECTRA. */

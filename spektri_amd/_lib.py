@@ -64,6 +64,9 @@ _SIGS = {
     "spk_set_channel_weights": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
     "spk_process": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "spk_process_async": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "spk_channel_records_packed": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, _P,
+                                                  ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32),
+                                                  ctypes.POINTER(ctypes.c_size_t)]),
     "spk_spectrum_records_packed": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, _P,
                                                    ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint32),
                                                    ctypes.POINTER(ctypes.c_size_t)]),

@@ -1148,10 +1148,19 @@ __global__ void adjoint_reduce_kernel(AdjReduceParams P) {
 
 // Copy channel outputs from the capacity layout (out_off) to the packed
 // layout (nblocks * 3 w_off / 4).  Grid y = channel.
+// Byte o < 24 of a record's metadata (data.rs:110-132: u64 seq, u64
+// seconds, u32 nanoseconds, u32 reserved, little-endian).
+__device__ __forceinline__ unsigned char metadata_byte(int o, unsigned long long seq, unsigned long long secs,
+                                                       unsigned nanos) {
+  if (o < 8) return (unsigned char)(seq >> (8 * o));
+  if (o < 16) return (unsigned char)(secs >> (8 * (o - 8)));
+  if (o < 20) return (unsigned char)(nanos >> (8 * (o - 16)));
+  return 0;
+}
+
 // Welch records as the reference sends them (spectrum.rs:120-153): per
-// record 24 bytes of metadata (data.rs:110-132: u64 seq, u64 seconds, u32
-// nanoseconds, u32 reserved, little-endian) followed by the quantised body.
-// One thread per output byte.
+// record 24 bytes of metadata followed by the quantised body.  One thread
+// per output byte.
 struct RecordPackParams {
   const unsigned char* body;  // nrec bodies of body_bytes each
   unsigned char* out;         // nrec records of 24 + body_bytes
@@ -1167,19 +1176,41 @@ __global__ void pack_records_kernel(RecordPackParams P) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (long long)gridDim.x * blockDim.x) {
     const long long r = i / rec, o = i % rec;
-    unsigned char b;
-    if (o >= 24) {
-      b = P.body[r * P.body_bytes + (o - 24)];
-    } else if (o < 8) {
-      b = (unsigned char)((P.first_seq + (unsigned long long)r) >> (8 * o));
-    } else if (o < 16) {
-      b = (unsigned char)(P.secs >> (8 * (o - 8)));
-    } else if (o < 20) {
-      b = (unsigned char)(P.nanos >> (8 * (o - 16)));
-    } else {
-      b = 0;
-    }
-    P.out[i] = b;
+    P.out[i] = o >= 24 ? P.body[r * P.body_bytes + (o - 24)]
+                       : metadata_byte((int)o, P.first_seq + (unsigned long long)r, P.secs, P.nanos);
+  }
+}
+
+// Channel records as the reference writes them (fcfb.rs:75-89): per live
+// channel, in id order, 24 bytes of metadata (the buffer's seq) and the
+// call's samples (block-major, 6 (L/8) per block, f32 re, im).  Record r
+// starts at byte 24 r + 8 nblocks prefix_r.  Grid: x over samples, y over
+// records (strided).
+struct ChanRecDesc {
+  long long src;     // first sample of the channel's output region
+  long long prefix;  // samples per block of the channels before it
+  int len;           // samples per block, 6 (L/8)
+  int pad;
+};
+struct ChanRecParams {
+  const cpx* src;
+  const ChanRecDesc* recs;
+  int nrec, nblocks;
+  unsigned char* out;
+  unsigned long long seq, secs;
+  unsigned nanos;
+};
+
+__global__ void pack_channel_records_kernel(ChanRecParams P) {
+  for (int r = (int)blockIdx.y; r < P.nrec; r += (int)gridDim.y) {
+    const ChanRecDesc d = P.recs[r];
+    unsigned char* o = P.out + 24LL * r + 8LL * P.nblocks * d.prefix;
+    if (blockIdx.x == 0 && threadIdx.x < 24) o[threadIdx.x] = metadata_byte((int)threadIdx.x, P.seq, P.secs, P.nanos);
+    cpx* body = reinterpret_cast<cpx*>(o + 24);  // 8-byte aligned: 24 r + 8 k
+    const cpx* s = P.src + d.src;
+    const long long n = (long long)P.nblocks * d.len;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+      body[i] = s[i];
   }
 }
 

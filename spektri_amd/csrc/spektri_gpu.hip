@@ -279,6 +279,10 @@ struct spk_ctx {
   int parity = 0;            // slot holding this call's position; the Welch kernel writes the other
   DevBuf qout;               // quantised Welch records (SPK_FLAG_SPECTRUM_U8 / _U16)
   DevBuf qpack;              // the same records with their metadata, packed (spk_spectrum_records_packed)
+  DevBuf crecs;              // spk::ChanRecDesc per live channel, id order
+  DevBuf cpack;              // channel records packed (spk_channel_records_packed)
+  int live_chans = 0;
+  long long live_len = 0;    // samples per block over the live channels
   int qfmt = 0;              // 0 none, 1 U8, 2 U16
   bool use_graphs = false;
   std::map<std::tuple<const void*, int, int, int>, hipGraphExec_t> graphs;
@@ -368,6 +372,19 @@ spk_status rebuild_channels(spk_ctx* c) {
     wall.insert(wall.end(), ch.w.begin(), ch.w.end());
     c->chan_out_off[i] = off;
     off += (long long)c->max_blocks * chan_out_len(ch.L);
+  }
+  {
+    std::vector<spk::ChanRecDesc> recs;
+    long long prefix = 0;
+    for (size_t i = 0; i < c->chans.size(); ++i) {
+      if (c->chans[i].removed) continue;
+      const int len = chan_out_len(c->chans[i].L);
+      recs.push_back(spk::ChanRecDesc{c->chan_out_off[i], prefix, len, 0});
+      prefix += len;
+    }
+    HIP_TRY(c, upload(c->crecs, recs));
+    c->live_chans = (int)recs.size();
+    c->live_len = prefix;
   }
   HIP_TRY(c, upload(c->weights, wall));
   HIP_TRY(c, c->chan_out.alloc((size_t)off * sizeof(cpx)));
@@ -1593,6 +1610,39 @@ spk_status spk_spectrum_records_packed(spk_ctx* c, uint64_t first_seq, uint64_t 
   hipLaunchKernelGGL(spk::pack_records_kernel, dim3(grid), dim3(256), 0, c->stream, P);
   HIP_TRY(c, hipGetLastError());
   HIP_TRY(c, hipMemcpyAsync(out, c->qpack.p, need, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return SPK_OK;
+}
+
+spk_status spk_channel_records_packed(spk_ctx* c, uint64_t seq, uint64_t secs, uint32_t nanos, void* out, size_t cap,
+                                      uint32_t* nrec, size_t* nbytes) {
+  if (!c || !nrec || !nbytes) return SPK_ERR_INVALID;
+  if (!c->have_output || c->chans_dirty) return fail(c, SPK_ERR_STATE, "no output yet");
+  const int nb = (int)c->last_blocks;
+  const uint32_t n = (uint32_t)c->live_chans;
+  const size_t need = (size_t)n * 24 + (size_t)8 * nb * (size_t)c->live_len;
+  *nrec = n;
+  *nbytes = need;
+  if (need > cap) return fail(c, SPK_ERR_CAPACITY, "output buffer too small");
+  if (!n) return SPK_OK;
+  if (!out) return SPK_ERR_INVALID;
+  if (c->cpack.bytes < need)
+    HIP_TRY(c, c->cpack.alloc((size_t)n * 24 + (size_t)8 * c->max_blocks * (size_t)c->live_len));
+  spk::ChanRecParams P{};
+  P.src = c->chan_out.as<cpx>();
+  P.recs = c->crecs.as<spk::ChanRecDesc>();
+  P.nrec = (int)n;
+  P.nblocks = nb;
+  P.out = c->cpack.as<unsigned char>();
+  P.seq = seq;
+  P.secs = secs;
+  P.nanos = nanos;
+  const long long per_rec = (long long)nb * c->live_len / n + 1;  // mean samples per record
+  const unsigned gx = (unsigned)std::max<long long>(1, std::min<long long>((per_rec + 255) / 256, 64));
+  const unsigned gy = std::min<uint32_t>(n, 65535u);
+  hipLaunchKernelGGL(spk::pack_channel_records_kernel, dim3(gx, gy), dim3(256), 0, c->stream, P);
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, hipMemcpyAsync(out, c->cpack.p, need, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return SPK_OK;
 }

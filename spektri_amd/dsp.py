@@ -311,6 +311,26 @@ class DspState:
                                               ctypes.byref(got)), self._ctx)
         return out
 
+    def channel_records_packed(self, seq: int, secs: int, nanos: int) -> dict:
+        """{channel id: record bytes} of the last call, as the reference writes
+        them (fcfb.rs:75-89), packed on the GPU and copied out at once."""
+        lib = _lib.load()
+        n, nb = ctypes.c_uint32(), ctypes.c_size_t()
+        st = lib.spk_channel_records_packed(self._ctx, seq, secs, nanos, None, 0, ctypes.byref(n), ctypes.byref(nb))
+        if nb.value == 0:
+            _lib.check(st, self._ctx)
+            return {}
+        buf = np.empty(nb.value, dtype=np.uint8)
+        _lib.check(lib.spk_channel_records_packed(self._ctx, seq, secs, nanos, buf.ctypes.data_as(ctypes.c_void_p),
+                                                  buf.nbytes, ctypes.byref(n), ctypes.byref(nb)), self._ctx)
+        out, o, blocks = {}, 0, self.last_blocks()
+        for cid, bn in sorted(self.channels):
+            size = 24 + 8 * blocks * 6 * (bn.bins // 8)
+            out[cid] = buf[o:o + size].tobytes()
+            o += size
+        assert o == nb.value and len(out) == n.value, (o, nb.value, len(out), n.value)
+        return out
+
     def spectrum_records_packed(self, first_seq: int, secs: int, nanos: int) -> bytes:
         """The last call's quantised records with their 24-byte metadata, as
         the reference sends them (spectrum.rs:120-153), packed on the GPU."""

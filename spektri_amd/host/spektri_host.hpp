@@ -286,24 +286,31 @@ class DspState {
     if (!pending_) return;
     const Metadata m = *pending_;
     pending_.reset();
-    std::vector<uint8_t> rec;
-    for (auto& f : filters_) {
-      size_t n = 0;
-      const float* dptr = nullptr;
-      check(spk_channel_output_device(ctx_, f.id, &dptr, &n));
-      samples_.resize(n);
-      if (n) check(spk_channel_output(ctx_, f.id, samples_.data(), samples_.size(), &n));
-      rec.clear();
-      serialize_metadata(rec, m, m.seq);
-      const uint8_t* p = reinterpret_cast<const uint8_t*>(samples_.data());
-      rec.insert(rec.end(), p, p + n * sizeof(float));  // f32 re, im pairs, little-endian host
-      f.output.write(rec);
+    // Channel records arrive packed with their metadata from the GPU, one
+    // per filter in id order (filters_ is in id order: ids are handed out
+    // in sequence and only accepted filters are kept).
+    const uint64_t secs = (uint64_t)m.systemtime.tv_sec;
+    const uint32_t nanos = (uint32_t)m.systemtime.tv_nsec;
+    if (!filters_.empty()) {
+      uint32_t nch = 0;
+      size_t cbytes = 0;
+      check_or_capacity(spk_channel_records_packed(ctx_, m.seq, secs, nanos, nullptr, 0, &nch, &cbytes));
+      chan_.resize(cbytes);
+      check(spk_channel_records_packed(ctx_, m.seq, secs, nanos, chan_.data(), chan_.size(), &nch, &cbytes));
+      if (nch != filters_.size()) throw std::runtime_error("channel record count mismatch");
+      uint32_t nb = 0;
+      check(spk_last_blocks(ctx_, &nb));
+      size_t o = 0;
+      for (auto& f : filters_) {
+        const size_t size = 24 + (size_t)8 * nb * (6 * (f.bins / 8));
+        f.output.write(chan_.data() + o, size);
+        o += size;
+      }
+      if (o != cbytes) throw std::runtime_error("channel record size mismatch");
     }
     // Spectrum records arrive packed with their metadata from the GPU.
     uint32_t nrec = 0;
     size_t nbytes = 0;
-    const uint64_t secs = (uint64_t)m.systemtime.tv_sec;
-    const uint32_t nanos = (uint32_t)m.systemtime.tv_nsec;
     check_or_capacity(spk_spectrum_records_packed(ctx_, spectrum_seq_, secs, nanos, nullptr, 0, &nrec, &nbytes));
     if (!nrec) return;
     spec_.resize(nbytes);
@@ -340,7 +347,7 @@ class DspState {
   std::unique_ptr<Output> spectrum_out_;
   std::optional<Metadata> pending_;
   uint64_t spectrum_seq_ = 0;
-  std::vector<float> samples_;
+  std::vector<uint8_t> chan_;
   std::vector<uint8_t> spec_;
 };
 

@@ -1,14 +1,15 @@
 """Channels added and removed between calls of one stream (Fcfb::add_filter,
 fcfb.rs:41-64; the reference drops finished filters, fcfb.rs:96-99): the
 remaining channels continue against the oracle, a removed channel's id
-reports an error and is not reused."""
+reports an error and is not reused, and the records packed on the GPU
+hold exactly the live channels."""
 import ctypes
 
 import numpy as np
 import pytest
 
 from oracle import spektri_oracle as O
-from spektri_amd import _lib
+from spektri_amd import _lib, wire
 from spektri_amd.dsp import DspParams, DspState
 
 from .gpu_util import report, rms
@@ -39,6 +40,14 @@ def test_add_and_remove_channels_between_calls(batched):
         for cid, (first, L, w) in chans.items():
             yr = O.channel_process(X[call * F:(call + 1) * F], N, first, np.asarray(w, np.float64))
             worst = max(worst, rms(st.channel_output(cid), yr))
+        # every live channel's record packed on the GPU (fcfb.rs:75-89) equals
+        # the host serialisation of the same samples, byte for byte
+        seq, secs, nanos = 2**40 + call, 1_700_000_000 + call, 999_999_999 - call
+        packed = st.channel_records_packed(seq, secs, nanos)
+        md = wire.serialize_metadata(seq, secs, nanos)
+        assert sorted(packed) == sorted(chans)
+        for cid in chans:
+            assert packed[cid] == wire.channel_record(md, st.channel_output(cid)), (call, cid)
 
     st.process(x[:per])
     check(0)

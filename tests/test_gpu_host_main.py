@@ -30,8 +30,9 @@ def records(buf: bytes, body: int):
     return out
 
 
-def test_command_line_host_end_to_end(tmp_path):
-    N, F, A, fs = 8192, 8, 5, 8.192e6  # bin spacing 1000 Hz
+@pytest.mark.parametrize("N", [8192, 12000])  # a planned size and a mixed-radix one
+def test_command_line_host_end_to_end(tmp_path, N):
+    F, A, fs = 8, 5, N * 1000.0  # bin spacing 1000 Hz
     H = 3 * N // 4
     nbuf = 4
     rng = np.random.default_rng(77)
@@ -71,7 +72,7 @@ def test_command_line_host_end_to_end(tmp_path):
         y = np.concatenate([np.frombuffer(c[4], "<c8") for c in crec]).reshape(nb, per)
         yr = O.channel_process(X, N, first, O.raised_cosine_weights(L).astype(np.float64))
         errs[f"L{L}_rms"] = rms(y, yr)
-    report("host_cli_end_to_end", spectrum_records=len(srec), spectrum_max_lsb=int(d.max()),
+    report(f"host_cli_end_to_end_N{N}", spectrum_records=len(srec), spectrum_max_lsb=int(d.max()),
            spectrum_frac_exact=float(np.mean(d == 0)), **errs)
     assert d.max() <= 1 and np.mean(d == 0) > 0.99
     assert max(errs.values()) < 1e-5, errs
