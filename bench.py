@@ -201,6 +201,19 @@ def main():
     t1.record(stream)
     torch.cuda.synchronize()
     ms_step = t0.elapsed_time(t1) / args.steps
+    # Per-step spread: 100 more steps with an event between consecutive
+    # steps (median and extremes of the intervals; ``ms_per_step`` stays the
+    # timed region's mean).
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(101)]
+    evs[0].record(stream)
+    for i in range(100):
+        st.process_device(ptr, n)
+        evs[i + 1].record(stream)
+    st.flush()
+    torch.cuda.synchronize()
+    per_step = sorted(evs[i].elapsed_time(evs[i + 1]) for i in range(100))
+    step_stats = {"median_ms": round(per_step[50], 4), "min_ms": round(per_step[0], 4),
+                  "max_ms": round(per_step[-1], 4), "steps": 100}
     # Per-kernel breakdown: the same steps launched eagerly with HIP events
     # around each launch (events are not recorded inside graphs).
     st.enable_timing(True)
@@ -212,6 +225,7 @@ def main():
     fused = st.fused_channels()
 
     H, K, A = wl.hop, wl.bins, wl.averages
+    pow2 = (wl.fft_size & (wl.fft_size - 1)) == 0
     chan_bytes = nb * sum(L * 8 + (3 * L // 4) * 8 for _, L, _ in wl.channels)
     path_bytes = nb * wl.bytes_per_block()
     all_fused = fused == len(wl.channels)
@@ -234,7 +248,8 @@ def main():
     traffic = traffic_from_profiles(args.workload)
     dominant = "fwd" if all_fused or kern["forward_ms"] >= chan_ms else "chan"
     roof = {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBPS,
-            "kernel": (("cplx_row_kernel + cplx_col_kernel" if wl.complex else
+            "kernel": (("gfft_pass_kernel passes" + ("" if wl.complex else " + untangle_kernel") if not pow2 else
+                        "cplx_row_kernel + cplx_col_kernel" if wl.complex else
                         "cplx_row_kernel + cplx_col_kernel + untangle_kernel" if wl.fft_size > 65536 else
                         f"fwd_real_kernel<M={wl.fft_size // 2}, FL={256 if all_fused else 0}>") if dominant == "fwd"
                        else "channel stage (" + ("batched" if tune and tune["mode"] == "batched" else "per-size") + ")"),
@@ -256,6 +271,7 @@ def main():
         "path": {"algorithmic_bytes_per_step": int(path_bytes), "achieved_GBps": round(path_gbps, 1),
                  "frac": round(path_gbps / HBM_PEAK_GBPS, 4), "bytes_per_input_sample": round(b_per_sample, 3),
                  "roofline_MSps": round(roofline_msps, 1)},
+        "step_stats": step_stats,
         "kernels_ms": {k: round(v, 4) for k, v in kern.items()},
         "fused_channels": fused,
         # the library replays graphs only for calls with channel launches
@@ -275,9 +291,9 @@ def main():
         torch.cuda.synchronize()
         out["copy_probe_bytes"] = n * 4
         del a, b
-    if args.cpu_seconds > 0 and wl.complex:
+    if args.cpu_seconds > 0 and (wl.complex or not pow2):
         out["cpu_baseline"] = None
-        out["cpu_baseline_note"] = "the CPU restatement covers s16 real input only"
+        out["cpu_baseline_note"] = "the CPU restatement covers s16 real input at power-of-two N only"
     elif args.cpu_seconds > 0:
         x_host = x.cpu().numpy()
         del x

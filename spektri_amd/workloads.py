@@ -147,6 +147,21 @@ def rx888_256k(nsamples: int | None = None) -> Workload:
     return w
 
 
+def mixed_48k(nsamples: int | None = None) -> Workload:
+    """rx888_64k's signal at an FFT size that is not a power of two
+    (N = 48000, the mixed-radix passes): 2^26 samples at fs = 48 MHz
+    (1 kHz bins), F = 8 (1864 blocks), Welch over 32 blocks, 64 channels of
+    256 bins between bins 1000 and 23000."""
+    wl = rx888_64k(nsamples)
+    N = 48000
+    w = Workload("mixed_48k", wl.seed, N, 48e6, 8, 32, wl.nsamples)
+    centres = round4(np.linspace(1000, 23000, 64))
+    w.channels = [(int(c) - 128, 256, raised_cosine_transition(256, 0.2)) for c in centres]
+    w.tones = [Tone(t.freq * 48e6 / wl.fs, t.amp, t.phase) for t in wl.tones]
+    w.sweep_amp, w.noise_sigma = wl.sweep_amp, wl.noise_sigma
+    return w
+
+
 def uneven_64k(nsamples: int | None = None) -> Workload:
     """Batched uneven channel set: as rx888_64k (signal, N, Welch over 32
     blocks) at F = 4 (1364 blocks), with 512 channels of IFFT size drawn
@@ -267,7 +282,7 @@ def design_16k(nsamples: int | None = None) -> Workload:
     return wl
 
 
-WORKLOADS = {"plain_8k": plain_8k, "rx888_64k": rx888_64k, "rx888_256k": rx888_256k, "uneven_64k": uneven_64k,
+WORKLOADS = {"plain_8k": plain_8k, "rx888_64k": rx888_64k, "rx888_256k": rx888_256k, "mixed_48k": mixed_48k, "uneven_64k": uneven_64k,
              "wideband_256k": wideband_256k,
              "design_16k": design_16k}
 
