@@ -266,7 +266,7 @@ spk_status spk_fused_channels(spk_ctx* ctx, uint32_t* n);
 /* ---- Differentiable channel weights (no counterpart in the reference; the
  * reference's weights are fixed raised cosines, fcfb.rs:197-203).  Packed
  * layouts follow channel id order over the channels not removed: weights L floats per channel; channel
- * outputs and their gradients nblocks * 3L/4 complex per channel; gathered-
+ * outputs and their gradients nblocks * 6 (L/8) complex per channel; gathered-
  * bin gradients nblocks * L complex per channel (nblocks = blocks of the
  * last process call).  All pointers are device pointers; the calls are
  * asynchronous on the context's stream unless stated. */
@@ -276,9 +276,7 @@ spk_status spk_fused_channels(spk_ctx* ctx, uint32_t* n);
 spk_status spk_set_weights_device(spk_ctx* ctx, const float* weights);
 /* Copy the last call's channel outputs into a packed device array of cap floats. */
 spk_status spk_channels_output_packed(spk_ctx* ctx, float* dst, size_t cap);
-/* (The packed calls and the adjoint take power-of-two IFFT sizes only;
- * other sizes return SPK_ERR_UNSUPPORTED.)
- * Adjoint of weight-multiply, IFFT and discard (FilterDsp::process,
+/* Adjoint of weight-multiply, IFFT and discard (FilterDsp::process,
  * fcfb.rs:154-188) for the last process call: ybar holds dLoss/dRe + i
  * dLoss/dIm of every packed output sample; writes wbar (packed weights
  * layout, summed over blocks) and, when xbar is not NULL, the gradient with

@@ -9,8 +9,9 @@ with respect to the weights comes from the library's adjoint kernel, not
 from torch.
 
 Packed layouts (channel id order): weights ``[sum L]``; outputs
-``[nblocks * sum(3L/4), 2]`` (re, im) with channel id's rows starting at
-``nblocks * sum_{c < id} 3 L_c / 4``.
+``[nblocks * sum(6 (L/8)), 2]`` (re, im) with channel id's rows starting at
+``nblocks * sum_{c < id} 6 (L_c / 8)`` (6 (L/8) = 3L/4 when 8 divides L;
+any IFFT size the library accepts).
 """
 from __future__ import annotations
 
@@ -60,7 +61,7 @@ class ChannelBank:
             self.layout.append(int(L))
         self.weight_count = sum(self.layout)
         self.nblocks = nblocks
-        self.output_floats = 2 * nblocks * sum(3 * L // 4 for L in self.layout)
+        self.output_floats = 2 * nblocks * sum(6 * (L // 8) for L in self.layout)
         self._out = torch.empty(self.output_floats // 2, 2, dtype=torch.float32, device=samples.device)
 
     def _run(self, weights: torch.Tensor):
@@ -81,10 +82,10 @@ class ChannelBank:
         return _StageFn.apply(weights, self)
 
     def split(self, packed: torch.Tensor):
-        """Per-channel views [nblocks * 3L/4, 2] of a packed output tensor."""
+        """Per-channel views [nblocks * 6 (L/8), 2] of a packed output tensor."""
         out, off = [], 0
         for L in self.layout:
-            n = self.nblocks * (3 * L // 4)
+            n = self.nblocks * (6 * (L // 8))
             out.append(packed[off:off + n])
             off += n
         return out

@@ -111,4 +111,124 @@ __global__ __launch_bounds__(kGenThreads) void chan_generic_kernel(GenChanParams
   }
 }
 
+// Adjoint of the generic channel stage (the same contract as
+// chan_adjoint_kernel): for output gradient ybar (packed layout), x = DFT_L
+// (forward sign) of ybar zero-padded to L at positions L/8 .. L/8 + 6 (L/8),
+// then per bin i (j = (i + L/2) mod L): wbar[j] += Re(conj(G[j]) x[i]) over
+// blocks and xbar[j] = W[j] x[i].  Tile (channel c, split s) takes blocks
+// s, s + S, ...; thread t keeps the weight sums of bins t + T u in
+// registers (at most kGenMaxL / kGenThreads of them: T = 256 above L = 256)
+// and writes them to wpart[c][s][j] (the splits are added in order by
+// adjoint_reduce_kernel).  Dynamic LDS: tiles_per_workgroup * 2 L cpx.
+struct GenAdjParams {
+  const cpx* spec;
+  long long spec_stride;
+  int N, K, real_input;
+  const float* weights;
+  const ChanDesc* chans;
+  const long long* pk_off;
+  int nch, nblocks, S;
+  const cpx* ybar;
+  cpx* xbar;           // may be null
+  float* wpart;        // [nch][S][L]
+  const cpx* tw;       // w_L^e, e = 0..L-1 (inverse sign; conjugated here)
+  int L;
+  int nstages;
+  int radix[kGenMaxStages];
+  int tile_threads;
+};
+
+__global__ __launch_bounds__(kGenThreads) void chan_generic_adjoint_kernel(GenAdjParams P) {
+  extern __shared__ __align__(16) float gen_lds[];
+  const int T = P.tile_threads;
+  const int L = P.L;
+  const int tl = threadIdx.x / T, t = threadIdx.x % T;
+  const int tpw = kGenThreads / T;
+  cpx* a0 = reinterpret_cast<cpx*>(gen_lds) + (long long)tl * 2 * L;
+  cpx* b0 = a0 + L;
+  constexpr int U = kGenMaxL / kGenThreads;
+  float wacc[U];
+
+  long long tile = (long long)blockIdx.x * tpw + tl;
+  const bool active = tile < (long long)P.nch * P.S;
+  if (!active) tile = 0;  // computes a valid tile, stores nothing (barriers stay uniform)
+  const int c = (int)(tile / P.S), split = (int)(tile % P.S);
+  const ChanDesc d = P.chans[c];
+  const float* W = P.weights + d.w_off;
+  const int lo = L / 8, n_out = 6 * (L / 8);
+#pragma unroll
+  for (int u = 0; u < U; ++u) wacc[u] = 0.f;
+
+  const int iters = (P.nblocks + P.S - 1) / P.S;
+  for (int it = 0; it < iters; ++it) {  // uniform trip count: the stages hold barriers
+    const int g = it * P.S + split;
+    const bool act = active && g < P.nblocks;
+    const int gg = g < P.nblocks ? g : 0;
+    const cpx* yb = P.ybar + (long long)P.nblocks * P.pk_off[c] + (long long)gg * n_out;
+    cpx* a = a0;
+    cpx* b = b0;
+    __syncthreads();
+    for (int i = t; i < L; i += T) a[i] = (i >= lo && i < lo + n_out) ? yb[i - lo] : cpx{0.f, 0.f};
+    int m = 1;
+    for (int s = 0; s < P.nstages; ++s) {
+      __syncthreads();
+      const int p = P.radix[s];
+      const int Lp = L / p;
+      const int mp = m * p;
+      const int scale = L / mp;
+      for (int o = t; o < L; o += T) {
+        const int k = o % m;
+        const int r = o / m;
+        const int q = r % p;
+        const int base = (r / p) * m + k;
+        const int step = (k + q * m) * scale;
+        cpx acc = a[base];
+        int e = step;
+        for (int j = 1; j < p; ++j) {
+          const cpx v = a[base + j * Lp];
+          const cpx w = P.tw[e];  // conj(w) below: forward sign
+          acc.x += v.x * w.x + v.y * w.y;
+          acc.y += v.y * w.x - v.x * w.y;
+          e += step;
+          if (e >= L) e -= L;
+        }
+        b[o] = acc;
+      }
+      cpx* tmp = a;
+      a = b;
+      b = tmp;
+      m = mp;
+    }
+    __syncthreads();
+    const cpx* X = P.spec + (long long)gg * P.spec_stride;
+    cpx* xb = P.xbar ? P.xbar + (long long)P.nblocks * d.w_off + (long long)gg * L : nullptr;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = t + T * u;
+      if (i >= L) break;
+      int j = i + L / 2;
+      if (j >= L) j -= L;
+      int mm = d.f_mod + j;
+      if (mm >= P.N) mm -= P.N;
+      const bool mirror = P.real_input && mm >= P.K;
+      cpx G = X[mirror ? P.N - mm : mm];
+      if (mirror) G.y = -G.y;
+      if (act) {
+        wacc[u] += G.x * a[i].x + G.y * a[i].y;
+        if (xb) xb[j] = cscale(a[i], W[j]);
+      }
+    }
+  }
+  if (active) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = t + T * u;
+      if (i >= L) break;
+      int j = i + L / 2;
+      if (j >= L) j -= L;
+      P.wpart[((long long)c * P.S + split) * L + j] = wacc[u];
+    }
+  }
+}
+
 }  // namespace spk

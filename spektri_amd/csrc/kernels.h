@@ -1046,7 +1046,8 @@ struct ChanAdjParams {
   const float* weights;
   const ChanDesc* chans;
   int nch, nblocks, S;
-  const cpx* ybar;
+  const cpx* ybar;      // packed layout (pack_outputs_kernel)
+  const long long* pk_off;
   cpx* xbar;     // may be null
   float* wpart;  // [nch][S][L]
   const cpx* tw2;  // forward-sign pass tables
@@ -1079,7 +1080,7 @@ __global__ __launch_bounds__(kChanThreads) void chan_adjoint_kernel(ChanAdjParam
     const int g = (it * P.S + split) * TPW + tl;
     const bool act = g < P.nblocks;
     const int gg = act ? g : 0;
-    const cpx* yb = P.ybar + (long long)P.nblocks * (d.w_off / 4 * 3) + (long long)gg * (3 * L / 4);
+    const cpx* yb = P.ybar + (long long)P.nblocks * P.pk_off[c] + (long long)gg * (3 * L / 4);
     cpx x[R];
 #pragma unroll
     for (int e = 0; e < R; ++e) {
@@ -1146,8 +1147,6 @@ __global__ void adjoint_reduce_kernel(AdjReduceParams P) {
   P.wbar[P.chans[c].w_off + j] = sum;
 }
 
-// Copy channel outputs from the capacity layout (out_off) to the packed
-// layout (nblocks * 3 w_off / 4).  Grid y = channel.
 // Byte o < 24 of a record's metadata (data.rs:110-132: u64 seq, u64
 // seconds, u32 nanoseconds, u32 reserved, little-endian).
 __device__ __forceinline__ unsigned char metadata_byte(int o, unsigned long long seq, unsigned long long secs,
@@ -1214,17 +1213,22 @@ __global__ void pack_channel_records_kernel(ChanRecParams P) {
   }
 }
 
+// Copy channel outputs from the capacity layout (out_off) to the packed
+// layout: channel c's nblocks * 6 (L/8) samples at nblocks * pk_off[c]
+// (pk_off: samples per block of the live channels before it, id order).
+// Grid y = channel of the group.
 struct PackParams {
   const cpx* src;
   const ChanDesc* chans;
+  const long long* pk_off;
   int L, nblocks;
   cpx* dst;
 };
 __global__ void pack_outputs_kernel(PackParams P) {
   const ChanDesc d = P.chans[blockIdx.y];
-  const long long n = (long long)P.nblocks * (3 * P.L / 4);
+  const long long n = (long long)P.nblocks * (6 * (P.L / 8));
   const cpx* s = P.src + d.out_off;
-  cpx* o = P.dst + (long long)P.nblocks * (d.w_off / 4 * 3);
+  cpx* o = P.dst + (long long)P.nblocks * P.pk_off[blockIdx.y];
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     o[i] = s[i];
 }

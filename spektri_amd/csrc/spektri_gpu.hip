@@ -72,6 +72,7 @@ struct ChanGroup {  // channels sharing one IFFT size
   bool generic = false;
   std::vector<int> radix;  // stage radices, product L
   DevBuf gtw;              // w_L^e, e = 0..L-1, inverse sign
+  DevBuf pk_off;           // long long per channel: packed-layout offset (samples per block before it)
 };
 
 // Output samples per block of a channel of IFFT size L: the reference keeps
@@ -395,17 +396,31 @@ spk_status rebuild_channels(spk_ctx* c) {
     g.L = c->chans[i].L;
     g.ids.push_back((uint32_t)i);
   }
+  // packed layout (spk_channels_output_packed, the adjoint's ybar): the
+  // live channels in id order, 6 (L/8) samples per block each
+  std::vector<long long> pk_prefix(c->chans.size(), 0);
+  {
+    long long acc = 0;
+    for (size_t i = 0; i < c->chans.size(); ++i) {
+      if (c->chans[i].removed) continue;
+      pk_prefix[i] = acc;
+      acc += chan_out_len(c->chans[i].L);
+    }
+  }
   for (auto& kv : c->groups) {
     ChanGroup& g = kv.second;
     const ChanPlan* p = find_chan_plan(g.L);
     std::vector<spk::ChanDesc> d;
+    std::vector<long long> pk;
     for (uint32_t id : g.ids) {
       const Channel& ch = c->chans[id];
       long long fm = ch.first % c->N;
       if (fm < 0) fm += c->N;
       d.push_back(spk::ChanDesc{c->chan_out_off[id], (int)fm, w_off[id]});
+      pk.push_back(pk_prefix[id]);
     }
     HIP_TRY(c, upload(g.desc, d));
+    HIP_TRY(c, upload(g.pk_off, pk));
     g.generic = p == nullptr;
     if (g.generic) {
       g.radix = generic_radices(g.L);
@@ -1734,28 +1749,21 @@ spk_status spk_set_weights_device(spk_ctx* c, const float* w) {
   return SPK_OK;
 }
 
-static bool has_generic_channels(const spk_ctx* c) {
-  for (const auto& ch : c->chans)
-    if (!ch.removed && !find_chan_plan(ch.L)) return true;
-  return false;
-}
-
 spk_status spk_channels_output_packed(spk_ctx* c, float* dst, size_t cap) {
   if (!c || !dst) return SPK_ERR_INVALID;
-  if (!c->have_output) return fail(c, SPK_ERR_STATE, "no process call yet");
-  if (has_generic_channels(c))
-    return fail(c, SPK_ERR_UNSUPPORTED, "packed outputs need power-of-two IFFT sizes from 8 to 8192");
-  const size_t need = 2 * (size_t)c->last_blocks * (packed_weight_floats(c) / 4 * 3);
+  if (!c->have_output || c->chans_dirty) return fail(c, SPK_ERR_STATE, "no output yet");
+  const size_t need = 2 * (size_t)c->last_blocks * (size_t)c->live_len;
   if (cap < need) return fail(c, SPK_ERR_CAPACITY, "destination smaller than the packed outputs");
   for (auto& kv : c->groups) {
     const ChanGroup& g = kv.second;
     spk::PackParams Q{};
     Q.src = c->chan_out.as<cpx>();
     Q.chans = g.desc.as<spk::ChanDesc>();
+    Q.pk_off = g.pk_off.as<long long>();
     Q.L = g.L;
     Q.nblocks = (int)c->last_blocks;
     Q.dst = reinterpret_cast<cpx*>(dst);
-    const long long n = (long long)c->last_blocks * (3 * g.L / 4);
+    const long long n = (long long)c->last_blocks * chan_out_len(g.L);
     const unsigned gx = (unsigned)std::min<long long>((n + 255) / 256, 1024);
     hipLaunchKernelGGL(spk::pack_outputs_kernel, dim3(gx, (unsigned)g.ids.size()), dim3(256), 0, c->stream, Q);
     HIP_TRY(c, hipGetLastError());
@@ -1763,16 +1771,66 @@ spk_status spk_channels_output_packed(spk_ctx* c, float* dst, size_t cap) {
   return SPK_OK;
 }
 
+// Adjoint of one group of IFFT sizes without a power-of-two plan
+// (chan_generic_adjoint_kernel, then the split sums in order).
+spk_status generic_adjoint(spk_ctx* c, const ChanGroup& g, int nb, const float* ybar, float* wbar, float* xbar) {
+  const int nch = (int)g.ids.size();
+  int T = 32;
+  while (T < 256 && 2 * T < g.L) T *= 2;  // as launch_generic
+  const int tpw = spk::kGenThreads / T;
+  int S = (2 * c->ncu * tpw + nch - 1) / nch;  // enough tiles for every CU
+  S = std::max(1, std::min(S, nb));
+  const size_t need = (size_t)nch * S * g.L * sizeof(float);
+  if (c->adj_part.bytes < need) HIP_TRY(c, c->adj_part.alloc(need));
+  spk::GenAdjParams Q{};
+  Q.spec = c->spec.as<cpx>();
+  Q.spec_stride = c->spec_stride;
+  Q.N = c->N;
+  Q.K = c->K;
+  Q.real_input = c->complex_in ? 0 : 1;
+  Q.weights = c->weights.as<float>();
+  Q.chans = g.desc.as<spk::ChanDesc>();
+  Q.pk_off = g.pk_off.as<long long>();
+  Q.nch = nch;
+  Q.nblocks = nb;
+  Q.S = S;
+  Q.ybar = reinterpret_cast<const cpx*>(ybar);
+  Q.xbar = reinterpret_cast<cpx*>(xbar);
+  Q.wpart = c->adj_part.as<float>();
+  Q.tw = g.gtw.as<cpx>();
+  Q.L = g.L;
+  Q.nstages = (int)g.radix.size();
+  for (size_t i = 0; i < g.radix.size(); ++i) Q.radix[i] = g.radix[i];
+  Q.tile_threads = T;
+  const long long tiles = (long long)nch * S;
+  const unsigned grid = (unsigned)((tiles + tpw - 1) / tpw);
+  const size_t lds = (size_t)tpw * 2 * g.L * sizeof(cpx);
+  hipLaunchKernelGGL(spk::chan_generic_adjoint_kernel, dim3(grid), dim3(spk::kGenThreads), lds, c->stream, Q);
+  HIP_TRY(c, hipGetLastError());
+  spk::AdjReduceParams Rp{};
+  Rp.wpart = c->adj_part.as<float>();
+  Rp.chans = g.desc.as<spk::ChanDesc>();
+  Rp.L = g.L;
+  Rp.S = S;
+  Rp.wbar = wbar;
+  hipLaunchKernelGGL(spk::adjoint_reduce_kernel, dim3((g.L + 255) / 256, nch), dim3(256), 0, c->stream, Rp);
+  HIP_TRY(c, hipGetLastError());
+  return SPK_OK;
+}
+
 spk_status spk_channels_adjoint(spk_ctx* c, const float* ybar, float* wbar, float* xbar) {
   if (!c || !ybar || !wbar) return SPK_ERR_INVALID;
-  if (!c->have_output) return fail(c, SPK_ERR_STATE, "no process call yet");
+  if (!c->have_output || c->chans_dirty) return fail(c, SPK_ERR_STATE, "no output yet");
   if (!c->keep_spec) return fail(c, SPK_ERR_STATE, "the adjoint needs a context created with SPK_FLAG_KEEP_SPECTRA");
-  if (has_generic_channels(c))
-    return fail(c, SPK_ERR_UNSUPPORTED, "the adjoint needs power-of-two IFFT sizes from 8 to 8192");
   const int nb = (int)c->last_blocks;
   for (auto& kv : c->groups) {
     ChanGroup& g = kv.second;
     const int nch = (int)g.ids.size();
+    if (g.generic) {
+      spk_status st = generic_adjoint(c, g, nb, ybar, wbar, xbar);
+      if (st != SPK_OK) return st;
+      continue;
+    }
     const int T = g.L / find_chan_plan(g.L)->R;
     const int tpw = spk::kChanThreads / T;
     // splits over blocks: enough workgroups for every CU, at least one tile-row of blocks each
@@ -1792,6 +1850,7 @@ spk_status spk_channels_adjoint(spk_ctx* c, const float* ybar, float* wbar, floa
     Q.nblocks = nb;
     Q.S = S;
     Q.ybar = reinterpret_cast<const cpx*>(ybar);
+    Q.pk_off = g.pk_off.as<long long>();
     Q.xbar = reinterpret_cast<cpx*>(xbar);
     Q.wpart = c->adj_part.as<float>();
     Q.tw2 = g.atw2.as<cpx>();

@@ -222,7 +222,7 @@ class DspState:
         layouts for the last call's block count."""
         nb = self.last_blocks()
         ls = [b.bins for _, b in self.channels]
-        return sum(ls), 2 * nb * sum(3 * L // 4 for L in ls), 2 * nb * sum(ls)
+        return sum(ls), 2 * nb * sum(6 * (L // 8) for L in ls), 2 * nb * sum(ls)
 
     def set_weights_device(self, dptr: int):
         _lib.check(_lib.load().spk_set_weights_device(self._ctx, ctypes.c_void_p(dptr)), self._ctx)

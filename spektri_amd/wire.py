@@ -3,7 +3,7 @@
 Restated from spektri/src/dsp/data.rs:58-171 so that Python tooling built on
 this library emits the same bytes as the reference host: every record starts
 with 24 bytes of metadata (data.rs:110-132), channel records carry the
-F * 3L/4 complex samples as little-endian f32 pairs (fcfb.rs:75-89,177-187),
+F * 6 (L/8) complex samples as little-endian f32 pairs (fcfb.rs:75-89,177-187),
 spectrum records the quantised bins (spectrum.rs:120-152), and every output
 stream is identified by a 24-byte topic (data.rs:137-171).
 """

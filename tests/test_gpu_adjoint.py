@@ -51,13 +51,22 @@ def stage_fp64(X, N, channels, weights):
     return outs, gathered
 
 
-def test_adjoint_mixed_sizes_matches_fp64_autograd_and_fd():
+CHANNEL_SETS = {
+    "pow2": [(-20, 64), (100, 256), (1900, 256), (500, 1024), (148, 32), (2040, 8)],
+    # IFFT sizes without a power-of-two plan (L = 100 and 12: 6 (L/8) < 3L/4)
+    # between power-of-two ones, so the packed offsets mix both
+    "generic": [(-20, 48), (100, 256), (1900, 100), (500, 300), (148, 32), (2040, 12)],
+}
+
+
+@pytest.mark.parametrize("kind", sorted(CHANNEL_SETS))
+def test_adjoint_mixed_sizes_matches_fp64_autograd_and_fd(kind):
     rng = np.random.default_rng(11)
     N, F = 4096, 8
     H = N - N // 4
     x = np.clip(np.round(rng.standard_normal(2 * F * H) * 3000
                          + 8000 * np.cos(2 * np.pi * 0.1234 * np.arange(2 * F * H))), -32768, 32767).astype(np.int16)
-    channels = [(-20, 64), (100, 256), (1900, 256), (500, 1024), (148, 32), (2040, 8)]
+    channels = CHANNEL_SETS[kind]
     wts = [rng.random(L).astype(np.float32) for _, L in channels]
     st, _ = DspState.init(DspParams(complex=False, fs_in=1.0, fft_size=N, ffts_per_buf=F,
                                     spectrum_averages=4, keep_spectra=True, max_blocks=2 * F))
@@ -84,8 +93,8 @@ def test_adjoint_mixed_sizes_matches_fp64_autograd_and_fd():
     yb = torch.tensor(ybar.cpu().double().numpy())
     loss, off = 0.0, 0
     for (first, L), y in zip(channels, outs):
-        n = nb * (3 * L // 4)
-        seg = yb[2 * off:2 * (off + n)].view(nb, 3 * L // 4, 2)
+        n = nb * (6 * (L // 8))
+        seg = yb[2 * off:2 * (off + n)].view(nb, 6 * (L // 8), 2)
         loss = loss + (seg[..., 0] * y.real + seg[..., 1] * y.imag).sum()
         off += n
     loss.backward()
@@ -101,15 +110,15 @@ def test_adjoint_mixed_sizes_matches_fp64_autograd_and_fd():
             o, _ = stage_fp64(X, N, channels, wlist)
             tot, off2 = 0.0, 0
             for (first, L), y in zip(channels, o):
-                n = nb * (3 * L // 4)
-                seg = yb[2 * off2:2 * (off2 + n)].view(nb, 3 * L // 4, 2)
+                n = nb * (6 * (L // 8))
+                seg = yb[2 * off2:2 * (off2 + n)].view(nb, 6 * (L // 8), 2)
                 tot += float((seg[..., 0] * y.real + seg[..., 1] * y.imag).sum())
                 off2 += n
             return tot
     base = [torch.tensor(w, dtype=torch.float64) for w in wts]
     eps, fd_err = 1e-3, 0.0
     offs = np.cumsum([0] + [L for _, L in channels])
-    for ci, jj in [(0, 3), (1, 0), (2, 200), (3, 511), (4, 31), (5, 5)]:
+    for ci, jj in [(0, 3), (1, 0), (2, 70), (3, 211), (4, 31), (5, 5)]:
         wp = [b.clone() for b in base]
         wm = [b.clone() for b in base]
         wp[ci][jj] += eps
@@ -117,7 +126,7 @@ def test_adjoint_mixed_sizes_matches_fp64_autograd_and_fd():
         fd = (loss64(wp) - loss64(wm)) / (2 * eps)
         g = wbar_g[offs[ci] + jj]
         fd_err = max(fd_err, abs(fd - g) / max(abs(fd), 1e-3 * np.abs(wref).max()))
-    report("adjoint_mixed_sizes", blocks=nb, wbar_rel=e_w, xbar_rel=e_x, fd_rel_max=fd_err)
+    report(f"adjoint_mixed_sizes_{kind}", blocks=nb, wbar_rel=e_w, xbar_rel=e_x, fd_rel_max=fd_err)
     assert e_w < REL and e_x < REL and fd_err < REL, (e_w, e_x, fd_err)
     st.close()
 
@@ -167,7 +176,7 @@ def test_design_gradient_fd_and_adam():
             # channels without loss terms get no gradient
             return float(val.detach()), np.concatenate([w.grad.numpy() if w.grad is not None else np.zeros(w.numel())
                                                for w in ws])
-        return float(val)
+        return float(val.detach())
 
     l64, gref = f64(base, grad=True)
     e_ref = np.linalg.norm(g_gpu - gref) / np.linalg.norm(gref)
@@ -201,3 +210,27 @@ def test_design_gradient_fd_and_adam():
     assert e_ref < REL and fd_err < REL, (e_ref, fd_err)
     assert losses[-1] < 0.5 * losses[0], (losses[0], losses[-1])
     bank.close()
+
+
+def test_packed_outputs_match_per_channel_outputs():
+    """spk_channels_output_packed: live channels in id order, 6 (L/8)
+    samples per block each, equal to the per-channel outputs, with power-of-
+    two and other IFFT sizes mixed and a channel removed in between."""
+    rng = np.random.default_rng(12)
+    N, F = 4096, 4
+    H = N - N // 4
+    x = np.clip(np.round(rng.standard_normal(F * H) * 3000), -32768, 32767).astype(np.int16)
+    st, _ = DspState.init(DspParams(complex=False, fs_in=1.0, fft_size=N, ffts_per_buf=F, spectrum_averages=4))
+    ids = [st.add_channel(first, L, rng.random(L).astype(np.float32)) for first, L in CHANNEL_SETS["generic"]]
+    st.remove_channel(ids[2])
+    dx = torch.from_numpy(x).cuda()
+    st.process_device(dx.data_ptr(), len(x))
+    st.synchronize()
+    _, ny, _ = st.packed_sizes()
+    out = torch.empty(ny, dtype=torch.float32, device="cuda")
+    st.channels_output_packed(out.data_ptr(), ny)
+    st.synchronize()
+    got = out.cpu().numpy().view(np.complex64)
+    want = np.concatenate([st.channel_output(cid).reshape(-1) for cid in ids if cid != ids[2]])
+    assert got.shape == want.shape and np.array_equal(got, want)
+    st.close()
