@@ -111,6 +111,80 @@ __global__ __launch_bounds__(kGenThreads) void chan_generic_kernel(GenChanParams
   }
 }
 
+// Wide channels (kGenMaxL < L <= kGenBigMaxL, any multiple of 4, and the
+// power-of-two 16384): one (channel, block) tile per workgroup of
+// kGenBigThreads threads over one static LDS buffer of L values.  Each
+// stage computes every thread's outputs (at most kGenBigMaxL /
+// kGenBigThreads) into registers from the buffer, waits, and writes them
+// back in place; the arithmetic is chan_generic_kernel's.
+constexpr int kGenBigMaxL = 20480;  // 160 KiB of LDS
+constexpr int kGenBigThreads = 1024;
+
+__global__ __launch_bounds__(kGenBigThreads) void chan_generic_big_kernel(GenChanParams P) {
+  __shared__ cpx buf[kGenBigMaxL];
+  constexpr int U = kGenBigMaxL / kGenBigThreads;
+  const int L = P.L;
+  const int t = (int)threadIdx.x;
+  const long long tile = blockIdx.x;  // grid = nch * nblocks exactly
+  const int c = (int)(tile % P.nch);
+  const int g = (int)(tile / P.nch);
+  const ChanDesc d = P.chans[c];
+  const cpx* X = P.spec + (long long)g * P.spec_stride;
+  const float* W = P.weights + d.w_off;
+  for (int i = t; i < L; i += kGenBigThreads) {
+    int jj = i + L / 2;
+    if (jj >= L) jj -= L;
+    int m = d.f_mod + jj;
+    if (m >= P.N) m -= P.N;
+    const bool mirror = P.real_input && m >= P.K;
+    cpx v = X[mirror ? P.N - m : m];
+    if (mirror) v.y = -v.y;
+    buf[i] = cscale(v, W[jj]);
+  }
+  int m = 1;
+  for (int s = 0; s < P.nstages; ++s) {
+    __syncthreads();
+    const int p = P.radix[s];
+    const int Lp = L / p;
+    const int mp = m * p;
+    const int scale = L / mp;
+    cpx val[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int o = t + kGenBigThreads * u;
+      if (o >= L) break;
+      const int k = o % m;
+      const int r = o / m;
+      const int q = r % p;
+      const int base = (r / p) * m + k;
+      const int step = (k + q * m) * scale;
+      cpx acc = buf[base];
+      int e = step;
+      for (int j = 1; j < p; ++j) {
+        const cpx v = buf[base + j * Lp];
+        const cpx w = P.tw[e];
+        acc.x += v.x * w.x - v.y * w.y;
+        acc.y += v.x * w.y + v.y * w.x;
+        e += step;
+        if (e >= L) e -= L;
+      }
+      val[u] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int o = t + kGenBigThreads * u;
+      if (o >= L) break;
+      buf[o] = val[u];
+    }
+    m = mp;
+  }
+  __syncthreads();
+  const int lo = L / 8, n = 6 * (L / 8);
+  cpx* out = P.out + d.out_off + (long long)g * n;
+  for (int i = t; i < n; i += kGenBigThreads) out[i] = buf[lo + i];
+}
+
 // Adjoint of the generic channel stage (the same contract as
 // chan_adjoint_kernel): for output gradient ybar (packed layout), x = DFT_L
 // (forward sign) of ybar zero-padded to L at positions L/8 .. L/8 + 6 (L/8),

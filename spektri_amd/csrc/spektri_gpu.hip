@@ -618,6 +618,27 @@ std::vector<int> batched_groups(const spk_ctx* c) {
 // One launch of spk::chan_generic_kernel for a group without a
 // power-of-two plan.
 hipError_t launch_generic(const spk_ctx* c, const ChanGroup& g, int nb, hipStream_t s) {
+  if (g.L > spk::kGenMaxL) {
+    spk::GenChanParams Q{};
+    Q.spec = c->spec.as<cpx>();
+    Q.spec_stride = c->spec_stride;
+    Q.N = c->N;
+    Q.K = c->K;
+    Q.real_input = c->complex_in ? 0 : 1;
+    Q.weights = c->weights.as<float>();
+    Q.chans = g.desc.as<spk::ChanDesc>();
+    Q.nch = (int)g.ids.size();
+    Q.nblocks = nb;
+    Q.out = c->chan_out.as<cpx>();
+    Q.tw = g.gtw.as<cpx>();
+    Q.L = g.L;
+    Q.nstages = (int)g.radix.size();
+    for (size_t i = 0; i < g.radix.size(); ++i) Q.radix[i] = g.radix[i];
+    Q.tile_threads = spk::kGenBigThreads;
+    const unsigned grid = (unsigned)((long long)Q.nch * nb);
+    hipLaunchKernelGGL(spk::chan_generic_big_kernel, dim3(grid), dim3(spk::kGenBigThreads), 0, s, Q);
+    return hipGetLastError();
+  }
   spk::GenChanParams Q{};
   Q.spec = c->spec.as<cpx>();
   Q.spec_stride = c->spec_stride;
@@ -1378,7 +1399,7 @@ spk_status spk_buffer_size_of(const spk_ctx* c, spk_buffer_size* out) {
 spk_status spk_add_channel(spk_ctx* c, int64_t first_bin, uint32_t bins, const float* weights, uint32_t* id) {
   if (!c) return SPK_ERR_INVALID;
   if (bins == 0 || bins % 4 != 0) return fail(c, SPK_ERR_INVALID, "IFFT size must be a positive multiple of 4");
-  if ((int)bins > c->N || (!find_chan_plan((int)bins) && (int)bins > spk::kGenMaxL))
+  if ((int)bins > c->N || (!find_chan_plan((int)bins) && (int)bins > spk::kGenBigMaxL))
     return fail(c, SPK_ERR_UNSUPPORTED,
                 "IFFT size must be <= N and a power of two up to 8192 or another multiple of 4 up to 4096");
   Channel ch;
@@ -1823,6 +1844,9 @@ spk_status spk_channels_adjoint(spk_ctx* c, const float* ybar, float* wbar, floa
   if (!c->have_output || c->chans_dirty) return fail(c, SPK_ERR_STATE, "no output yet");
   if (!c->keep_spec) return fail(c, SPK_ERR_STATE, "the adjoint needs a context created with SPK_FLAG_KEEP_SPECTRA");
   const int nb = (int)c->last_blocks;
+  for (const auto& kv : c->groups)
+    if (kv.second.generic && kv.second.L > spk::kGenMaxL)
+      return fail(c, SPK_ERR_UNSUPPORTED, "the adjoint takes IFFT sizes up to 8192 (other than powers of two: 4096)");
   for (auto& kv : c->groups) {
     ChanGroup& g = kv.second;
     const int nch = (int)g.ids.size();

@@ -203,7 +203,7 @@ def test_design_gradient_fd_and_adam():
         lv = loss32(bank.split(bank.apply(w)))
         lv.backward()
         opt.step()
-        losses.append(float(lv))
+        losses.append(float(lv.detach()))
     report("adjoint_design_16k", blocks=nb, channels=len(chans), terms=len(loss32.rows), loss_fp32=float(loss),
            loss_fp64=l64, grad_rel_vs_fp64=e_ref, fd_rel_max=fd_err, adam_first=losses[0], adam_last=losses[-1])
     assert abs(float(loss) - l64) < 1e-4 * abs(l64) + 1e-7

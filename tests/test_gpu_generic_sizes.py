@@ -3,8 +3,9 @@
 The reference turns (fs_out, fc_out) into L = round4(fs_out / bin spacing)
 (fcfb.rs:225-248), so any multiple of 4 is a valid filter, and it keeps
 buf[L/8 .. L/8*7] with integer division (fcfb.rs:176-177): 6 (L/8) samples
-per block, none at all for L = 4.  These sizes run in the mixed-radix kernel
-(chan_generic.h).  Checked against the FP64 oracle at 1e-5 RMS of full
+per block, none at all for L = 4.  These sizes run in the mixed-radix
+kernels (chan_generic.h: several channels per workgroup up to 4096, one
+per workgroup above, with the power-of-two 16384 too).  Checked against the FP64 oracle at 1e-5 RMS of full
 scale, real and complex input, beside power-of-two channels in both launch
 forms and with graph replay.
 """
@@ -21,7 +22,8 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
-SIZES = [4, 12, 20, 24, 28, 36, 44, 60, 100, 132, 180, 308, 508, 1020, 2044, 4092, 4084]
+SIZES = [4, 12, 20, 24, 28, 36, 44, 60, 100, 132, 180, 308, 508, 1020, 2044, 4092, 4084,
+         4100, 10000, 16380, 16384]
 
 
 def chans_for(N, K, rng, cplx):
@@ -119,8 +121,19 @@ def test_filter_at_48k_gives_l24():
 
 
 def test_generic_sizes_rejected_where_unsupported():
-    st, _ = DspState.init(DspParams(complex=False, fs_in=1.0, fft_size=16384, ffts_per_buf=2,
+    N = 32768
+    st, _ = DspState.init(DspParams(complex=False, fs_in=1.0, fft_size=N, ffts_per_buf=2,
                                     spectrum_averages=2, keep_spectra=True))
     with pytest.raises(_lib.SpektriError):
-        st.add_channel(0, 4100, None)  # above the generic kernel's 4096
+        st.add_channel(0, 20484, None)  # above the wide kernel's 20480
+    with pytest.raises(_lib.SpektriError):
+        st.add_channel(0, 32768, None)  # a power of two above 16384 without a plan
+    st.add_channel(0, 4100, None)
+    x = torch.zeros(2 * (N - N // 4), dtype=torch.int16, device="cuda")
+    st.process_device(x.data_ptr(), x.numel())
+    st.synchronize()
+    assert st.channel_output(0).shape == (2, 6 * (4100 // 8))
+    buf = torch.zeros(1 << 20, dtype=torch.float32, device="cuda")
+    with pytest.raises(_lib.SpektriError):
+        st.channels_adjoint(buf.data_ptr(), buf.data_ptr(), 0)  # no adjoint above 4096 (not a power of two)
     st.close()
