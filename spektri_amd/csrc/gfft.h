@@ -93,4 +93,200 @@ __global__ __launch_bounds__(kGfftThreads) void gfft_pass_kernel(GfftPassParams 
   }
 }
 
+// Four-step form, Nc = N1 N2 with N1, N2 <= kG4MaxLine: two launches
+// instead of one per radix.  Input pair n = N2 n1 + n2, output bin
+// k = k1 + N1 k2:
+//   A (columns): Y[k1 N2 + n2] = W_Nc^(n2 k1) sum_n1 x[N2 n1 + n2] W_N1^(n1 k1)
+//   B (rows):    X[k1 + N1 k2] = sum_n2 Y[k1 N2 + n2] W_N2^(n2 k2)
+// A workgroup takes kG4Lines adjacent lines (columns n2 in A, rows k1 in
+// B), so every global read and write is a run of kG4Lines consecutive
+// complex values or a whole row.  Each line's DFT runs in LDS over the
+// radices of its length (forward sign), ping-ponging between two LDS
+// buffers: radices 2, 3, 4, 5, 7 and 8 as butterflies with their R-point
+// DFT unrolled (g4_stage), larger primes as one table sum per output
+// (g4_stage_any).
+constexpr int kG4Lines = 16;
+constexpr int kG4MaxLine = 255;  // two buffers of 16 padded lines within 64 KiB of dynamic LDS
+constexpr int kG4Threads = 256;
+constexpr int kG4MaxStages = 12;
+
+struct Gfft4Params {
+  const void* src;         // A: this call's samples (pairs)
+  const void* hist;        // A: the previous call's last Oc pairs
+  int Hc, Oc;
+  float wscale;
+  const cpx* Y;            // B: column results
+  cpx* out;                // A: Y rows (stride Nc); B: spectrum rows
+  long long out_stride;
+  const cpx* tw_line;      // w_n^e, e = 0..n-1, forward sign (n = line length)
+  const cpx* tw_full;      // A: W_Nc^m, m = 0..Nc-1
+  int Nc, N1, N2;
+  int n;                   // line length: N1 in A, N2 in B
+  int nlines;              // N2 in A, N1 in B
+  int nstages;
+  int radix[kG4MaxStages];
+  int nblocks;
+};
+
+// a / d for 0 <= a < 2^12, 1 <= d <= 2^8 through the float reciprocal:
+// (a + 1/2) / d is at least 1/(2 d) >= 2^-9 from the nearest integer and
+// the two roundings move it by at most 2^-10 (relative 2^-23 times
+// a / d < 2^12), so truncation gives the integer quotient.
+__device__ __forceinline__ int g4_div(int a, float inv_d) { return (int)(((float)a + 0.5f) * inv_d); }
+
+// One stage of radix R (compile time) over the kG4Lines lines, from LDS
+// buffer `in` to `out`: butterfly (l, c, k), k < m, reads
+// v_j = in_l[c m + k + j n/R], applies w_n^(j k n/(m R)) and the R-point
+// DFT (roots w_R^(jq) = w_n^(jq n/R)), and writes out_l[m (c R + q) + k].
+template <int R>
+__device__ __forceinline__ void g4_stage(const cpx* in, cpx* out, int LS, int n, int m, const cpx* tw, int t) {
+  const int nbf = n / R;
+  const int tot = kG4Lines * nbf;
+  const int s = n / (m * R);
+  const float inv_nbf = 1.0f / (float)nbf, inv_m = 1.0f / (float)m;
+  cpx w[R];
+#pragma unroll
+  for (int q = 1; q < R; ++q) w[q] = tw[q * nbf];
+  for (int f = t; f < tot; f += kG4Threads) {
+    const int l = g4_div(f, inv_nbf), bidx = f - l * nbf;
+    const int c = g4_div(bidx, inv_m), k = bidx - c * m;
+    const cpx* a = in + l * LS + c * m + k;
+    cpx v[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = a[j * nbf];
+    if (m > 1) {
+#pragma unroll
+      for (int j = 1; j < R; ++j) v[j] = cmul(v[j], tw[j * k * s]);  // j k s < n
+    }
+    cpx* o = out + l * LS + m * (c * R) + k;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      cpx acc = v[0];
+#pragma unroll
+      for (int j = 1; j < R; ++j) {
+        const int e = (j * q) % R;
+        if (e == 0) {
+          acc.x += v[j].x;
+          acc.y += v[j].y;
+        } else {
+          acc.x += v[j].x * w[e].x - v[j].y * w[e].y;
+          acc.y += v[j].x * w[e].y + v[j].y * w[e].x;
+        }
+      }
+      o[q * m] = acc;
+    }
+  }
+}
+
+// Radices without a butterfly of their own: every output as one sum.
+__device__ __forceinline__ void g4_stage_any(const cpx* in, cpx* out, int LS, int n, int m, int p, const cpx* tw,
+                                             int t) {
+  const int total = kG4Lines * n;
+  const int np = n / p;
+  const int mp = m * p;
+  const int scale = n / mp;
+  for (int f = t; f < total; f += kG4Threads) {
+    const int l = f / n, o = f - l * n;
+    const cpx* a = in + l * LS;
+    const int k = o % m;
+    const int r = o / m;
+    const int q = r % p;
+    const int base = (r / p) * m + k;
+    const int step = (k + q * m) * scale;
+    cpx acc = a[base];
+    int e = step;
+    for (int j = 1; j < p; ++j) {
+      const cpx x = a[base + j * np];
+      const cpx w = tw[e];
+      acc.x += x.x * w.x - x.y * w.y;
+      acc.y += x.x * w.y + x.y * w.x;
+      e += step;
+      if (e >= n) e -= n;
+    }
+    out[l * LS + o] = acc;
+  }
+}
+
+template <int LOAD, bool STEP_A>
+__global__ __launch_bounds__(kG4Threads) void gfft4_kernel(Gfft4Params P) {
+  extern __shared__ __align__(16) float g4_lds[];  // 2 kG4Lines (n + 1) cpx
+  const int n = P.n;
+  const int LS = n + 1;  // padded line stride
+  cpx* bufs[2] = {reinterpret_cast<cpx*>(g4_lds), reinterpret_cast<cpx*>(g4_lds) + kG4Lines * LS};
+  cpx* buf = bufs[0];
+  const int t = (int)threadIdx.x;
+  const int c0 = (int)blockIdx.x * kG4Lines;
+  const int b = (int)blockIdx.y;
+  const int total = kG4Lines * n;
+
+  // load: A reads x[N2 n1 + n2] with n2 = c0 + l fastest; B reads row k1 = c0 + l.
+  // Every load of the thread is issued before the first LDS store.
+  constexpr int UL = (kG4Lines * kG4MaxLine + kG4Threads - 1) / kG4Threads;
+  GfftPassParams Q{};
+  Q.src = P.src;
+  Q.hist = P.hist;
+  Q.Hc = P.Hc;
+  Q.Oc = P.Oc;
+  Q.wscale = P.wscale;
+  cpx vals[UL];
+#pragma unroll
+  for (int u = 0; u < UL; ++u) {
+    const int f = t + kG4Threads * u;
+    vals[u] = cpx{0.f, 0.f};
+    if (f < total) {
+      const int l = STEP_A ? f % kG4Lines : f / n;
+      const int i = STEP_A ? f / kG4Lines : f % n;
+      const int line = c0 + l;
+      if (line < P.nlines) {
+        if constexpr (STEP_A)
+          vals[u] = gfft_load<LOAD>(Q, b, P.N2 * i + line);
+        else
+          vals[u] = P.Y[(long long)b * P.Nc + (long long)line * P.N2 + i];
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < UL; ++u) {
+    const int f = t + kG4Threads * u;
+    if (f < total) {
+      const int l = STEP_A ? f % kG4Lines : f / n;
+      const int i = STEP_A ? f / kG4Lines : f % n;
+      buf[l * LS + i] = vals[u];
+    }
+  }
+
+  int m = 1;
+  for (int s = 0; s < P.nstages; ++s) {
+    __syncthreads();
+    const int p = P.radix[s];
+    cpx* nxt = bufs[(s + 1) & 1];
+    switch (p) {
+      case 2: g4_stage<2>(buf, nxt, LS, n, m, P.tw_line, t); break;
+      case 3: g4_stage<3>(buf, nxt, LS, n, m, P.tw_line, t); break;
+      case 4: g4_stage<4>(buf, nxt, LS, n, m, P.tw_line, t); break;
+      case 5: g4_stage<5>(buf, nxt, LS, n, m, P.tw_line, t); break;
+      case 7: g4_stage<7>(buf, nxt, LS, n, m, P.tw_line, t); break;
+      case 8: g4_stage<8>(buf, nxt, LS, n, m, P.tw_line, t); break;
+      default: g4_stage_any(buf, nxt, LS, n, m, p, P.tw_line, t); break;
+    }
+    buf = nxt;
+    m *= p;
+  }
+  __syncthreads();
+
+  // store: A writes Y[k1 N2 + n2] W_Nc^(n2 k1); B writes X[k1 + N1 k2] (line = k1)
+  cpx* out = P.out + (long long)b * P.out_stride;
+  for (int f = t; f < total; f += kG4Threads) {
+    const int l = f % kG4Lines, kk = f / kG4Lines;
+    const int line = c0 + l;
+    if (line >= P.nlines) continue;
+    const cpx v = buf[l * LS + kk];
+    if constexpr (STEP_A) {
+      out[(long long)kk * P.N2 + line] = cmul(v, P.tw_full[line * kk]);  // n2 k1 < Nc
+    } else {
+      out[line + (long long)P.N1 * kk] = v;
+    }
+  }
+}
+
 }  // namespace spk

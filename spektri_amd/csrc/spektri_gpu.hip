@@ -269,6 +269,12 @@ struct spk_ctx {
   bool gfft = false;
   std::vector<int> gfft_radices;
   DevBuf gtw;
+  // four-step form (gfft4_kernel): Nc = N1 N2, stage radices per line length,
+  // w_n^e tables of the two line lengths
+  bool gfft4 = false;
+  int g4N1 = 0, g4N2 = 0;
+  std::vector<int> g4r1, g4r2;
+  DevBuf g4tw1, g4tw2;
   DevBuf adj_part;           // adjoint per-split weight sums
   int ncu = 0;
   // channel stage launch form and batched tuning (chunk blocks, rounds per item, workgroups per CU)
@@ -839,12 +845,68 @@ hipError_t dispatch_gfft_pass(int R, const spk::GfftPassParams& P, int load, dim
   return hipGetLastError();
 }
 
-// Mixed-radix forward transform of nb blocks into the spectrum rows: the
-// first pass frames and windows the input, passes alternate between the
-// scratch rows (cY) and the spectrum rows so that the last writes the
-// spectrum rows.
+// Four-step split Nc = N1 N2 with both line lengths within one LDS tile
+// (N1 the largest such divisor up to sqrt(Nc)); false when there is none.
+bool gfft4_split(int Nc, int* N1, int* N2) {
+  for (int d = (int)std::sqrt((double)Nc); d >= 2; --d)
+    if (Nc % d == 0 && d <= spk::kG4MaxLine && Nc / d <= spk::kG4MaxLine) {
+      *N1 = d;
+      *N2 = Nc / d;
+      return true;
+    }
+  return false;
+}
+
+// Mixed-radix forward transform of nb blocks into the spectrum rows.
+// Four-step form: columns (framing and windowing the input as they load
+// it) into the scratch rows, then rows into the spectrum rows.  Otherwise
+// one pass per radix: the first pass frames and windows the input, passes
+// alternate between the scratch rows (cY) and the spectrum rows so that the
+// last writes the spectrum rows.
 spk_status enqueue_gfft(spk_ctx* c, const void* src, int nb, hipStream_t s) {
   const int pk = c->complex_in ? 1 : 2;
+  if (c->gfft4) {
+    spk::Gfft4Params P{};
+    P.src = src;
+    P.hist = c->hist.p;
+    P.Hc = c->H / pk;
+    P.Oc = c->O / pk;
+    P.wscale = c->wscale;
+    P.tw_full = c->gtw.as<cpx>();
+    P.Nc = c->Nc;
+    P.N1 = c->g4N1;
+    P.N2 = c->g4N2;
+    P.nblocks = nb;
+    // A: N1-point columns, N2 of them
+    P.out = c->cY.as<cpx>();
+    P.out_stride = c->Nc;
+    P.tw_line = c->g4tw1.as<cpx>();
+    P.n = c->g4N1;
+    P.nlines = c->g4N2;
+    P.nstages = (int)c->g4r1.size();
+    for (size_t i = 0; i < c->g4r1.size(); ++i) P.radix[i] = c->g4r1[i];
+    const dim3 ga((unsigned)((c->g4N2 + spk::kG4Lines - 1) / spk::kG4Lines), (unsigned)nb);
+    const size_t la = (size_t)2 * spk::kG4Lines * (c->g4N1 + 1) * sizeof(cpx);
+    if (c->s16_pairs)
+      hipLaunchKernelGGL((spk::gfft4_kernel<1, true>), ga, dim3(spk::kG4Threads), la, s, P);
+    else
+      hipLaunchKernelGGL((spk::gfft4_kernel<2, true>), ga, dim3(spk::kG4Threads), la, s, P);
+    HIP_TRY(c, hipGetLastError());
+    // B: N2-point rows, N1 of them
+    P.Y = c->cY.as<cpx>();
+    P.out = c->spec.as<cpx>();
+    P.out_stride = c->spec_stride;
+    P.tw_line = c->g4tw2.as<cpx>();
+    P.n = c->g4N2;
+    P.nlines = c->g4N1;
+    P.nstages = (int)c->g4r2.size();
+    for (size_t i = 0; i < c->g4r2.size(); ++i) P.radix[i] = c->g4r2[i];
+    const dim3 gb((unsigned)((c->g4N1 + spk::kG4Lines - 1) / spk::kG4Lines), (unsigned)nb);
+    const size_t lb = (size_t)2 * spk::kG4Lines * (c->g4N2 + 1) * sizeof(cpx);
+    hipLaunchKernelGGL((spk::gfft4_kernel<0, false>), gb, dim3(spk::kG4Threads), lb, s, P);
+    HIP_TRY(c, hipGetLastError());
+    return SPK_OK;
+  }
   const int np = (int)c->gfft_radices.size();
   spk::GfftPassParams P{};
   P.src = src;
@@ -1299,6 +1361,17 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
       std::vector<cpx> g((size_t)c->Nc);
       for (int m = 0; m < c->Nc; ++m) g[(size_t)m] = wn(m, c->Nc);
       if (upload(c->gtw, g) != hipSuccess) return cleanup(SPK_ERR_DEVICE);
+      c->gfft4 = gfft4_split(c->Nc, &c->g4N1, &c->g4N2);
+      if (c->gfft4) {
+        c->g4r1 = generic_radices(c->g4N1);
+        c->g4r2 = generic_radices(c->g4N2);
+        std::vector<cpx> t1((size_t)c->g4N1), t2((size_t)c->g4N2);
+        for (int e = 0; e < c->g4N1; ++e) t1[(size_t)e] = wn(e, c->g4N1);
+        for (int e = 0; e < c->g4N2; ++e) t2[(size_t)e] = wn(e, c->g4N2);
+        if ((int)c->g4r1.size() > spk::kG4MaxStages || (int)c->g4r2.size() > spk::kG4MaxStages ||
+            upload(c->g4tw1, t1) != hipSuccess || upload(c->g4tw2, t2) != hipSuccess)
+          return cleanup(SPK_ERR_DEVICE);
+      }
       if (hipDeviceSynchronize() != hipSuccess) return cleanup(SPK_ERR_DEVICE);
       *out = c;
       return SPK_OK;
