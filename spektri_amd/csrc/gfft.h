@@ -106,7 +106,7 @@ __global__ __launch_bounds__(kGfftThreads) void gfft_pass_kernel(GfftPassParams 
 // DFT unrolled (g4_stage), larger primes as one table sum per output
 // (g4_stage_any).
 constexpr int kG4Lines = 16;
-constexpr int kG4MaxLine = 255;  // two buffers of 16 padded lines within 64 KiB of dynamic LDS
+constexpr int kG4MaxLine = 247;  // two buffers of 16 padded lines and the twiddles within 64 KiB of dynamic LDS
 constexpr int kG4Threads = 256;
 constexpr int kG4MaxStages = 12;
 
@@ -209,11 +209,13 @@ __device__ __forceinline__ void g4_stage_any(const cpx* in, cpx* out, int LS, in
 
 template <int LOAD, bool STEP_A>
 __global__ __launch_bounds__(kG4Threads) void gfft4_kernel(Gfft4Params P) {
-  extern __shared__ __align__(16) float g4_lds[];  // 2 kG4Lines (n + 1) cpx
+  extern __shared__ __align__(16) float g4_lds[];  // 2 kG4Lines (n + 1) + n cpx
   const int n = P.n;
   const int LS = n + 1;  // padded line stride
   cpx* bufs[2] = {reinterpret_cast<cpx*>(g4_lds), reinterpret_cast<cpx*>(g4_lds) + kG4Lines * LS};
   cpx* buf = bufs[0];
+  cpx* tw = bufs[1] + kG4Lines * LS;  // the line's w_n^e, read by every butterfly
+  for (int e = (int)threadIdx.x; e < n; e += kG4Threads) tw[e] = P.tw_line[e];
   const int t = (int)threadIdx.x;
   const int c0 = (int)blockIdx.x * kG4Lines;
   const int b = (int)blockIdx.y;
@@ -261,13 +263,13 @@ __global__ __launch_bounds__(kG4Threads) void gfft4_kernel(Gfft4Params P) {
     const int p = P.radix[s];
     cpx* nxt = bufs[(s + 1) & 1];
     switch (p) {
-      case 2: g4_stage<2>(buf, nxt, LS, n, m, P.tw_line, t); break;
-      case 3: g4_stage<3>(buf, nxt, LS, n, m, P.tw_line, t); break;
-      case 4: g4_stage<4>(buf, nxt, LS, n, m, P.tw_line, t); break;
-      case 5: g4_stage<5>(buf, nxt, LS, n, m, P.tw_line, t); break;
-      case 7: g4_stage<7>(buf, nxt, LS, n, m, P.tw_line, t); break;
-      case 8: g4_stage<8>(buf, nxt, LS, n, m, P.tw_line, t); break;
-      default: g4_stage_any(buf, nxt, LS, n, m, p, P.tw_line, t); break;
+      case 2: g4_stage<2>(buf, nxt, LS, n, m, tw, t); break;
+      case 3: g4_stage<3>(buf, nxt, LS, n, m, tw, t); break;
+      case 4: g4_stage<4>(buf, nxt, LS, n, m, tw, t); break;
+      case 5: g4_stage<5>(buf, nxt, LS, n, m, tw, t); break;
+      case 7: g4_stage<7>(buf, nxt, LS, n, m, tw, t); break;
+      case 8: g4_stage<8>(buf, nxt, LS, n, m, tw, t); break;
+      default: g4_stage_any(buf, nxt, LS, n, m, p, tw, t); break;
     }
     buf = nxt;
     m *= p;

@@ -886,7 +886,7 @@ spk_status enqueue_gfft(spk_ctx* c, const void* src, int nb, hipStream_t s) {
     P.nstages = (int)c->g4r1.size();
     for (size_t i = 0; i < c->g4r1.size(); ++i) P.radix[i] = c->g4r1[i];
     const dim3 ga((unsigned)((c->g4N2 + spk::kG4Lines - 1) / spk::kG4Lines), (unsigned)nb);
-    const size_t la = (size_t)2 * spk::kG4Lines * (c->g4N1 + 1) * sizeof(cpx);
+    const size_t la = ((size_t)2 * spk::kG4Lines * (c->g4N1 + 1) + c->g4N1) * sizeof(cpx);
     if (c->s16_pairs)
       hipLaunchKernelGGL((spk::gfft4_kernel<1, true>), ga, dim3(spk::kG4Threads), la, s, P);
     else
@@ -902,7 +902,7 @@ spk_status enqueue_gfft(spk_ctx* c, const void* src, int nb, hipStream_t s) {
     P.nstages = (int)c->g4r2.size();
     for (size_t i = 0; i < c->g4r2.size(); ++i) P.radix[i] = c->g4r2[i];
     const dim3 gb((unsigned)((c->g4N1 + spk::kG4Lines - 1) / spk::kG4Lines), (unsigned)nb);
-    const size_t lb = (size_t)2 * spk::kG4Lines * (c->g4N2 + 1) * sizeof(cpx);
+    const size_t lb = ((size_t)2 * spk::kG4Lines * (c->g4N2 + 1) + c->g4N2) * sizeof(cpx);
     hipLaunchKernelGGL((spk::gfft4_kernel<0, false>), gb, dim3(spk::kG4Threads), lb, s, P);
     HIP_TRY(c, hipGetLastError());
     return SPK_OK;
