@@ -204,9 +204,9 @@ def test_design_gradient_fd_and_adam():
         lv.backward()
         opt.step()
         losses.append(float(lv.detach()))
-    report("adjoint_design_16k", blocks=nb, channels=len(chans), terms=len(loss32.rows), loss_fp32=float(loss),
+    report("adjoint_design_16k", blocks=nb, channels=len(chans), terms=len(loss32.rows), loss_fp32=float(loss.detach()),
            loss_fp64=l64, grad_rel_vs_fp64=e_ref, fd_rel_max=fd_err, adam_first=losses[0], adam_last=losses[-1])
-    assert abs(float(loss) - l64) < 1e-4 * abs(l64) + 1e-7
+    assert abs(float(loss.detach()) - l64) < 1e-4 * abs(l64) + 1e-7
     assert e_ref < REL and fd_err < REL, (e_ref, fd_err)
     assert losses[-1] < 0.5 * losses[0], (losses[0], losses[-1])
     bank.close()
