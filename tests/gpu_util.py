@@ -34,13 +34,17 @@ def rms(a, b) -> float:
     return float(np.sqrt(np.mean(np.abs(d) ** 2))) if d.size else 0.0
 
 
-REPORT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out",
-                      "parity_report.json")
+# JSON file that collects every test's error figures, when set
+REPORT = os.environ.get("SPK_PARITY_REPORT", "")
 
 
 def report(key: str, **vals):
-    """Append per-config error figures to the JSON parity report (REPORT)."""
-    os.makedirs(os.path.dirname(REPORT), exist_ok=True)
+    """Print per-config error figures and append them to the JSON parity
+    report named by $SPK_PARITY_REPORT (if set)."""
+    print(f"[parity] {key}: {vals}", flush=True)
+    if not REPORT:
+        return
+    os.makedirs(os.path.dirname(os.path.abspath(REPORT)), exist_ok=True)
     data = {}
     if os.path.exists(REPORT):
         try:
@@ -49,4 +53,3 @@ def report(key: str, **vals):
             data = {}
     data[key] = vals
     json.dump(data, open(REPORT, "w"), indent=1, sort_keys=True)
-    print(f"[parity] {key}: {vals}", flush=True)
