@@ -91,3 +91,34 @@ def test_nearest_freq_is_accepted_by_add_filter():
         assert bn is not None and bn.bins % 4 == 0
         assert abs(nf[0] - fso) <= 2 * fs / N + 1e-6 and abs(nf[1] - fco) <= 2 * fs / N + 1e-6
     assert dsp.nearest_freq(fs, 0.0, N, 1.0, 0.0) is None  # rounds to 0 bins
+
+
+def _largest_prime(n: int) -> int:
+    p, big = 2, 1
+    while p * p <= n:
+        while n % p == 0:
+            big, n = p, n // p
+        p += 1
+    return max(big, n) if n > 1 else big
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_fft_size_acceptance_rule(cplx):
+    """spk_create accepts (and then, without a GPU, fails at the first HIP
+    call with SPK_ERR_DEVICE) exactly the sizes it documents: powers of two
+    in the planned ranges, and every other multiple of 8 from 16 to 2^20
+    whose transform length (N/2 real, N complex) has no prime factor above
+    31; every other reference-valid size is SPK_ERR_UNSUPPORTED."""
+    lib = _lib.load()
+    out = ctypes.c_void_p()
+    fmt = _lib.FMT_CF32LE if cplx else _lib.FMT_S16LE
+    sizes = list(range(4, 4100, 4)) + [48000, 61440, 96000, 100000, 65536 * 3, 2 ** 19, 2 ** 20, 2 ** 20 + 8,
+                                       2 ** 21, 8 * 37 * 41, 8 * 31 * 31 * 31]
+    for N in sizes:
+        Nc = N if cplx else N // 2
+        pow2 = N & (N - 1) == 0
+        planned = pow2 and ((1024 <= N <= 262144) if cplx else (1024 <= N <= 524288))
+        mixed = N % 8 == 0 and 16 <= N <= 2 ** 20 and _largest_prime(Nc) <= 31
+        want = 3 if (planned or mixed) else 2
+        cfg = _lib.SpkConfig(N, fmt, 0.0, 8, 16, 0)
+        assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == want, (N, cplx)
