@@ -98,15 +98,19 @@ __global__ __launch_bounds__(kGfftThreads) void gfft_pass_kernel(GfftPassParams 
 // k = k1 + N1 k2:
 //   A (columns): Y[k1 N2 + n2] = W_Nc^(n2 k1) sum_n1 x[N2 n1 + n2] W_N1^(n1 k1)
 //   B (rows):    X[k1 + N1 k2] = sum_n2 Y[k1 N2 + n2] W_N2^(n2 k2)
-// A workgroup takes kG4Lines adjacent lines (columns n2 in A, rows k1 in
-// B), so every global read and write is a run of kG4Lines consecutive
-// complex values or a whole row.  Each line's DFT runs in LDS over the
+// A workgroup takes LINES adjacent lines (columns n2 in A, rows k1 in B),
+// so every global read and write is a run of LINES consecutive values or
+// a whole row: 8 lines of up to 480 points, or 16 of up to 247 (A, where
+// the runs of s16 pairs are short).  Each line's DFT runs in LDS over the
 // radices of its length (forward sign), ping-ponging between two LDS
 // buffers: radices 2, 3, 4, 5, 7 and 8 as butterflies with their R-point
 // DFT unrolled (g4_stage), larger primes as one table sum per output
 // (g4_stage_any).
-constexpr int kG4Lines = 16;
-constexpr int kG4MaxLine = 247;  // two buffers of 16 padded lines and the twiddles within 64 KiB of dynamic LDS
+constexpr int kG4Lines = 8;
+constexpr int kG4MaxLine = 480;  // two buffers of 8 padded lines and the twiddles within 64 KiB of dynamic LDS
+constexpr int kG4LinesWide = 16;
+constexpr int kG4MaxLineWide = 247;  // the same for 16 lines
+constexpr int g4_max_line(int lines) { return lines == kG4LinesWide ? kG4MaxLineWide : kG4MaxLine; }
 constexpr int kG4Threads = 256;
 constexpr int kG4MaxStages = 12;
 
@@ -128,20 +132,22 @@ struct Gfft4Params {
   int nblocks;
 };
 
-// a / d for 0 <= a < 2^12, 1 <= d <= 2^8 through the float reciprocal:
+// a / d for 0 <= a < 2^12, 1 <= d <= 2^8 through the float reciprocal
+// (butterfly numbers stay below LINES g4_max_line(LINES) / 2 <= 1976,
+// line positions below 240, sub-transform sizes m at most 240):
 // (a + 1/2) / d is at least 1/(2 d) >= 2^-9 from the nearest integer and
 // the two roundings move it by at most 2^-10 (relative 2^-23 times
 // a / d < 2^12), so truncation gives the integer quotient.
 __device__ __forceinline__ int g4_div(int a, float inv_d) { return (int)(((float)a + 0.5f) * inv_d); }
 
-// One stage of radix R (compile time) over the kG4Lines lines, from LDS
+// One stage of radix R (compile time) over the LINES lines, from LDS
 // buffer `in` to `out`: butterfly (l, c, k), k < m, reads
 // v_j = in_l[c m + k + j n/R], applies w_n^(j k n/(m R)) and the R-point
 // DFT (roots w_R^(jq) = w_n^(jq n/R)), and writes out_l[m (c R + q) + k].
-template <int R>
+template <int R, int LINES>
 __device__ __forceinline__ void g4_stage(const cpx* in, cpx* out, int LS, int n, int m, const cpx* tw, int t) {
   const int nbf = n / R;
-  const int tot = kG4Lines * nbf;
+  const int tot = LINES * nbf;
   const int s = n / (m * R);
   const float inv_nbf = 1.0f / (float)nbf, inv_m = 1.0f / (float)m;
   cpx w[R];
@@ -179,9 +185,10 @@ __device__ __forceinline__ void g4_stage(const cpx* in, cpx* out, int LS, int n,
 }
 
 // Radices without a butterfly of their own: every output as one sum.
+template <int LINES>
 __device__ __forceinline__ void g4_stage_any(const cpx* in, cpx* out, int LS, int n, int m, int p, const cpx* tw,
                                              int t) {
-  const int total = kG4Lines * n;
+  const int total = LINES * n;
   const int np = n / p;
   const int mp = m * p;
   const int scale = n / mp;
@@ -207,23 +214,23 @@ __device__ __forceinline__ void g4_stage_any(const cpx* in, cpx* out, int LS, in
   }
 }
 
-template <int LOAD, bool STEP_A>
+template <int LOAD, bool STEP_A, int LINES>
 __global__ __launch_bounds__(kG4Threads) void gfft4_kernel(Gfft4Params P) {
-  extern __shared__ __align__(16) float g4_lds[];  // 2 kG4Lines (n + 1) + n cpx
+  extern __shared__ __align__(16) float g4_lds[];  // 2 LINES (n + 1) + n cpx
   const int n = P.n;
   const int LS = n + 1;  // padded line stride
-  cpx* bufs[2] = {reinterpret_cast<cpx*>(g4_lds), reinterpret_cast<cpx*>(g4_lds) + kG4Lines * LS};
+  cpx* bufs[2] = {reinterpret_cast<cpx*>(g4_lds), reinterpret_cast<cpx*>(g4_lds) + LINES * LS};
   cpx* buf = bufs[0];
-  cpx* tw = bufs[1] + kG4Lines * LS;  // the line's w_n^e, read by every butterfly
+  cpx* tw = bufs[1] + LINES * LS;  // the line's w_n^e, read by every butterfly
   for (int e = (int)threadIdx.x; e < n; e += kG4Threads) tw[e] = P.tw_line[e];
   const int t = (int)threadIdx.x;
-  const int c0 = (int)blockIdx.x * kG4Lines;
+  const int c0 = (int)blockIdx.x * LINES;
   const int b = (int)blockIdx.y;
-  const int total = kG4Lines * n;
+  const int total = LINES * n;
 
   // load: A reads x[N2 n1 + n2] with n2 = c0 + l fastest; B reads row k1 = c0 + l.
   // Every load of the thread is issued before the first LDS store.
-  constexpr int UL = (kG4Lines * kG4MaxLine + kG4Threads - 1) / kG4Threads;
+  constexpr int UL = (LINES * g4_max_line(LINES) + kG4Threads - 1) / kG4Threads;
   GfftPassParams Q{};
   Q.src = P.src;
   Q.hist = P.hist;
@@ -236,8 +243,8 @@ __global__ __launch_bounds__(kG4Threads) void gfft4_kernel(Gfft4Params P) {
     const int f = t + kG4Threads * u;
     vals[u] = cpx{0.f, 0.f};
     if (f < total) {
-      const int l = STEP_A ? f % kG4Lines : f / n;
-      const int i = STEP_A ? f / kG4Lines : f % n;
+      const int l = STEP_A ? f % LINES : f / n;
+      const int i = STEP_A ? f / LINES : f % n;
       const int line = c0 + l;
       if (line < P.nlines) {
         if constexpr (STEP_A)
@@ -251,8 +258,8 @@ __global__ __launch_bounds__(kG4Threads) void gfft4_kernel(Gfft4Params P) {
   for (int u = 0; u < UL; ++u) {
     const int f = t + kG4Threads * u;
     if (f < total) {
-      const int l = STEP_A ? f % kG4Lines : f / n;
-      const int i = STEP_A ? f / kG4Lines : f % n;
+      const int l = STEP_A ? f % LINES : f / n;
+      const int i = STEP_A ? f / LINES : f % n;
       buf[l * LS + i] = vals[u];
     }
   }
@@ -263,13 +270,13 @@ __global__ __launch_bounds__(kG4Threads) void gfft4_kernel(Gfft4Params P) {
     const int p = P.radix[s];
     cpx* nxt = bufs[(s + 1) & 1];
     switch (p) {
-      case 2: g4_stage<2>(buf, nxt, LS, n, m, tw, t); break;
-      case 3: g4_stage<3>(buf, nxt, LS, n, m, tw, t); break;
-      case 4: g4_stage<4>(buf, nxt, LS, n, m, tw, t); break;
-      case 5: g4_stage<5>(buf, nxt, LS, n, m, tw, t); break;
-      case 7: g4_stage<7>(buf, nxt, LS, n, m, tw, t); break;
-      case 8: g4_stage<8>(buf, nxt, LS, n, m, tw, t); break;
-      default: g4_stage_any(buf, nxt, LS, n, m, p, tw, t); break;
+      case 2: g4_stage<2, LINES>(buf, nxt, LS, n, m, tw, t); break;
+      case 3: g4_stage<3, LINES>(buf, nxt, LS, n, m, tw, t); break;
+      case 4: g4_stage<4, LINES>(buf, nxt, LS, n, m, tw, t); break;
+      case 5: g4_stage<5, LINES>(buf, nxt, LS, n, m, tw, t); break;
+      case 7: g4_stage<7, LINES>(buf, nxt, LS, n, m, tw, t); break;
+      case 8: g4_stage<8, LINES>(buf, nxt, LS, n, m, tw, t); break;
+      default: g4_stage_any<LINES>(buf, nxt, LS, n, m, p, tw, t); break;
     }
     buf = nxt;
     m *= p;
@@ -279,7 +286,7 @@ __global__ __launch_bounds__(kG4Threads) void gfft4_kernel(Gfft4Params P) {
   // store: A writes Y[k1 N2 + n2] W_Nc^(n2 k1); B writes X[k1 + N1 k2] (line = k1)
   cpx* out = P.out + (long long)b * P.out_stride;
   for (int f = t; f < total; f += kG4Threads) {
-    const int l = f % kG4Lines, kk = f / kG4Lines;
+    const int l = f % LINES, kk = f / LINES;
     const int line = c0 + l;
     if (line >= P.nlines) continue;
     const cpx v = buf[l * LS + kk];

@@ -885,12 +885,21 @@ spk_status enqueue_gfft(spk_ctx* c, const void* src, int nb, hipStream_t s) {
     P.nlines = c->g4N2;
     P.nstages = (int)c->g4r1.size();
     for (size_t i = 0; i < c->g4r1.size(); ++i) P.radix[i] = c->g4r1[i];
-    const dim3 ga((unsigned)((c->g4N2 + spk::kG4Lines - 1) / spk::kG4Lines), (unsigned)nb);
-    const size_t la = ((size_t)2 * spk::kG4Lines * (c->g4N1 + 1) + c->g4N1) * sizeof(cpx);
-    if (c->s16_pairs)
-      hipLaunchKernelGGL((spk::gfft4_kernel<1, true>), ga, dim3(spk::kG4Threads), la, s, P);
-    else
-      hipLaunchKernelGGL((spk::gfft4_kernel<2, true>), ga, dim3(spk::kG4Threads), la, s, P);
+    // 16 lines while they fit: longer runs of the input pairs
+    const int la_lines = c->g4N1 <= spk::kG4MaxLineWide ? spk::kG4LinesWide : spk::kG4Lines;
+    const dim3 ga((unsigned)((c->g4N2 + la_lines - 1) / la_lines), (unsigned)nb);
+    const size_t la = ((size_t)2 * la_lines * (c->g4N1 + 1) + c->g4N1) * sizeof(cpx);
+    if (la_lines == spk::kG4LinesWide) {
+      if (c->s16_pairs)
+        hipLaunchKernelGGL((spk::gfft4_kernel<1, true, spk::kG4LinesWide>), ga, dim3(spk::kG4Threads), la, s, P);
+      else
+        hipLaunchKernelGGL((spk::gfft4_kernel<2, true, spk::kG4LinesWide>), ga, dim3(spk::kG4Threads), la, s, P);
+    } else {
+      if (c->s16_pairs)
+        hipLaunchKernelGGL((spk::gfft4_kernel<1, true, spk::kG4Lines>), ga, dim3(spk::kG4Threads), la, s, P);
+      else
+        hipLaunchKernelGGL((spk::gfft4_kernel<2, true, spk::kG4Lines>), ga, dim3(spk::kG4Threads), la, s, P);
+    }
     HIP_TRY(c, hipGetLastError());
     // B: N2-point rows, N1 of them
     P.Y = c->cY.as<cpx>();
@@ -903,7 +912,7 @@ spk_status enqueue_gfft(spk_ctx* c, const void* src, int nb, hipStream_t s) {
     for (size_t i = 0; i < c->g4r2.size(); ++i) P.radix[i] = c->g4r2[i];
     const dim3 gb((unsigned)((c->g4N1 + spk::kG4Lines - 1) / spk::kG4Lines), (unsigned)nb);
     const size_t lb = ((size_t)2 * spk::kG4Lines * (c->g4N2 + 1) + c->g4N2) * sizeof(cpx);
-    hipLaunchKernelGGL((spk::gfft4_kernel<0, false>), gb, dim3(spk::kG4Threads), lb, s, P);
+    hipLaunchKernelGGL((spk::gfft4_kernel<0, false, spk::kG4Lines>), gb, dim3(spk::kG4Threads), lb, s, P);
     HIP_TRY(c, hipGetLastError());
     return SPK_OK;
   }

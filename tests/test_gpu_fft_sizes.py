@@ -37,6 +37,7 @@ CASES = [
     ("cf32le", 12000),    # complex: 8 4 3 5 5 5
     ("cf32le", 768),      # complex: 8 8 4 3
     ("cf32le", 512),      # complex power of two below the two-step plans
+    ("cf32le", 96000),    # four-step lines above 256: 300 x 320
     ("cs16le", 6000),     # s16 pairs read in place: 8 2 3 5 5 5
     ("cs8", 2 * 17 * 19 * 8),  # converted complex format, 8 2 17 19
 ]
