@@ -1374,12 +1374,14 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
       if (c->gfft4) {
         c->g4r1 = generic_radices(c->g4N1);
         c->g4r2 = generic_radices(c->g4N2);
+        // (lines of at most 480 points have at most 5 stages; more keeps the per-radix passes)
+        c->gfft4 = (int)c->g4r1.size() <= spk::kG4MaxStages && (int)c->g4r2.size() <= spk::kG4MaxStages;
+      }
+      if (c->gfft4) {
         std::vector<cpx> t1((size_t)c->g4N1), t2((size_t)c->g4N2);
         for (int e = 0; e < c->g4N1; ++e) t1[(size_t)e] = wn(e, c->g4N1);
         for (int e = 0; e < c->g4N2; ++e) t2[(size_t)e] = wn(e, c->g4N2);
-        if ((int)c->g4r1.size() > spk::kG4MaxStages || (int)c->g4r2.size() > spk::kG4MaxStages ||
-            upload(c->g4tw1, t1) != hipSuccess || upload(c->g4tw2, t2) != hipSuccess)
-          return cleanup(SPK_ERR_DEVICE);
+        if (upload(c->g4tw1, t1) != hipSuccess || upload(c->g4tw2, t2) != hipSuccess) return cleanup(SPK_ERR_DEVICE);
       }
       if (hipDeviceSynchronize() != hipSuccess) return cleanup(SPK_ERR_DEVICE);
       *out = c;
