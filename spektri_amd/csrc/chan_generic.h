@@ -1,5 +1,5 @@
-// Channel stage for IFFT sizes that are multiples of 4 but not powers of two
-// (the reference accepts any multiple of 4, fcfb.rs:225-248, and plans
+// Channel stage for IFFT sizes that are multiples of 4 without a power-of-two
+// plan (the reference accepts any multiple of 4, fcfb.rs:225-248, and plans
 // rustfft for it, fcfb.rs:144-150).  Same arithmetic contract as the
 // power-of-two kernels (fcfb.rs:154-188):
 //   B[i] = W[j] get_bin(first + j), j = (i + L/2) mod L,

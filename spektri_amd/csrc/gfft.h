@@ -1,6 +1,8 @@
-// Forward transform for FFT sizes that are not powers of two (rustfft plans
-// any N: multifft.rs:26-28,45).  Mixed-radix Stockham passes through global
-// memory over the Nc-point complex transform of every block (Nc = N for
+// Forward transform for FFT sizes outside the power-of-two plans (rustfft
+// plans any N: multifft.rs:26-28,45), in two forms: the four-step form
+// below (gfft4_kernel) when Nc splits into two lines of at most 480
+// points, otherwise mixed-radix Stockham passes through global memory,
+// one per radix, over the Nc-point complex transform of every block (Nc = N for
 // complex input, N/2 packed sample pairs for real input, which the untangle
 // kernel of cplx.h then turns into the real spectrum).  The first pass reads
 // the framed, windowed input (history of the last O samples, then this

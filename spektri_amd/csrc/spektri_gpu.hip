@@ -1485,7 +1485,7 @@ spk_status spk_add_channel(spk_ctx* c, int64_t first_bin, uint32_t bins, const f
   if (bins == 0 || bins % 4 != 0) return fail(c, SPK_ERR_INVALID, "IFFT size must be a positive multiple of 4");
   if ((int)bins > c->N || (!find_chan_plan((int)bins) && (int)bins > spk::kGenBigMaxL))
     return fail(c, SPK_ERR_UNSUPPORTED,
-                "IFFT size must be <= N and a power of two up to 8192 or another multiple of 4 up to 4096");
+                "IFFT size must be <= N and at most 20480 (powers of two: 8192, or 16384)");
   Channel ch;
   ch.first = first_bin;
   ch.L = (int)bins;
