@@ -110,8 +110,7 @@ const char* spk_last_error(const spk_ctx* ctx);
  * other multiples of 4 up to 4096 in a mixed-radix kernel with several
  * channels per workgroup, and every multiple of 4 up to 20480 in a
  * mixed-radix kernel with one channel per workgroup; larger ones return
- * SPK_ERR_UNSUPPORTED (spk_channels_adjoint: sizes up to 8192, other than
- * powers of two up to 4096).  A channel gives 6 * (L / 8) samples per block
+ * SPK_ERR_UNSUPPORTED.  A channel gives 6 * (L / 8) samples per block
  * (the integer slice buf[L/8 .. L/8*7] of fcfb.rs:176-177: 3L/4 when 8 | L,
  * none for L = 4);
  * weights: L floats in j order (weights[j] multiplies bin first + j), or NULL

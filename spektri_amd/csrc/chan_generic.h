@@ -305,4 +305,91 @@ __global__ __launch_bounds__(kGenThreads) void chan_generic_adjoint_kernel(GenAd
   }
 }
 
+// Adjoint of the wide channels (chan_generic_big_kernel's sizes): one
+// (channel, block split) tile per workgroup of kGenBigThreads threads over
+// one static LDS buffer; each stage computes a thread's outputs into
+// registers, waits and writes them back (forward sign: conjugated
+// twiddles).  Thread t keeps the weight sums of bins t + kGenBigThreads u.
+__global__ __launch_bounds__(kGenBigThreads) void chan_generic_big_adjoint_kernel(GenAdjParams P) {
+  __shared__ cpx buf[kGenBigMaxL];
+  constexpr int U = kGenBigMaxL / kGenBigThreads;
+  const int L = P.L;
+  const int t = (int)threadIdx.x;
+  const int c = (int)blockIdx.x / P.S, split = (int)blockIdx.x % P.S;  // grid = nch * S exactly
+  const ChanDesc d = P.chans[c];
+  const float* W = P.weights + d.w_off;
+  const int lo = L / 8, n_out = 6 * (L / 8);
+  float wacc[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) wacc[u] = 0.f;
+  for (int g = split; g < P.nblocks; g += P.S) {  // the same trip count for the whole workgroup
+    const cpx* yb = P.ybar + (long long)P.nblocks * P.pk_off[c] + (long long)g * n_out;
+    __syncthreads();
+    for (int i = t; i < L; i += kGenBigThreads) buf[i] = (i >= lo && i < lo + n_out) ? yb[i - lo] : cpx{0.f, 0.f};
+    int m = 1;
+    for (int s = 0; s < P.nstages; ++s) {
+      __syncthreads();
+      const int p = P.radix[s];
+      const int Lp = L / p;
+      const int mp = m * p;
+      const int scale = L / mp;
+      cpx val[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int o = t + kGenBigThreads * u;
+        if (o >= L) break;
+        const int k = o % m;
+        const int r = o / m;
+        const int q = r % p;
+        const int base = (r / p) * m + k;
+        const int step = (k + q * m) * scale;
+        cpx acc = buf[base];
+        int e = step;
+        for (int j = 1; j < p; ++j) {
+          const cpx v = buf[base + j * Lp];
+          const cpx w = P.tw[e];  // conj(w) below: forward sign
+          acc.x += v.x * w.x + v.y * w.y;
+          acc.y += v.y * w.x - v.x * w.y;
+          e += step;
+          if (e >= L) e -= L;
+        }
+        val[u] = acc;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int o = t + kGenBigThreads * u;
+        if (o >= L) break;
+        buf[o] = val[u];
+      }
+      m = mp;
+    }
+    __syncthreads();
+    const cpx* X = P.spec + (long long)g * P.spec_stride;
+    cpx* xb = P.xbar ? P.xbar + (long long)P.nblocks * d.w_off + (long long)g * L : nullptr;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = t + kGenBigThreads * u;
+      if (i >= L) break;
+      int j = i + L / 2;
+      if (j >= L) j -= L;
+      int mm = d.f_mod + j;
+      if (mm >= P.N) mm -= P.N;
+      const bool mirror = P.real_input && mm >= P.K;
+      cpx G = X[mirror ? P.N - mm : mm];
+      if (mirror) G.y = -G.y;
+      wacc[u] += G.x * buf[i].x + G.y * buf[i].y;
+      if (xb) xb[j] = cscale(buf[i], W[j]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int i = t + kGenBigThreads * u;
+    if (i >= L) break;
+    int j = i + L / 2;
+    if (j >= L) j -= L;
+    P.wpart[((long long)c * P.S + split) * L + j] = wacc[u];
+  }
+}
+
 }  // namespace spk

@@ -1880,6 +1880,44 @@ spk_status spk_channels_output_packed(spk_ctx* c, float* dst, size_t cap) {
 // (chan_generic_adjoint_kernel, then the split sums in order).
 spk_status generic_adjoint(spk_ctx* c, const ChanGroup& g, int nb, const float* ybar, float* wbar, float* xbar) {
   const int nch = (int)g.ids.size();
+  if (g.L > spk::kGenMaxL) {
+    // wide channels: one (channel, split) tile per 1024-thread workgroup
+    const int S = std::max(1, std::min((2 * c->ncu + nch - 1) / nch, nb));
+    const size_t need = (size_t)nch * S * g.L * sizeof(float);
+    if (c->adj_part.bytes < need) HIP_TRY(c, c->adj_part.alloc(need));
+    spk::GenAdjParams Q{};
+    Q.spec = c->spec.as<cpx>();
+    Q.spec_stride = c->spec_stride;
+    Q.N = c->N;
+    Q.K = c->K;
+    Q.real_input = c->complex_in ? 0 : 1;
+    Q.weights = c->weights.as<float>();
+    Q.chans = g.desc.as<spk::ChanDesc>();
+    Q.pk_off = g.pk_off.as<long long>();
+    Q.nch = nch;
+    Q.nblocks = nb;
+    Q.S = S;
+    Q.ybar = reinterpret_cast<const cpx*>(ybar);
+    Q.xbar = reinterpret_cast<cpx*>(xbar);
+    Q.wpart = c->adj_part.as<float>();
+    Q.tw = g.gtw.as<cpx>();
+    Q.L = g.L;
+    Q.nstages = (int)g.radix.size();
+    for (size_t i = 0; i < g.radix.size(); ++i) Q.radix[i] = g.radix[i];
+    Q.tile_threads = spk::kGenBigThreads;
+    hipLaunchKernelGGL(spk::chan_generic_big_adjoint_kernel, dim3((unsigned)(nch * S)), dim3(spk::kGenBigThreads), 0,
+                       c->stream, Q);
+    HIP_TRY(c, hipGetLastError());
+    spk::AdjReduceParams Rp{};
+    Rp.wpart = c->adj_part.as<float>();
+    Rp.chans = g.desc.as<spk::ChanDesc>();
+    Rp.L = g.L;
+    Rp.S = S;
+    Rp.wbar = wbar;
+    hipLaunchKernelGGL(spk::adjoint_reduce_kernel, dim3((g.L + 255) / 256, nch), dim3(256), 0, c->stream, Rp);
+    HIP_TRY(c, hipGetLastError());
+    return SPK_OK;
+  }
   int T = 32;
   while (T < 256 && 2 * T < g.L) T *= 2;  // as launch_generic
   const int tpw = spk::kGenThreads / T;
@@ -1928,9 +1966,6 @@ spk_status spk_channels_adjoint(spk_ctx* c, const float* ybar, float* wbar, floa
   if (!c->have_output || c->chans_dirty) return fail(c, SPK_ERR_STATE, "no output yet");
   if (!c->keep_spec) return fail(c, SPK_ERR_STATE, "the adjoint needs a context created with SPK_FLAG_KEEP_SPECTRA");
   const int nb = (int)c->last_blocks;
-  for (const auto& kv : c->groups)
-    if (kv.second.generic && kv.second.L > spk::kGenMaxL)
-      return fail(c, SPK_ERR_UNSUPPORTED, "the adjoint takes IFFT sizes up to 8192 (other than powers of two: 4096)");
   for (auto& kv : c->groups) {
     ChanGroup& g = kv.second;
     const int nch = (int)g.ids.size();

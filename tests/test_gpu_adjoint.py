@@ -56,13 +56,16 @@ CHANNEL_SETS = {
     # IFFT sizes without a power-of-two plan (L = 100 and 12: 6 (L/8) < 3L/4)
     # between power-of-two ones, so the packed offsets mix both
     "generic": [(-20, 48), (100, 256), (1900, 100), (500, 300), (148, 32), (2040, 12)],
+    # wide channels (one workgroup per tile): L = 5000 and the power of two 16384 at N = 16384
+    "wide": [(-20, 48), (100, 5000), (1900, 100), (500, 16384), (148, 32), (2040, 12)],
 }
+FFT_SIZE = {"pow2": 4096, "generic": 4096, "wide": 16384}
 
 
 @pytest.mark.parametrize("kind", sorted(CHANNEL_SETS))
 def test_adjoint_mixed_sizes_matches_fp64_autograd_and_fd(kind):
     rng = np.random.default_rng(11)
-    N, F = 4096, 8
+    N, F = FFT_SIZE[kind], 8
     H = N - N // 4
     x = np.clip(np.round(rng.standard_normal(2 * F * H) * 3000
                          + 8000 * np.cos(2 * np.pi * 0.1234 * np.arange(2 * F * H))), -32768, 32767).astype(np.int16)
