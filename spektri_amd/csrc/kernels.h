@@ -1158,8 +1158,8 @@ __device__ __forceinline__ unsigned char metadata_byte(int o, unsigned long long
 }
 
 // Welch records as the reference sends them (spectrum.rs:120-153): per
-// record 24 bytes of metadata followed by the quantised body.  One thread
-// per output byte.
+// record 24 bytes of metadata followed by the quantised body.  Grid: x over
+// the bytes of a record, y over records (strided).
 struct RecordPackParams {
   const unsigned char* body;  // nrec bodies of body_bytes each
   unsigned char* out;         // nrec records of 24 + body_bytes
@@ -1171,12 +1171,12 @@ struct RecordPackParams {
 
 __global__ void pack_records_kernel(RecordPackParams P) {
   const long long rec = 24 + P.body_bytes;
-  const long long total = rec * P.nrec;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (long long)gridDim.x * blockDim.x) {
-    const long long r = i / rec, o = i % rec;
-    P.out[i] = o >= 24 ? P.body[r * P.body_bytes + (o - 24)]
-                       : metadata_byte((int)o, P.first_seq + (unsigned long long)r, P.secs, P.nanos);
+  for (int r = (int)blockIdx.y; r < P.nrec; r += (int)gridDim.y) {
+    unsigned char* out = P.out + (long long)r * rec;
+    const unsigned char* body = P.body + (long long)r * P.body_bytes;
+    for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < rec;
+         o += (long long)gridDim.x * blockDim.x)
+      out[o] = o >= 24 ? body[o - 24] : metadata_byte((int)o, P.first_seq + (unsigned long long)r, P.secs, P.nanos);
   }
 }
 

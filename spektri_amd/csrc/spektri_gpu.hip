@@ -1726,8 +1726,9 @@ spk_status spk_spectrum_records_packed(spk_ctx* c, uint64_t first_seq, uint64_t 
   P.first_seq = first_seq;
   P.secs = secs;
   P.nanos = nanos;
-  const unsigned grid = (unsigned)std::min<long long>(((long long)need + 255) / 256, (long long)c->ncu * 8);
-  hipLaunchKernelGGL(spk::pack_records_kernel, dim3(grid), dim3(256), 0, c->stream, P);
+  const unsigned gx = (unsigned)std::min<long long>((24 + body + 255) / 256, 64);
+  const unsigned gy = std::min<uint32_t>(n, 65535u);
+  hipLaunchKernelGGL(spk::pack_records_kernel, dim3(gx, gy), dim3(256), 0, c->stream, P);
   HIP_TRY(c, hipGetLastError());
   HIP_TRY(c, hipMemcpyAsync(out, c->qpack.p, need, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
