@@ -26,7 +26,8 @@ enum ConvKind : int {
 struct ConvParams {
   const uint32_t* src;  // raw stream, 4-byte aligned
   void* dst;            // converted stream (device buffer)
-  long long words;      // raw 32-bit words to convert
+  long long words;      // whole raw 32-bit words to convert
+  int tail;             // raw bytes after them (1-byte kinds only, 0..3), read one by one
   int kind;
 };
 
@@ -37,9 +38,21 @@ __device__ __forceinline__ uint32_t bswap16x2(uint32_t v) {
 }
 
 // One raw word per thread per iteration (a wave reads 256 contiguous bytes);
-// the output of word i starts at byte i * (output bytes per raw word).
+// the output of word i starts at byte i * (output bytes per raw word).  The
+// 1-3 bytes of a stream of 1-byte numbers that end inside a word are read
+// byte by byte (never past the stream) by the first thread.
 __global__ void convert_kernel(ConvParams P) {
   const long long stride = (long long)gridDim.x * blockDim.x;
+  if (P.tail > 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+    const uint8_t* b = reinterpret_cast<const uint8_t*>(P.src + P.words);
+    for (int j = 0; j < P.tail; ++j) {
+      const long long o = 4 * P.words + j;
+      const uint32_t v = b[j];
+      if (P.kind == kConvS8ToS16) static_cast<int16_t*>(P.dst)[o] = (int16_t)(int8_t)(uint8_t)v;
+      else if (P.kind == kConvU8ToF32) static_cast<float*>(P.dst)[o] = u8_sample(v);
+      else if (P.kind == kConvS8ToF32) static_cast<float*>(P.dst)[o] = s8_sample(v);
+    }
+  }
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < P.words; i += stride) {
     const uint32_t v = P.src[i];
     switch (P.kind) {

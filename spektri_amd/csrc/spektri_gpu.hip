@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <new>
 #include <string>
@@ -35,6 +36,23 @@ const char* kNullCtxMsg = "null context";
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
   ~DevBuf() {
     if (p) (void)hipFree(p);
   }
@@ -213,7 +231,9 @@ const FormatInfo* format_info(uint32_t fmt) {
 }  // namespace
 
 // Batched channel stage plan (chan_batched_kernel): classes, class-sorted
-// descriptors, and the per-XCD item queues for one block count.
+// descriptors, and the per-XCD item queues for one block count.  A captured
+// graph holds the plan's device pointers, so plans are kept per block count
+// and freed only together with every graph (drop_graphs_and_plans).
 struct BatchPlan {
   std::vector<int> cls_L;
   DevBuf descs, items, item_off;
@@ -222,6 +242,7 @@ struct BatchPlan {
   int maxL = 0;
   size_t nitems = 0;
 };
+constexpr size_t kMaxCachedPlans = 8;  // graphs and batch plans kept at once
 constexpr int kXcdQueues = 8;  // MI355X: 8 XCDs; workgroup w is dispatched to XCD w mod 8
 
 struct spk_ctx {
@@ -280,7 +301,7 @@ struct spk_ctx {
   // channel stage launch form and batched tuning (chunk blocks, rounds per item, workgroups per CU)
   int chan_mode = SPK_CHAN_PER_SIZE;
   int tune_chunk = 8, tune_rounds = 2, tune_wgs_per_cu = 0;
-  BatchPlan batch;
+  std::map<int, BatchPlan> batches;  // per block count
   DevBuf chan_counters;      // kXcdQueues ints
   DevBuf pos;                // device stream position (blocks consumed), two long long slots
   int parity = 0;            // slot holding this call's position; the Welch kernel writes the other
@@ -339,6 +360,28 @@ spk_status hip_fail(spk_ctx* c, hipError_t e, const char* what) {
   return fail(c, SPK_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Every extern "C" entry point ends in this handler: a C++ exception (host
+// allocation in std containers, std::thread) becomes a status code and
+// never crosses the C boundary.
+spk_status entry_exception(spk_ctx* c) noexcept {
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    if (c) c->err = "host allocation failed";
+  } catch (const std::exception& ex) {
+    try {
+      if (c) c->err = ex.what();
+    } catch (...) {
+    }
+  } catch (...) {
+    try {
+      if (c) c->err = "unexpected C++ exception";
+    } catch (...) {
+    }
+  }
+  return SPK_ERR_DEVICE;
+}
+
 #define HIP_TRY(ctx, expr)                              \
   do {                                                  \
     hipError_t _e = (expr);                             \
@@ -362,11 +405,24 @@ std::vector<float> raised_cosine_weights(int L) {
 
 int fused_lds_dummy(int M);
 
+void drop_graphs(spk_ctx* c) {
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+}
+// Graphs hold device pointers of the batch plans: free plans only with them.
+void drop_graphs_and_plans(spk_ctx* c) {
+  drop_graphs(c);
+  c->batches.clear();
+}
+
 // Lay out channel outputs and descriptors, upload weights and tables.
+// Every captured graph and batch plan refers to the old buffers and is
+// dropped here.
 spk_status rebuild_channels(spk_ctx* c) {
   // Tables are uploaded with null-stream copies, which do not wait for the
   // context stream: no launch of an earlier call may still read them.
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  drop_graphs_and_plans(c);
   std::vector<float> wall;
   c->chan_out_off.assign(c->chans.size(), -1);
   c->chan_w_off.assign(c->chans.size(), -1);
@@ -486,7 +542,6 @@ spk_status rebuild_channels(spk_ctx* c) {
       c->fscat_nyq = nyq;
     }
   }
-  c->batch.nb = -1;
   c->chans_dirty = false;
   return SPK_OK;
 }
@@ -674,7 +729,8 @@ hipError_t launch_generic(const spk_ctx* c, const ChanGroup& g, int nb, hipStrea
 // Cut the (block, channel) tiles of nb blocks into per-XCD item queues.
 spk_status build_batch(spk_ctx* c, int nb) {
   HIP_TRY(c, hipStreamSynchronize(c->stream));  // as in rebuild_channels
-  BatchPlan& B = c->batch;
+  if (c->batches.size() >= kMaxCachedPlans && !c->batches.count(nb)) drop_graphs_and_plans(c);
+  BatchPlan& B = c->batches[nb];
   std::vector<int> ls = batched_groups(c);
   std::sort(ls.begin(), ls.end(), std::greater<int>());  // big tiles first within a chunk
   B.cls_L = ls;
@@ -730,11 +786,11 @@ spk_status build_batch(spk_ctx* c, int nb) {
 }
 
 spk_status launch_batched(spk_ctx* c, int nb, hipStream_t s) {
-  if (c->batch.nb != nb) {
+  if (!c->batches.count(nb)) {  // (graph capture builds the plan first, in run_blocks)
     spk_status st = build_batch(c, nb);
     if (st != SPK_OK) return st;
   }
-  const BatchPlan& B = c->batch;
+  const BatchPlan& B = c->batches.at(nb);
   if (B.cls_L.empty()) return SPK_OK;
   spk::BatchParams P{};
   P.spec = c->spec.as<cpx>();
@@ -808,9 +864,11 @@ const void* enqueue_convert(spk_ctx* c, const void* raw, int nb, hipStream_t s) 
   spk::ConvParams P{};
   P.src = static_cast<const uint32_t*>(raw);
   P.dst = c->conv.p;
-  P.words = (long long)nb * c->H * c->raw_bps / 4;
+  const long long bytes = (long long)nb * c->H * c->raw_bps;
+  P.words = bytes / 4;
+  P.tail = (int)(bytes % 4);
   P.kind = c->conv_kind;
-  const long long want = (P.words + 255) / 256;
+  const long long want = (P.words + 255) / 256 + (P.tail ? 1 : 0);
   const int grid = (int)std::min<long long>(want, (long long)c->ncu * 8);
   hipLaunchKernelGGL(spk::convert_kernel, dim3(grid), dim3(256), 0, s, P);
   return c->conv.p;
@@ -1153,11 +1211,6 @@ spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hi
   return SPK_OK;
 }
 
-void drop_graphs(spk_ctx* c) {
-  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
-  c->graphs.clear();
-}
-
 // Whether a call's launch sequence is worth a graph: with channel launches
 // after the forward transform.  A real-input call whose channels all run
 // inside the forward kernel is two launches (forward, finaliser), and
@@ -1173,7 +1226,6 @@ bool graph_pays(const spk_ctx* c) {
 // once and replayed when it has channel launches (graph_pays).
 spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
   if (c->chans_dirty) {
-    drop_graphs(c);
     spk_status st = rebuild_channels(c);
     if (st != SPK_OK) return st;
   }
@@ -1183,7 +1235,7 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
       if (c->graphs.size() >= 8) drop_graphs(c);
-      if (c->chan_mode == SPK_CHAN_BATCHED && c->batch.nb != nb) {
+      if (c->chan_mode == SPK_CHAN_BATCHED && !c->batches.count(nb)) {
         spk_status st = build_batch(c, nb);  // uploads happen outside the capture
         if (st != SPK_OK) return st;
       }
@@ -1243,6 +1295,7 @@ extern "C" {
 const char* spk_last_error(const spk_ctx* ctx) { return ctx ? ctx->err.c_str() : kNullCtxMsg; }
 
 spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
+  try {
   if (!cfg || !out) return SPK_ERR_INVALID;
   *out = nullptr;
   spk_ctx* c = new (std::nothrow) spk_ctx();
@@ -1451,9 +1504,13 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   if (hipDeviceSynchronize() != hipSuccess) return cleanup(SPK_ERR_DEVICE);
   *out = c;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(nullptr);
+  }
 }
 
 spk_status spk_destroy(spk_ctx* c) {
+  try {
   if (!c) return SPK_ERR_INVALID;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->copy_stream) {
@@ -1470,17 +1527,25 @@ spk_status spk_destroy(spk_ctx* c) {
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_buffer_size_of(const spk_ctx* c, spk_buffer_size* out) {
+  try {
   if (!c || !out) return SPK_ERR_INVALID;
   out->overlap = (uint64_t)c->O;
   out->new_samples = (uint64_t)c->F * c->H;
   out->total = out->overlap + out->new_samples;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(const_cast<spk_ctx*>(c));
+  }
 }
 
 spk_status spk_add_channel(spk_ctx* c, int64_t first_bin, uint32_t bins, const float* weights, uint32_t* id) {
+  try {
   if (!c) return SPK_ERR_INVALID;
   if (bins == 0 || bins % 4 != 0) return fail(c, SPK_ERR_INVALID, "IFFT size must be a positive multiple of 4");
   if ((int)bins > c->N || (!find_chan_plan((int)bins) && (int)bins > spk::kGenBigMaxL))
@@ -1494,27 +1559,39 @@ spk_status spk_add_channel(spk_ctx* c, int64_t first_bin, uint32_t bins, const f
   c->chans_dirty = true;
   if (id) *id = (uint32_t)(c->chans.size() - 1);
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_remove_channel(spk_ctx* c, uint32_t id) {
+  try {
   if (!c) return SPK_ERR_INVALID;
   if (id >= c->chans.size() || c->chans[id].removed) return fail(c, SPK_ERR_INVALID, "no such channel");
   c->chans[id].removed = true;
   c->chans[id].w.clear();
   c->chans_dirty = true;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_set_channel_weights(spk_ctx* c, uint32_t id, const float* weights) {
+  try {
   if (!c || !weights) return SPK_ERR_INVALID;
   if (id >= c->chans.size() || c->chans[id].removed) return fail(c, SPK_ERR_INVALID, "no such channel");
   Channel& ch = c->chans[id];
   ch.w.assign(weights, weights + ch.L);
   c->chans_dirty = true;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_process(spk_ctx* c, const void* raw_new, size_t nbytes) {
+  try {
   if (!c || !raw_new) return SPK_ERR_INVALID;
   const size_t want = (size_t)c->F * c->H * c->raw_bps;
   if (nbytes != want) return fail(c, SPK_ERR_INVALID, "buffer must hold exactly F * 3N/4 samples");
@@ -1524,6 +1601,9 @@ spk_status spk_process(spk_ctx* c, const void* raw_new, size_t nbytes) {
   if (st != SPK_OK) return st;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 }  // extern "C"
@@ -1569,6 +1649,7 @@ spk_status async_setup(spk_ctx* c) {
 extern "C" {
 
 spk_status spk_input_buffer(spk_ctx* c, void** host_ptr, size_t* nbytes) {
+  try {
   if (!c || !host_ptr) return SPK_ERR_INVALID;
   {
     spk_status st = async_setup(c);
@@ -1579,9 +1660,13 @@ spk_status spk_input_buffer(spk_ctx* c, void** host_ptr, size_t* nbytes) {
   *host_ptr = c->apin[s];
   if (nbytes) *nbytes = (size_t)c->F * c->H * c->raw_bps;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_process_async(spk_ctx* c, const void* raw_new, size_t nbytes) {
+  try {
   if (!c || !raw_new) return SPK_ERR_INVALID;
   const size_t want = (size_t)c->F * c->H * c->raw_bps;
   if (nbytes != want) return fail(c, SPK_ERR_INVALID, "buffer must hold exactly F * 3N/4 samples");
@@ -1607,9 +1692,13 @@ spk_status spk_process_async(spk_ctx* c, const void* raw_new, size_t nbytes) {
   HIP_TRY(c, hipEventRecord(c->a_used[s], c->stream));
   c->aslot ^= 1;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_process_device(spk_ctx* c, const void* d_raw, uint64_t nsamples) {
+  try {
   if (!c || !d_raw) return SPK_ERR_INVALID;
   if (reinterpret_cast<uintptr_t>(d_raw) % 4 != 0) return fail(c, SPK_ERR_INVALID, "input must be 4-byte aligned");
   const uint64_t per_buf = (uint64_t)c->F * c->H;
@@ -1617,24 +1706,36 @@ spk_status spk_process_device(spk_ctx* c, const void* d_raw, uint64_t nsamples) 
   if (nb == 0) return fail(c, SPK_ERR_INVALID, "fewer samples than one buffer");
   if (nb > (uint64_t)c->max_blocks) return fail(c, SPK_ERR_CAPACITY, "more blocks than max_blocks");
   return run_blocks(c, d_raw, (int)nb);
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_last_blocks(const spk_ctx* c, uint32_t* nblocks) {
+  try {
   if (!c || !nblocks) return SPK_ERR_INVALID;
   *nblocks = c->last_blocks;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(const_cast<spk_ctx*>(c));
+  }
 }
 
 spk_status spk_channel_output_device(spk_ctx* c, uint32_t id, const float** dptr, size_t* n) {
+  try {
   if (!c || !dptr || !n) return SPK_ERR_INVALID;
   if (id >= c->chans.size() || c->chans[id].removed) return fail(c, SPK_ERR_INVALID, "no such channel");
   if (!c->have_output || c->chans_dirty) return fail(c, SPK_ERR_STATE, "no output yet");
   *dptr = reinterpret_cast<const float*>(c->chan_out.as<cpx>() + c->chan_out_off[id]);
   *n = (size_t)c->last_blocks * chan_out_len(c->chans[id].L) * 2;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_channel_output(spk_ctx* c, uint32_t id, float* out, size_t cap, size_t* n) {
+  try {
   if (!c || !out || !n) return SPK_ERR_INVALID;
   const float* d = nullptr;
   spk_status st = spk_channel_output_device(c, id, &d, n);
@@ -1643,9 +1744,13 @@ spk_status spk_channel_output(spk_ctx* c, uint32_t id, float* out, size_t cap, s
   HIP_TRY(c, hipMemcpyAsync(out, d, *n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_spectrum_records_device(spk_ctx* c, const float** dptr, uint32_t* nrec) {
+  try {
   if (!c || !dptr || !nrec) return SPK_ERR_INVALID;
   {
     spk_status st = flush_records(c);
@@ -1654,9 +1759,13 @@ spk_status spk_spectrum_records_device(spk_ctx* c, const float** dptr, uint32_t*
   *dptr = c->welch_out.as<float>();
   *nrec = c->have_output ? c->last_records : 0;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_spectrum_records(spk_ctx* c, float* out, size_t cap, uint32_t* nrec) {
+  try {
   if (!c || !nrec) return SPK_ERR_INVALID;
   const float* d = nullptr;
   spk_status st = spk_spectrum_records_device(c, &d, nrec);
@@ -1667,9 +1776,13 @@ spk_status spk_spectrum_records(spk_ctx* c, float* out, size_t cap, uint32_t* nr
   HIP_TRY(c, hipMemcpyAsync(out, d, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_spectrum_quantized_device(spk_ctx* c, const void** dptr, uint32_t* nrec, uint32_t* bytes_per_bin) {
+  try {
   if (!c || !dptr || !nrec || !bytes_per_bin) return SPK_ERR_INVALID;
   if (!c->qfmt) return fail(c, SPK_ERR_STATE, "context created without SPK_FLAG_SPECTRUM_U8 / _U16");
   {
@@ -1680,9 +1793,13 @@ spk_status spk_spectrum_quantized_device(spk_ctx* c, const void** dptr, uint32_t
   *nrec = c->have_output ? c->last_records : 0;
   *bytes_per_bin = (uint32_t)c->qfmt;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_spectrum_quantized(spk_ctx* c, void* out, size_t cap, uint32_t* nrec) {
+  try {
   if (!c || !nrec) return SPK_ERR_INVALID;
   if (!c->qfmt) return fail(c, SPK_ERR_STATE, "context created without SPK_FLAG_SPECTRUM_U8 / _U16");
   {
@@ -1699,10 +1816,14 @@ spk_status spk_spectrum_quantized(spk_ctx* c, void* out, size_t cap, uint32_t* n
   }
   *nrec = n;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_spectrum_records_packed(spk_ctx* c, uint64_t first_seq, uint64_t secs, uint32_t nanos, void* out,
                                        size_t cap, uint32_t* nrec, size_t* nbytes) {
+  try {
   if (!c || !nrec || !nbytes) return SPK_ERR_INVALID;
   if (!c->qfmt) return fail(c, SPK_ERR_STATE, "context created without SPK_FLAG_SPECTRUM_U8 / _U16");
   {
@@ -1733,10 +1854,14 @@ spk_status spk_spectrum_records_packed(spk_ctx* c, uint64_t first_seq, uint64_t 
   HIP_TRY(c, hipMemcpyAsync(out, c->qpack.p, need, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_channel_records_packed(spk_ctx* c, uint64_t seq, uint64_t secs, uint32_t nanos, void* out, size_t cap,
                                       uint32_t* nrec, size_t* nbytes) {
+  try {
   if (!c || !nrec || !nbytes) return SPK_ERR_INVALID;
   if (!c->have_output || c->chans_dirty) return fail(c, SPK_ERR_STATE, "no output yet");
   const int nb = (int)c->last_blocks;
@@ -1766,16 +1891,24 @@ spk_status spk_channel_records_packed(spk_ctx* c, uint64_t seq, uint64_t secs, u
   HIP_TRY(c, hipMemcpyAsync(out, c->cpack.p, need, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_spectra_device(spk_ctx* c, const float** dptr, uint64_t* stride) {
+  try {
   if (!c || !dptr || !stride) return SPK_ERR_INVALID;
   *dptr = c->spec.as<float>();
   *stride = (uint64_t)c->spec_stride;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_reset(spk_ctx* c) {
+  try {
   if (!c) return SPK_ERR_INVALID;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   // On the context's stream: it does not wait for the null stream.
@@ -1789,9 +1922,13 @@ spk_status spk_reset(spk_ctx* c) {
   c->last_records = 0;
   c->have_output = false;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_set_stream(spk_ctx* c, void* stream) {
+  try {
   if (!c) return SPK_ERR_INVALID;
   {
     spk_status st = flush_records(c);
@@ -1801,22 +1938,34 @@ spk_status spk_set_stream(spk_ctx* c, void* stream) {
   c->stream = stream ? reinterpret_cast<hipStream_t>(stream) : c->own_stream;
   drop_graphs(c);
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_synchronize(spk_ctx* c) {
+  try {
   if (!c) return SPK_ERR_INVALID;
   spk_status st = flush_records(c);
   if (st != SPK_OK) return st;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_flush(spk_ctx* c) {
+  try {
   if (!c) return SPK_ERR_INVALID;
   return flush_records(c);
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_enable_timing(spk_ctx* c, int on) {
+  try {
   if (!c) return SPK_ERR_INVALID;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   c->timing = on != 0;
@@ -1824,6 +1973,9 @@ spk_status spk_enable_timing(spk_ctx* c, int on) {
   c->timed_calls = 0;
   for (double& v : c->ms_sum) v = 0;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 size_t packed_weight_floats(const spk_ctx* c) {
@@ -1834,6 +1986,7 @@ size_t packed_weight_floats(const spk_ctx* c) {
 }
 
 spk_status spk_set_weights_device(spk_ctx* c, const float* w) {
+  try {
   if (!c || !w) return SPK_ERR_INVALID;
   if (c->chans.empty()) return fail(c, SPK_ERR_STATE, "no channels");
   if (c->chans_dirty) {
@@ -1853,9 +2006,13 @@ spk_status spk_set_weights_device(spk_ctx* c, const float* w) {
     off += (size_t)ch.L;
   }
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_channels_output_packed(spk_ctx* c, float* dst, size_t cap) {
+  try {
   if (!c || !dst) return SPK_ERR_INVALID;
   if (!c->have_output || c->chans_dirty) return fail(c, SPK_ERR_STATE, "no output yet");
   const size_t need = 2 * (size_t)c->last_blocks * (size_t)c->live_len;
@@ -1875,6 +2032,9 @@ spk_status spk_channels_output_packed(spk_ctx* c, float* dst, size_t cap) {
     HIP_TRY(c, hipGetLastError());
   }
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 // Adjoint of one group of IFFT sizes without a power-of-two plan
@@ -1963,6 +2123,7 @@ spk_status generic_adjoint(spk_ctx* c, const ChanGroup& g, int nb, const float* 
 }
 
 spk_status spk_channels_adjoint(spk_ctx* c, const float* ybar, float* wbar, float* xbar) {
+  try {
   if (!c || !ybar || !wbar) return SPK_ERR_INVALID;
   if (!c->have_output || c->chans_dirty) return fail(c, SPK_ERR_STATE, "no output yet");
   if (!c->keep_spec) return fail(c, SPK_ERR_STATE, "the adjoint needs a context created with SPK_FLAG_KEEP_SPECTRA");
@@ -2010,17 +2171,25 @@ spk_status spk_channels_adjoint(spk_ctx* c, const float* ybar, float* wbar, floa
     HIP_TRY(c, hipGetLastError());
   }
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_set_channel_mode(spk_ctx* c, int mode) {
+  try {
   if (!c || (mode != SPK_CHAN_PER_SIZE && mode != SPK_CHAN_BATCHED)) return SPK_ERR_INVALID;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   c->chan_mode = mode;
   drop_graphs(c);
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_autotune(spk_ctx* c, uint32_t nblocks, spk_tune_result* out) {
+  try {
   if (!c) return SPK_ERR_INVALID;
   if (c->chans_dirty) {
     spk_status st = rebuild_channels(c);
@@ -2030,6 +2199,9 @@ spk_status spk_autotune(spk_ctx* c, uint32_t nblocks, spk_tune_result* out) {
   if (nb > c->max_blocks || nb > (int)std::max<long long>(c->max_blocks, 1)) return fail(c, SPK_ERR_CAPACITY, "nblocks");
   if (batched_groups(c).empty()) return fail(c, SPK_ERR_STATE, "no channel needs the channel stage");
   hipStream_t s = c->stream;
+  HIP_TRY(c, hipStreamSynchronize(s));
+  // the plans are cut again for every setting tried
+  drop_graphs_and_plans(c);
   hipEvent_t e0, e1;
   HIP_TRY(c, hipEventCreate(&e0));
   HIP_TRY(c, hipEventCreate(&e1));
@@ -2080,7 +2252,7 @@ spk_status spk_autotune(spk_ctx* c, uint32_t nblocks, spk_tune_result* out) {
         c->tune_chunk = ch;
         c->tune_rounds = rd;
         c->tune_wgs_per_cu = w;
-        c->batch.nb = -1;
+        c->batches.clear();
         const float ms = time_it([&]() { return launch_batched(c, nb, s); });
         if (ms < 0.f) {
           (void)hipEventDestroy(e0);
@@ -2101,9 +2273,8 @@ spk_status spk_autotune(spk_ctx* c, uint32_t nblocks, spk_tune_result* out) {
   c->tune_chunk = bc;
   c->tune_rounds = br;
   c->tune_wgs_per_cu = bw;
-  c->batch.nb = -1;
   c->chan_mode = best < per_size ? SPK_CHAN_BATCHED : SPK_CHAN_PER_SIZE;
-  drop_graphs(c);
+  drop_graphs_and_plans(c);
   if (out) {
     out->per_size_ms = per_size;
     out->batched_ms = best;
@@ -2113,9 +2284,13 @@ spk_status spk_autotune(spk_ctx* c, uint32_t nblocks, spk_tune_result* out) {
     out->mode = c->chan_mode;
   }
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_fused_channels(spk_ctx* c, uint32_t* n) {
+  try {
   if (!c || !n) return SPK_ERR_INVALID;
   if (c->chans_dirty) {
     spk_status st = rebuild_channels(c);
@@ -2123,9 +2298,13 @@ spk_status spk_fused_channels(spk_ctx* c, uint32_t* n) {
   }
   *n = c->fused ? (uint32_t)c->groups.at(kFusedL).ids.size() : 0u;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_last_timing(spk_ctx* c, float ms[3]) {
+  try {
   if (!c || !ms) return SPK_ERR_INVALID;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   for (; c->ev_done + 4 <= c->ev_used; c->ev_done += 4) {
@@ -2140,10 +2319,14 @@ spk_status spk_last_timing(spk_ctx* c, float ms[3]) {
   if (c->ev_done == c->ev_used) c->ev_used = c->ev_done = 0;  // recycle the pool
   for (int i = 0; i < 3; ++i) ms[i] = c->timed_calls ? (float)(c->ms_sum[i] / c->timed_calls) : 0.f;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
 }
 
 spk_status spk_freq_to_bins(double fs_in, double fc_in, uint32_t fft_size, double fs_out, double fc_out,
                             uint32_t* bins, int64_t* first) {
+  try {
   if (!bins || !first || fft_size == 0) return SPK_ERR_INVALID;
   const double spacing = fs_in / (double)fft_size;
   const int64_t b = (int64_t)(round_half_away(fs_out / spacing / 4.0) * 4.0);
@@ -2152,19 +2335,27 @@ spk_status spk_freq_to_bins(double fs_in, double fc_in, uint32_t fft_size, doubl
   *bins = (uint32_t)b;
   *first = center - b / 2;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(nullptr);
+  }
 }
 
 spk_status spk_bins_to_freq(double fs_in, double fc_in, uint32_t fft_size, uint32_t bins, int64_t first,
                             double* fs_out, double* fc_out) {
+  try {
   if (!fs_out || !fc_out || fft_size == 0) return SPK_ERR_INVALID;
   const double spacing = fs_in / (double)fft_size;
   *fs_out = spacing * (double)bins;
   *fc_out = fc_in + spacing * (double)(first + (int64_t)(bins / 2));
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(nullptr);
+  }
 }
 
 spk_status spk_freq_to_bins_exact(double fs_in, double fc_in, uint32_t fft_size, double fs_out, double fc_out,
                                   uint32_t* bins, int64_t* first) {
+  try {
   uint32_t b = 0;
   int64_t f = 0;
   spk_status st = spk_freq_to_bins(fs_in, fc_in, fft_size, fs_out, fc_out, &b, &f);
@@ -2175,6 +2366,9 @@ spk_status spk_freq_to_bins_exact(double fs_in, double fc_in, uint32_t fft_size,
   *bins = b;
   *first = f;
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(nullptr);
+  }
 }
 
 }  // extern "C"

@@ -144,3 +144,36 @@ def test_mixed_radix_graph_replays(name, N):
         X = oracle_blocks(N, stream, fmt, call * F, F)
         check_call(N, f"fft_size_graphs_{name}_N{N}_call{call}", st, ids, chans, X, acc, True)
     st.close()
+
+
+@pytest.mark.parametrize("name", ["u8", "s8"])
+def test_byte_formats_with_odd_block_counts(name):
+    """1-byte real formats at N = 1000 (H = 750) with three blocks per call:
+    each call converts 2250 bytes, so its last two samples end inside a
+    32-bit word.  Every sample must be converted (they reach the last block
+    and the history of the next call)."""
+    fmt = {"u8": O.FMT_U8, "s8": O.FMT_S8}[name]
+    N, Fo, Ao = 1000, 3, 2
+    H = N - N // 4
+    rng = np.random.default_rng(1000 + fmt)
+    chans = channels(N, False, rng)
+    per_buf = Fo * H
+    assert per_buf % 4 == 2
+    raw = raw_stream(fmt, 5 * per_buf, rng)
+    st, _ = DspState.init(DspParams(complex=False, fs_in=1.0, fft_size=N, ffts_per_buf=Fo, spectrum_averages=Ao,
+                                    input_format=fmt, keep_spectra=True, max_blocks=3 * Fo))
+    ids = [st.add_channel(f, L, w) for f, L, w in chans]
+    acc = O.WelchAccumulator(N // 2 + 1, N, Ao)
+    geom = O.Geometry(N, False, Fo)
+    x = O.convert(raw, fmt)
+    spectra = lambda first, nb: O.forward_spectra(O.frame_blocks(x, geom, nb, first), geom,
+                                                  O.input_format_scaling(fmt))
+    for call in range(2):
+        st.process(raw[call * per_buf:(call + 1) * per_buf])
+        check_call(N, f"odd_blocks_{name}_host_call{call}", st, ids, chans, spectra(call * Fo, Fo), acc, True)
+    # a device stream of one buffer plus one sample: the buffer ends inside a word
+    dev = torch.from_numpy(raw[2 * per_buf:3 * per_buf + 1].copy()).to("cuda")
+    st.process_device(dev.data_ptr(), per_buf + 1)
+    st.synchronize()
+    check_call(N, f"odd_blocks_{name}_device", st, ids, chans, spectra(2 * Fo, Fo), acc, True)
+    st.close()

@@ -175,7 +175,12 @@ def test_freq_to_bins_reference_kat_and_round_trip():
         for center in range(-1000, 1000, 52):
             fso, fco = d * bins, d * center
             assert O.freq_to_bins_exact(fs, 0.0, N, fso, fco) == (bins, center - bins // 2)
-    assert O.freq_to_bins_exact(fs, 0.0, N, d * 6, 0.0) is None or True
+    # 6 bins round to 8 (6/4 = 1.5), which maps back to another width: not exact
+    assert O.freq_to_bins(fs, 0.0, N, d * 6, 0.0) == (8, -4)
+    assert O.freq_to_bins_exact(fs, 0.0, N, d * 6, 0.0) is None
+    # 10/4 = 2.5 rounds half away from zero (Rust f64::round) to 3 -> 12 bins
+    assert O.freq_to_bins(fs, 0.0, N, d * 10, 0.0) == (12, -6)
+    assert O.freq_to_bins_exact(fs, 0.0, N, d * 10, 0.0) is None
     assert O.freq_to_bins(fs, 0.0, N, 1.0, 0.0) is None  # bins round to 0
 
 
