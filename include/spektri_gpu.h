@@ -285,6 +285,16 @@ spk_status spk_channels_output_packed(spk_ctx* ctx, float* dst, size_t cap);
  * respect to every gathered bin get_bin(first + j).  Needs a context created
  * with SPK_FLAG_KEEP_SPECTRA (SPK_ERR_STATE otherwise). */
 spk_status spk_channels_adjoint(spk_ctx* ctx, const float* ybar, float* wbar, float* xbar);
+/* spk_channels_adjoint without any device allocation: no counterpart in the
+ * reference.  It is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops that forbid hipMalloc).  The per-(channel, block
+ * split) weight sums go to one fixed 8 MiB scratch array per GPU, held
+ * until the process ends, instead of a context buffer grown on demand;
+ * SPK_ERR_CAPACITY when a channel group needs more than 2^21 floats there
+ * (nch * S * L, S the group's block splits).  Same kernels and results as
+ * spk_channels_adjoint.  Calls that overlap on one GPU (several contexts or
+ * streams) share the scratch and race; the caller orders them. */
+spk_status spk_channels_adjoint_noalloc(spk_ctx* ctx, const float* ybar, float* wbar, float* xbar);
 
 /* freq_to_bins / bins_to_freq / freq_to_bins_exact (spektri/src/dsp/fcfb.rs:225-285).
  * Return SPK_ERR_INVALID when the reference returns None. */

@@ -97,6 +97,7 @@ _SIGS = {
     "spk_autotune": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.POINTER(SpkTuneResult)]),
     "spk_channels_output_packed": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "spk_channels_adjoint": (ctypes.c_int, [_P, _P, _P, _P]),
+    "spk_channels_adjoint_noalloc": (ctypes.c_int, [_P, _P, _P, _P]),
     "spk_freq_to_bins": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_uint32,
                                         ctypes.c_double, ctypes.c_double,
                                         ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int64)]),

@@ -29,7 +29,33 @@
 
 using spk::cpx;
 
+// Scratch of spk_channels_adjoint_noalloc: the per-(channel, split) weight
+// sums that spk_channels_adjoint keeps in a context buffer it allocates
+// (adj_part).  Every GPU the library runs on holds one copy until the
+// process ends.  Host code never uses the name as an address: the device
+// address comes from hipGetSymbolAddress, once per device (scratch_address).
+constexpr size_t kNoAllocScratchFloats = size_t(2) << 20;  // 8 MiB
+static __device__ __align__(256) float scratch[kNoAllocScratchFloats];
+
+#include <mutex>
+
 namespace {
+
+// Device address of `scratch` on the current device, looked up on the
+// first context created on that device (nullptr when the lookup fails).
+float* scratch_address(int dev) {
+  static std::mutex mu;
+  static float* addr[256] = {};
+  static bool done[256] = {};
+  if (dev < 0 || dev >= 256) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!done[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) addr[dev] = static_cast<float*>(p);
+    done[dev] = true;
+  }
+  return addr[dev];
+}
 
 const char* kNullCtxMsg = "null context";
 
@@ -297,6 +323,7 @@ struct spk_ctx {
   std::vector<int> g4r1, g4r2;
   DevBuf g4tw1, g4tw2;
   DevBuf adj_part;           // adjoint per-split weight sums
+  float* noalloc_scratch = nullptr;  // this device's `scratch` (spk_channels_adjoint_noalloc)
   int ncu = 0;
   // channel stage launch form and batched tuning (chunk blocks, rounds per item, workgroups per CU)
   int chan_mode = SPK_CHAN_PER_SIZE;
@@ -1382,6 +1409,7 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
     return cleanup(SPK_ERR_DEVICE);
   c->ncu = ncu;
+  c->noalloc_scratch = scratch_address(dev);
   c->spec_stride = ((long long)c->K + 63) / 64 * 64;
   c->welch_out_cap = c->max_blocks / c->A + 1;
   if (c->chan_counters.alloc(kXcdQueues * sizeof(int)) != hipSuccess ||
@@ -2039,13 +2067,45 @@ spk_status spk_channels_output_packed(spk_ctx* c, float* dst, size_t cap) {
 
 // Adjoint of one group of IFFT sizes without a power-of-two plan
 // (chan_generic_adjoint_kernel, then the split sums in order).
-spk_status generic_adjoint(spk_ctx* c, const ChanGroup& g, int nb, const float* ybar, float* wbar, float* xbar) {
+}  // extern "C"
+
+namespace {
+
+// Block splits S of a group's adjoint launch and the floats of per-(channel,
+// split) weight sums it writes (nch * S * L).
+size_t adjoint_split(const spk_ctx* c, const ChanGroup& g, int nb, int* S_out) {
   const int nch = (int)g.ids.size();
-  if (g.L > spk::kGenMaxL) {
+  int S;
+  if (g.generic && g.L > spk::kGenMaxL) {
     // wide channels: one (channel, split) tile per 1024-thread workgroup
-    const int S = std::max(1, std::min((2 * c->ncu + nch - 1) / nch, nb));
-    const size_t need = (size_t)nch * S * g.L * sizeof(float);
-    if (c->adj_part.bytes < need) HIP_TRY(c, c->adj_part.alloc(need));
+    S = std::max(1, std::min((2 * c->ncu + nch - 1) / nch, nb));
+  } else if (g.generic) {
+    int T = 32;
+    while (T < 256 && 2 * T < g.L) T *= 2;  // as launch_generic
+    const int tpw = spk::kGenThreads / T;
+    S = (2 * c->ncu * tpw + nch - 1) / nch;  // enough tiles for every CU
+    S = std::max(1, std::min(S, nb));
+  } else {
+    const int T = g.L / find_chan_plan(g.L)->R;
+    const int tpw = spk::kChanThreads / T;
+    // splits over blocks: enough workgroups for every CU, at least one tile-row of blocks each
+    S = (2 * c->ncu + nch - 1) / nch;
+    S = std::max(1, std::min(S, (nb + tpw - 1) / tpw));
+  }
+  *S_out = S;
+  return (size_t)nch * S * g.L;
+}
+
+// One group's adjoint: tiles write their weight sums to part[channel][split][j]
+// (every entry, so part needs no clearing), then adjoint_reduce_kernel adds
+// the splits in order into wbar.  Groups run one after another on the
+// context's stream and reuse part.
+spk_status group_adjoint(spk_ctx* c, const ChanGroup& g, int nb, const float* ybar, float* wbar, float* xbar,
+                         float* part) {
+  const int nch = (int)g.ids.size();
+  int S = 1;
+  (void)adjoint_split(c, g, nb, &S);
+  if (g.generic) {
     spk::GenAdjParams Q{};
     Q.spec = c->spec.as<cpx>();
     Q.spec_stride = c->spec_stride;
@@ -2060,89 +2120,27 @@ spk_status generic_adjoint(spk_ctx* c, const ChanGroup& g, int nb, const float* 
     Q.S = S;
     Q.ybar = reinterpret_cast<const cpx*>(ybar);
     Q.xbar = reinterpret_cast<cpx*>(xbar);
-    Q.wpart = c->adj_part.as<float>();
+    Q.wpart = part;
     Q.tw = g.gtw.as<cpx>();
     Q.L = g.L;
     Q.nstages = (int)g.radix.size();
     for (size_t i = 0; i < g.radix.size(); ++i) Q.radix[i] = g.radix[i];
-    Q.tile_threads = spk::kGenBigThreads;
-    hipLaunchKernelGGL(spk::chan_generic_big_adjoint_kernel, dim3((unsigned)(nch * S)), dim3(spk::kGenBigThreads), 0,
-                       c->stream, Q);
-    HIP_TRY(c, hipGetLastError());
-    spk::AdjReduceParams Rp{};
-    Rp.wpart = c->adj_part.as<float>();
-    Rp.chans = g.desc.as<spk::ChanDesc>();
-    Rp.L = g.L;
-    Rp.S = S;
-    Rp.wbar = wbar;
-    hipLaunchKernelGGL(spk::adjoint_reduce_kernel, dim3((g.L + 255) / 256, nch), dim3(256), 0, c->stream, Rp);
-    HIP_TRY(c, hipGetLastError());
-    return SPK_OK;
-  }
-  int T = 32;
-  while (T < 256 && 2 * T < g.L) T *= 2;  // as launch_generic
-  const int tpw = spk::kGenThreads / T;
-  int S = (2 * c->ncu * tpw + nch - 1) / nch;  // enough tiles for every CU
-  S = std::max(1, std::min(S, nb));
-  const size_t need = (size_t)nch * S * g.L * sizeof(float);
-  if (c->adj_part.bytes < need) HIP_TRY(c, c->adj_part.alloc(need));
-  spk::GenAdjParams Q{};
-  Q.spec = c->spec.as<cpx>();
-  Q.spec_stride = c->spec_stride;
-  Q.N = c->N;
-  Q.K = c->K;
-  Q.real_input = c->complex_in ? 0 : 1;
-  Q.weights = c->weights.as<float>();
-  Q.chans = g.desc.as<spk::ChanDesc>();
-  Q.pk_off = g.pk_off.as<long long>();
-  Q.nch = nch;
-  Q.nblocks = nb;
-  Q.S = S;
-  Q.ybar = reinterpret_cast<const cpx*>(ybar);
-  Q.xbar = reinterpret_cast<cpx*>(xbar);
-  Q.wpart = c->adj_part.as<float>();
-  Q.tw = g.gtw.as<cpx>();
-  Q.L = g.L;
-  Q.nstages = (int)g.radix.size();
-  for (size_t i = 0; i < g.radix.size(); ++i) Q.radix[i] = g.radix[i];
-  Q.tile_threads = T;
-  const long long tiles = (long long)nch * S;
-  const unsigned grid = (unsigned)((tiles + tpw - 1) / tpw);
-  const size_t lds = (size_t)tpw * 2 * g.L * sizeof(cpx);
-  hipLaunchKernelGGL(spk::chan_generic_adjoint_kernel, dim3(grid), dim3(spk::kGenThreads), lds, c->stream, Q);
-  HIP_TRY(c, hipGetLastError());
-  spk::AdjReduceParams Rp{};
-  Rp.wpart = c->adj_part.as<float>();
-  Rp.chans = g.desc.as<spk::ChanDesc>();
-  Rp.L = g.L;
-  Rp.S = S;
-  Rp.wbar = wbar;
-  hipLaunchKernelGGL(spk::adjoint_reduce_kernel, dim3((g.L + 255) / 256, nch), dim3(256), 0, c->stream, Rp);
-  HIP_TRY(c, hipGetLastError());
-  return SPK_OK;
-}
-
-spk_status spk_channels_adjoint(spk_ctx* c, const float* ybar, float* wbar, float* xbar) {
-  try {
-  if (!c || !ybar || !wbar) return SPK_ERR_INVALID;
-  if (!c->have_output || c->chans_dirty) return fail(c, SPK_ERR_STATE, "no output yet");
-  if (!c->keep_spec) return fail(c, SPK_ERR_STATE, "the adjoint needs a context created with SPK_FLAG_KEEP_SPECTRA");
-  const int nb = (int)c->last_blocks;
-  for (auto& kv : c->groups) {
-    ChanGroup& g = kv.second;
-    const int nch = (int)g.ids.size();
-    if (g.generic) {
-      spk_status st = generic_adjoint(c, g, nb, ybar, wbar, xbar);
-      if (st != SPK_OK) return st;
-      continue;
+    if (g.L > spk::kGenMaxL) {
+      Q.tile_threads = spk::kGenBigThreads;
+      hipLaunchKernelGGL(spk::chan_generic_big_adjoint_kernel, dim3((unsigned)(nch * S)), dim3(spk::kGenBigThreads),
+                         0, c->stream, Q);
+    } else {
+      int T = 32;
+      while (T < 256 && 2 * T < g.L) T *= 2;  // as launch_generic
+      const int tpw = spk::kGenThreads / T;
+      Q.tile_threads = T;
+      const long long tiles = (long long)nch * S;
+      const unsigned grid = (unsigned)((tiles + tpw - 1) / tpw);
+      const size_t lds = (size_t)tpw * 2 * g.L * sizeof(cpx);
+      hipLaunchKernelGGL(spk::chan_generic_adjoint_kernel, dim3(grid), dim3(spk::kGenThreads), lds, c->stream, Q);
     }
-    const int T = g.L / find_chan_plan(g.L)->R;
-    const int tpw = spk::kChanThreads / T;
-    // splits over blocks: enough workgroups for every CU, at least one tile-row of blocks each
-    int S = (2 * c->ncu + nch - 1) / nch;
-    S = std::max(1, std::min(S, (nb + tpw - 1) / tpw));
-    const size_t need = (size_t)nch * S * g.L * sizeof(float);
-    if (c->adj_part.bytes < need) HIP_TRY(c, c->adj_part.alloc(need));
+    HIP_TRY(c, hipGetLastError());
+  } else {
     spk::ChanAdjParams Q{};
     Q.spec = c->spec.as<cpx>();
     Q.spec_stride = c->spec_stride;
@@ -2157,20 +2155,65 @@ spk_status spk_channels_adjoint(spk_ctx* c, const float* ybar, float* wbar, floa
     Q.ybar = reinterpret_cast<const cpx*>(ybar);
     Q.pk_off = g.pk_off.as<long long>();
     Q.xbar = reinterpret_cast<cpx*>(xbar);
-    Q.wpart = c->adj_part.as<float>();
+    Q.wpart = part;
     Q.tw2 = g.atw2.as<cpx>();
     Q.tw3 = g.atw3.as<cpx>();
     HIP_TRY(c, dispatch_chan_adjoint(g.L, Q, c->stream));
-    spk::AdjReduceParams Rp{};
-    Rp.wpart = c->adj_part.as<float>();
-    Rp.chans = g.desc.as<spk::ChanDesc>();
-    Rp.L = g.L;
-    Rp.S = S;
-    Rp.wbar = wbar;
-    hipLaunchKernelGGL(spk::adjoint_reduce_kernel, dim3((g.L + 255) / 256, nch), dim3(256), 0, c->stream, Rp);
-    HIP_TRY(c, hipGetLastError());
+  }
+  spk::AdjReduceParams Rp{};
+  Rp.wpart = part;
+  Rp.chans = g.desc.as<spk::ChanDesc>();
+  Rp.L = g.L;
+  Rp.S = S;
+  Rp.wbar = wbar;
+  hipLaunchKernelGGL(spk::adjoint_reduce_kernel, dim3((g.L + 255) / 256, nch), dim3(256), 0, c->stream, Rp);
+  HIP_TRY(c, hipGetLastError());
+  return SPK_OK;
+}
+
+// The adjoint over every group, with the weight sums in part (part_floats
+// long) or, when part is null, in the context's adj_part (grown as needed).
+spk_status channels_adjoint(spk_ctx* c, const float* ybar, float* wbar, float* xbar, float* part,
+                            size_t part_floats) {
+  if (!c || !ybar || !wbar) return SPK_ERR_INVALID;
+  if (!c->have_output || c->chans_dirty) return fail(c, SPK_ERR_STATE, "no output yet");
+  if (!c->keep_spec) return fail(c, SPK_ERR_STATE, "the adjoint needs a context created with SPK_FLAG_KEEP_SPECTRA");
+  const int nb = (int)c->last_blocks;
+  size_t need = 0;
+  for (const auto& kv : c->groups) {
+    int S = 1;
+    need = std::max(need, adjoint_split(c, kv.second, nb, &S));
+  }
+  if (part) {
+    if (need > part_floats)
+      return fail(c, SPK_ERR_CAPACITY, "the adjoint's weight sums exceed the fixed scratch size");
+  } else {
+    if (c->adj_part.bytes < need * sizeof(float)) HIP_TRY(c, c->adj_part.alloc(need * sizeof(float)));
+    part = c->adj_part.as<float>();
+  }
+  for (const auto& kv : c->groups) {
+    spk_status st = group_adjoint(c, kv.second, nb, ybar, wbar, xbar, part);
+    if (st != SPK_OK) return st;
   }
   return SPK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+spk_status spk_channels_adjoint(spk_ctx* c, const float* ybar, float* wbar, float* xbar) {
+  try {
+    return channels_adjoint(c, ybar, wbar, xbar, nullptr, 0);
+  } catch (...) {
+    return entry_exception(c);
+  }
+}
+
+spk_status spk_channels_adjoint_noalloc(spk_ctx* c, const float* ybar, float* wbar, float* xbar) {
+  try {
+    if (c && !c->noalloc_scratch) return fail(c, SPK_ERR_DEVICE, "no device address for the fixed scratch");
+    return channels_adjoint(c, ybar, wbar, xbar, c ? c->noalloc_scratch : nullptr, kNoAllocScratchFloats);
   } catch (...) {
     return entry_exception(c);
   }

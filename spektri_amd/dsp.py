@@ -235,6 +235,14 @@ class DspState:
         _lib.check(_lib.load().spk_channels_adjoint(self._ctx, ctypes.c_void_p(ybar_ptr), ctypes.c_void_p(wbar_ptr),
                                                     ctypes.c_void_p(xbar_ptr or None)), self._ctx)
 
+    def channels_adjoint_noalloc(self, ybar_ptr: int, wbar_ptr: int, xbar_ptr: int = 0):
+        """channels_adjoint with its weight sums in the library's fixed
+        per-GPU scratch (no device allocation; SPK_ERR_CAPACITY past 2^21
+        floats per channel group)."""
+        _lib.check(_lib.load().spk_channels_adjoint_noalloc(self._ctx, ctypes.c_void_p(ybar_ptr),
+                                                            ctypes.c_void_p(wbar_ptr),
+                                                            ctypes.c_void_p(xbar_ptr or None)), self._ctx)
+
     # -- channel stage launch form --------------------------------------------
     def set_channel_mode(self, batched: bool):
         """Plain per-size launches (False) or the batched persistent launch (True)."""
