@@ -149,13 +149,18 @@ static double now_s(void) {
 /*
  * Process whole buffers of F blocks from the s16 stream x until `seconds`
  * have elapsed or nbuf_max buffers are done.  Returns elapsed seconds;
- * *done receives the buffer count.  Outputs are computed into scratch
- * buffers (the reference serialises them to bytes; that cost is omitted).
- * Optional checking outputs: spec_out (F*nbuf x K complex split re/im) and
- * welch_out (records x K) may be NULL.
+ * *done receives the buffer count.  The reference serialises its outputs to
+ * bytes; that cost is omitted.  When chan_out is not NULL, every channel's
+ * output of every processed block is stored there (channel c at float
+ * offset 2 * nbuf_max * F * sum_{c' < c} 6 (L_c' / 8), interleaved re, im);
+ * when welch_out is not NULL, every finished record acc / A is stored there
+ * (K floats each) and *nrec receives the count.  Otherwise the channel
+ * outputs go to a per-buffer scratch and only one bin of each record is
+ * kept.
  */
 double spkport_run(const int16_t* x, int64_t nbuf_max, int N, int F, int A, int nch, const int64_t* firsts,
-                   const int32_t* Ls, const float* weights, double seconds, int threads, int* done) {
+                   const int32_t* Ls, const float* weights, double seconds, int threads, int* done,
+                   float* chan_out_all, float* welch_out, int* nrec) {
   const int O = N / 4, H = N - O, K = N / 2 + 1;
   const float scaling = 1.0f / 32767.0f;
   const float wv = scaling / (float)N; /* dsp.rs:70-73 */
@@ -178,6 +183,15 @@ double spkport_run(const int16_t* x, int64_t nbuf_max, int N, int F, int A, int 
   float* acc = (float*)calloc((size_t)K, sizeof(float));
   unsigned accn = 0;
   float* chan_out = (float*)malloc(sizeof(float) * 2 * (size_t)nch * F * (3 * (size_t)maxL / 4 + 1));
+  size_t* coff = (size_t*)calloc((size_t)nch, sizeof(size_t)); /* float offsets in chan_out_all */
+  {
+    size_t o = 0;
+    for (int c = 0; c < nch; ++c) {
+      coff[c] = o;
+      o += 2 * (size_t)nbuf_max * F * (6 * (size_t)(Ls[c] / 8));
+    }
+  }
+  int recs = 0;
   volatile float sink = 0.f;
 
   const double t0 = now_s();
@@ -232,7 +246,13 @@ double spkport_run(const int16_t* x, int64_t nbuf_max, int N, int F, int A, int 
         acc[k] += cr * cr + ci * ci;
       }
       if (++accn >= (unsigned)A) {
-        sink += acc[K / 2] / (float)accn;
+        if (welch_out) {
+          float* r = welch_out + (size_t)recs * K;
+          for (int k = 0; k < K; ++k) r[k] = acc[k] / (float)accn;
+        } else {
+          sink += acc[K / 2] / (float)accn;
+        }
+        ++recs;
         memset(acc, 0, sizeof(float) * (size_t)K);
         accn = 0;
       }
@@ -245,8 +265,11 @@ double spkport_run(const int16_t* x, int64_t nbuf_max, int N, int F, int A, int 
       float* tr = (float*)malloc(sizeof(float) * 4 * (size_t)L);
       float *ti = tr + L, *ur = tr + 2 * L, *ui = tr + 3 * L;
       const float* W = weights + woff[c];
-      float* out = chan_out + 2 * (size_t)c * F * (3 * (size_t)maxL / 4 + 1);
+      const size_t per_blk = 2 * (size_t)(6 * (L / 8));
+      float* out0 = chan_out_all ? chan_out_all + coff[c] + (size_t)b * F * per_blk
+                                 : chan_out + 2 * (size_t)c * F * (3 * (size_t)maxL / 4 + 1);
       for (int g = 0; g < F; ++g) {
+        float* out = out0 + (chan_out_all ? (size_t)g * per_blk : 0);
         const float* Xr = specr + (size_t)g * K;
         const float* Xi = speci + (size_t)g * K;
         for (int i = 0; i < L; ++i) {
@@ -271,6 +294,8 @@ double spkport_run(const int16_t* x, int64_t nbuf_max, int N, int F, int A, int 
   }
   const double el = now_s() - t0;
   *done = (int)b;
+  if (nrec) *nrec = recs;
+  free(coff);
   plan_free(&fwd);
   for (int c = 0; c < nch; ++c) plan_free(&inv[c]);
   free(inv);
@@ -312,4 +337,129 @@ void spkport_spectra(const int16_t* x, int nblk, int N, float* out_re, float* ou
   }
   free(zr);
   plan_free(&fwd);
+}
+
+/*
+ * Welch and channel stage of spektri_port's loop on spectra computed
+ * elsewhere (the pocketfft forward FFTs of oracle/cpu_port.py).
+ * spkport_post_create plans every channel's inverse FFT once;
+ * spkport_post takes nblk blocks of K complex bins (interleaved re, im;
+ * K = N/2 + 1 for real input, N for complex).  Welch is accumulated block
+ * after block on the calling thread (spectrum.rs:89-116) into acc / *accn
+ * across calls; a finished record is stored in welch_out (when not NULL)
+ * and counted in *nrec.  Channels run in parallel (fcfb.rs:74,154-188);
+ * their outputs go to chan_out (when not NULL) at float offset
+ * 2 * out_blocks * sum_{c' < c} 6 (L_c' / 8), block blk0 + g of this call
+ * at 2 * (blk0 + g) * 6 (L_c / 8) after that.
+ */
+typedef struct {
+  int nch;
+  fft_plan* inv;
+  size_t *woff, *coff;
+  int64_t out_blocks;
+} post_ctx;
+
+void* spkport_post_create(int nch, const int32_t* Ls, int64_t out_blocks) {
+  post_ctx* p = (post_ctx*)calloc(1, sizeof(post_ctx));
+  p->nch = nch;
+  p->out_blocks = out_blocks;
+  p->inv = (fft_plan*)calloc((size_t)nch, sizeof(fft_plan));
+  p->woff = (size_t*)calloc((size_t)nch, sizeof(size_t));
+  p->coff = (size_t*)calloc((size_t)nch, sizeof(size_t));
+  size_t wsum = 0;
+  for (int c = 0; c < nch; ++c) {
+    plan_init(&p->inv[c], Ls[c], +1);
+    p->woff[c] = wsum;
+    wsum += (size_t)Ls[c];
+    p->coff[c] = c == 0 ? 0 : p->coff[c - 1] + 2 * (size_t)out_blocks * (6 * (size_t)(Ls[c - 1] / 8));
+  }
+  return p;
+}
+
+void spkport_post_destroy(void* h) {
+  post_ctx* p = (post_ctx*)h;
+  for (int c = 0; c < p->nch; ++c) plan_free(&p->inv[c]);
+  free(p->inv);
+  free(p->woff);
+  free(p->coff);
+  free(p);
+}
+
+void spkport_post(void* h, const float* X, int nblk, int N, int complex_in, int A, float* acc, int* accn,
+                  const int64_t* firsts, const int32_t* Ls, const float* weights, int threads, float* chan_out,
+                  int64_t blk0, float* welch_out, int* nrec) {
+  const post_ctx* p = (const post_ctx*)h;
+  const int K = complex_in ? N : N / 2 + 1;
+  const int nch = p->nch;
+  if (threads > 0) omp_set_num_threads(threads);
+  for (int g = 0; g < nblk; ++g) {
+    const float* B = X + 2 * (size_t)g * K;
+    /* the two end bins take their outer neighbour through get_bin
+       (complex: wrap around; real: conjugate mirror) */
+    for (int e = 0; e < 2; ++e) {
+      const int k = e == 0 ? 0 : K - 1;
+      float lr, li, rr, ri;
+      if (complex_in) {
+        const int km = (k - 1 + N) % N, kp = (k + 1) % N;
+        lr = B[2 * km]; li = B[2 * km + 1];
+        rr = B[2 * kp]; ri = B[2 * kp + 1];
+      } else if (k == 0) {
+        lr = B[2]; li = -B[3];
+        rr = B[2]; ri = B[3];
+      } else {
+        lr = B[2 * (k - 1)]; li = B[2 * (k - 1) + 1];
+        rr = B[2 * (N - k - 1)]; ri = -B[2 * (N - k - 1) + 1];
+      }
+      const float cr = B[2 * k] - (lr + rr) * 0.5f, ci = B[2 * k + 1] - (li + ri) * 0.5f;
+      acc[k] += cr * cr + ci * ci;
+    }
+    for (int k = 1; k < K - 1; ++k) {
+      const float cr = B[2 * k] - (B[2 * k - 2] + B[2 * k + 2]) * 0.5f;
+      const float ci = B[2 * k + 1] - (B[2 * k - 1] + B[2 * k + 3]) * 0.5f;
+      acc[k] += cr * cr + ci * ci;
+    }
+    if (++*accn >= A) {
+      if (welch_out) {
+        float* r = welch_out + (size_t)*nrec * K;
+        for (int k = 0; k < K; ++k) r[k] = acc[k] / (float)*accn;
+      }
+      ++*nrec;
+      memset(acc, 0, sizeof(float) * (size_t)K);
+      *accn = 0;
+    }
+  }
+#pragma omp parallel for schedule(dynamic)
+  for (int c = 0; c < nch; ++c) {
+    const int L = Ls[c];
+    float* tr = (float*)malloc(sizeof(float) * 4 * (size_t)L);
+    float *ti = tr + L, *ur = tr + 2 * L, *ui = tr + 3 * L;
+    const float* W = weights + p->woff[c];
+    const size_t per_blk = 2 * (size_t)(6 * (L / 8));
+    for (int g = 0; g < nblk; ++g) {
+      const float* B = X + 2 * (size_t)g * K;
+      for (int i = 0; i < L; ++i) {
+        const int j = (i + L / 2) % L;
+        long m = ((firsts[c] + j) % N + N) % N;
+        float re, im;
+        if (complex_in || m < K) {
+          re = B[2 * m];
+          im = B[2 * m + 1];
+        } else {
+          re = B[2 * (N - m)];
+          im = -B[2 * (N - m) + 1];
+        }
+        tr[i] = W[j] * re;
+        ti[i] = W[j] * im;
+      }
+      fft_exec(&p->inv[c], tr, ti, ur, ui);
+      if (chan_out) {
+        float* out = chan_out + p->coff[c] + (size_t)(blk0 + g) * per_blk;
+        for (int i = L / 8, o = 0; i < (L / 8) * 7; ++i, ++o) {
+          out[2 * o] = tr[i];
+          out[2 * o + 1] = ti[i];
+        }
+      }
+    }
+    free(tr);
+  }
 }
