@@ -113,7 +113,7 @@ struct FwdParams {
   float wscale;
   int nblocks;
   int H, O;
-  // fused channel stage (FL > 0): every channel has IFFT size FL, lies in
+  // fused channel stage (FL >= 8): every channel has IFFT size FL, lies in
   // bins [0, K) and no two channels share a bin
   const ChanDesc* fchans;
   int fnch;
@@ -137,7 +137,31 @@ struct FwdParams {
   // between launches; the launch's last taker resets it).
   FinParams fin;
   int* fin_ticket;
+  // General fused stage (FL = kFusedGen1 / kFusedGen2, below): the scatter
+  // tables of fscat hold fused_targets(FL) tables of M/8 uint4 one after the
+  // other, entries LDS cpx index | conj << 15; the Nyquist bin's second
+  // target; per-tile descriptors; per (round, thread) slots (see gen_slot);
+  // pass-2 twiddles of the tiles of 32, 64, 128 and 256 points.
+  int fscat_nyq2;
+  const struct GenTile* gtiles;
+  const int* gslots;
+  int grounds;
+  const cpx* gtw[4];
 };
+
+// A tile of the general fused stage: one channel, its LDS region (cpx
+// offset, FL + FL/32 values), weights and output.
+struct GenTile {
+  long long out_off;
+  int lds;
+  int w_off;
+};
+// Thread slot of the general fused stage: bit 31 idle, bits 0-2 size class
+// c (FL = 8 << c; every lane of a wave has the same class), bits 3-7 the
+// thread's index in the tile, bits 8-30 the tile.
+__host__ __device__ constexpr int gen_slot(int tile, int ftt, int cls, bool idle) {
+  return (idle ? (int)0x80000000 : 0) | (tile << 8) | (ftt << 3) | cls;
+}
 
 // Fused channel tiles: FL points as 16 x (FL/16), 16 points per thread.
 template <int FL>
@@ -152,6 +176,34 @@ template <>
 struct FusedCfg<0> {
   static constexpr int RF = 1, TF = 1, RF1 = 1, RF2 = 1, STRIDE = 0;
 };
+// The general fused stage: channels of any power-of-two IFFT size from 8 to
+// 256 (mixed), reaching past DC or Nyquist (get_bin's mirror), overlapping,
+// several tiles per thread when there are more than the workgroup holds at
+// once; its tiles share the LDS area of the exchange planes.  kFusedGen1
+// scatters every bin to at most one tile position, kFusedGen2 to two.
+constexpr int kFusedGen1 = 1;
+constexpr int kFusedGen2 = 2;
+__host__ __device__ constexpr bool fused_general(int FL) { return FL == kFusedGen1 || FL == kFusedGen2; }
+__host__ __device__ constexpr int fused_targets(int FL) { return FL == kFusedGen2 ? 2 : 1; }
+template <>
+struct FusedCfg<kFusedGen1> {
+  static constexpr int RF = 1, TF = 1, RF1 = 1, RF2 = 1, STRIDE = 0;
+};
+template <>
+struct FusedCfg<kFusedGen2> {
+  static constexpr int RF = 1, TF = 1, RF1 = 1, RF2 = 1, STRIDE = 0;
+};
+// Tile layout of an FL-point channel in the general fused stage: FL =
+// RF1 x RF2 with 16 points per thread (FL / 16 threads), one thread for 8
+// and 16 points.
+template <int FL>
+struct GenTileCfg {
+  static constexpr int RF = FL < 16 ? FL : 16;
+  static constexpr int TF = FL / RF;
+  static constexpr int RF1 = RF;
+  static constexpr int RF2 = FL / RF;
+  static constexpr int STRIDE = FL + (FL >> 5);
+};
 
 template <int M, int R, int R1, int R2, int FL>
 struct FwdRealCfg {
@@ -165,7 +217,7 @@ struct FwdRealCfg {
   static_assert(T >= 64 && T <= 1024, "one wave minimum, 1024 threads maximum");
   static constexpr bool SPLIT = lds_padded_size(M) * 8 > kLdsBudgetBytes;
   static constexpr int PLANE_FLOATS = SPLIT ? lds_padded_size(M) : 2 * lds_padded_size(M);
-  static constexpr int TILE_FLOATS = FL > 0 ? (T / FusedCfg<FL>::TF) * FusedCfg<FL>::STRIDE * 2 : 0;
+  static constexpr int TILE_FLOATS = FL > kFusedGen2 ? (T / FusedCfg<FL>::TF) * FusedCfg<FL>::STRIDE * 2 : 0;
   static constexpr int LDS_FLOATS = PLANE_FLOATS > TILE_FLOATS ? PLANE_FLOATS : TILE_FLOATS;
   // After the main area: the pass-3 outputs of lanes 0 and 32 of wave 0
   // (2 R complex), which are their own mirrors.
@@ -395,6 +447,51 @@ __device__ __forceinline__ void welch_finalize_quad(const FinParams& F, long lon
 // after a barrier from the edge table.  Channels of the fused stage are
 // gathered by scattering each owned bin into an LDS tile buffer through the
 // bin -> position map.
+// One thread's part of a general fused tile (fcfb.rs:154-188): weight the
+// gathered bins, inverse FFT of FL points in the tile's LDS region, keep
+// [FL/8, 7FL/8).  The tile's threads lie in one wave and only they touch its
+// region until the next block's first exchange barrier.
+template <int FL>
+__device__ __forceinline__ void gen_tile(const FwdParams& P, cpx* ldsc, int tile, int ftt, bool act, int g) {
+  using TC = GenTileCfg<FL>;
+  constexpr int RF = TC::RF, TF = TC::TF;
+  const GenTile gd = P.gtiles[tile];
+  const float* wts = P.weights + gd.w_off;
+  cpx* tl = ldsc + gd.lds;
+  cpx gx[RF];
+#pragma unroll
+  for (int e = 0; e < RF; ++e) {
+    const int n = st_in_index<FL, RF, TF, TC::RF1>(ftt, e);
+    gx[e] = cscale(tl[st_in_addr<FL, RF, TF, TC::RF1>(ftt, e)], wts[(n + FL / 2) & (FL - 1)]);
+  }
+  st_butterflies<FL, RF, TF, TC::RF1, 1, +1>(gx, ftt, nullptr);
+  if constexpr (TC::RF2 > 1) {
+    tile_exchange<TF>(
+        gx, reinterpret_cast<float*>(tl), [&](int e) { return st_out_addr<FL, RF, TF, TC::RF1, 1>(ftt, e); },
+        [&](int e) { return st_in_addr<FL, RF, TF, TC::RF2>(ftt, e); });
+    constexpr int ci = FL == 32 ? 0 : (FL == 64 ? 1 : (FL == 128 ? 2 : 3));
+    st_butterflies<FL, RF, TF, TC::RF2, TC::RF1, +1>(gx, ftt, P.gtw[ci]);
+  }
+  if (act) {
+    cpx* out = P.fout + gd.out_off + (long long)g * (3 * FL / 4);
+    constexpr int PL = FL / TC::RF2;
+#pragma unroll
+    for (int e = 0; e < RF; ++e) {
+      const int u = e / TC::RF2, q = e % TC::RF2;
+      const int idx = ftt + TF * u + q * PL;
+      if (idx >= FL / 8 && idx < (FL / 8) * 7) out[idx - FL / 8] = gx[e];
+    }
+  }
+}
+
+// Write X to one target of the general fused scatter (LDS cpx index in the
+// low 15 bits, conjugate when bit 15 is set: get_bin's mirror).
+__device__ __forceinline__ void gen_put(cpx* ldsc, unsigned entry, cpx X) {
+  cpx v = X;
+  v.y = __uint_as_float(__float_as_uint(X.y) ^ ((entry & 0x8000u) << 16));
+  ldsc[entry & 0x7FFFu] = v;
+}
+
 template <int M, int R, int R1, int R2, int FL, bool F32>
 __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   using C = FwdRealCfg<M, R, R1, R2, FL>;
@@ -520,6 +617,7 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     const unsigned tvoff = (unsigned)t * (unsigned)sizeof(float);
     float pv[R];
     uint4 sw[2];
+    uint4 swg[fused_targets(FL)][2];
     auto load_group = [&](auto jc) {
       constexpr int j = decltype(jc)::value;
       if (fresh) {
@@ -530,7 +628,11 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
         for (int q = 8 * j; q < 8 * j + 8; ++q)
           pv[q] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(prow, tvoff, q * T * (int)sizeof(float), kCpolSc1));
       }
-      if constexpr (FL > 0) sw[j & 1] = P.fscat[j * T + t];
+      if constexpr (FL > kFusedGen2) sw[j & 1] = P.fscat[j * T + t];
+      if constexpr (fused_general(FL)) {
+#pragma unroll
+        for (int d = 0; d < fused_targets(FL); ++d) swg[d][j & 1] = P.fscat[(d * (R / 8) + j) * T + t];
+      }
     };
     load_group(std::integral_constant<int, 0>{});
 
@@ -624,7 +726,11 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
           const cpx XN = {0.5f * (S1 - S2), 0.5f * (D2 + D1)};
           if (P.keep_spec) spec[M] = XN;
           ering[kEdgeNyq * R] = XN;
-          if constexpr (FL > 0) reinterpret_cast<cpx*>(lds)[P.fscat_nyq] = XN;
+          if constexpr (FL > kFusedGen2) reinterpret_cast<cpx*>(lds)[P.fscat_nyq] = XN;
+          if constexpr (fused_general(FL)) {
+            gen_put(reinterpret_cast<cpx*>(lds), (unsigned)P.fscat_nyq, XN);
+            if constexpr (fused_targets(FL) > 1) gen_put(reinterpret_cast<cpx*>(lds), (unsigned)P.fscat_nyq2, XN);
+          }
         }
       }
       // Welch neighbours by DPP: lanes l-1 and l+1 (left/right for lower
@@ -637,7 +743,16 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
       dy = __builtin_fmaf(-2.f, X.y, dy);
       const float acc = __builtin_fmaf(0.25f, __builtin_fmaf(dy, dy, dx * dx), pv[q]);
       __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(acc), prow, tvoff, q * T * (int)sizeof(float), 0);
-      if constexpr (FL > 0) {
+      if constexpr (fused_general(FL)) {
+        // general scatter: every target of this bin (a dummy slot when none)
+#pragma unroll
+        for (int d = 0; d < fused_targets(FL); ++d) {
+          const uint4 w4 = swg[d][(q / 8) & 1];
+          const unsigned wd = (q & 7) < 2 ? w4.x : (q & 7) < 4 ? w4.y : (q & 7) < 6 ? w4.z : w4.w;
+          gen_put(reinterpret_cast<cpx*>(lds), (q & 1) ? (wd >> 16) : (wd & 0xFFFFu), X);
+        }
+      }
+      if constexpr (FL > kFusedGen2) {
         // Channel scatter (fcfb.rs:164-172): bin first + j feeds input
         // position (j + FL/2) mod FL of its channel's tile; the index table
         // sends other bins to a dummy slot.
@@ -701,9 +816,33 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
       edge_finish();
     }
 
+    // General fused stage: the thread's tile slot of every round (the
+    // size class is the same for every lane of a wave).
+    if constexpr (fused_general(FL)) {
+      edge_prep();
+      __syncthreads();  // tile buffers and edge ring complete
+      edge_issue();
+      for (int r = 0; r < P.grounds; ++r) {
+        const int sl = P.gslots[r * T + t];
+        const int cls = __builtin_amdgcn_readfirstlane(sl & 7);
+        const int tile = (sl >> 8) & 0x7FFFFF, ftt = (sl >> 3) & 31;
+        const bool act = sl >= 0;
+        cpx* ldsc = reinterpret_cast<cpx*>(lds);
+        switch (cls) {
+          case 0: gen_tile<8>(P, ldsc, tile, ftt, act, g); break;
+          case 1: gen_tile<16>(P, ldsc, tile, ftt, act, g); break;
+          case 2: gen_tile<32>(P, ldsc, tile, ftt, act, g); break;
+          case 3: gen_tile<64>(P, ldsc, tile, ftt, act, g); break;
+          case 4: gen_tile<128>(P, ldsc, tile, ftt, act, g); break;
+          default: gen_tile<256>(P, ldsc, tile, ftt, act, g); break;
+        }
+      }
+      edge_finish();
+    }
+
     // Fused channel stage (fcfb.rs:154-188): weight the gathered bins,
     // inverse FFT in the tile's LDS region, keep [FL/8, 7FL/8).
-    if constexpr (FL > 0) {
+    if constexpr (FL > kFusedGen2) {
       constexpr int TF = FC::TF, RF = FC::RF;
       const int fc = t / TF, ftt = t % TF;
       const bool fact = fc < P.fnch;

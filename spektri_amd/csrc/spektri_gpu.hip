@@ -117,6 +117,7 @@ struct ChanGroup {  // channels sharing one IFFT size
   std::vector<int> radix;  // stage radices, product L
   DevBuf gtw;              // w_L^e, e = 0..L-1, inverse sign
   DevBuf pk_off;           // long long per channel: packed-layout offset (samples per block before it)
+  bool fused = false;      // computed inside the forward kernel
 };
 
 // Output samples per block of a channel of IFFT size L: the reference keeps
@@ -349,7 +350,13 @@ struct spk_ctx {
   // on flush).  pend_nb == 0: none.
   int pend_nb = 0, pend_G = 0, pend_parity = 0;
   DevBuf fin_ticket;          // chunk counter of the forward kernel's record phase
-  int gmax[2] = {0, 0};      // persistent workgroups of the unfused / fused forward kernel
+  int gmax[3] = {0, 0, 0};   // persistent workgroups of the forward kernel: no fused stage / L = 256 / general
+  int fmode = 0;             // fused stage of the forward kernel: 0, kFusedL, spk::kFusedGen1 or kFusedGen2
+  // general fused stage: tile descriptors, per-(round, thread) slots, pass-2
+  // twiddles of the 32..256-point tiles, second Nyquist target
+  DevBuf gtiles, gslots, gentw[4];
+  size_t weights_live = 0;   // floats of live channels' weights (then 256 zeros)
+  int grounds = 0, fscat_nyq2 = 0;
   bool keep_spec = false;
   bool fused = false;        // channels run inside the forward kernel
 
@@ -442,6 +449,102 @@ void drop_graphs_and_plans(spk_ctx* c) {
   c->batches.clear();
 }
 
+// LDS dummy slot and tile capacity (cpx) of a forward kernel instantiation
+// (s16 loader); -1 when there is none.
+int fwd_lds_dummy(int M, int FL);
+int fwd_tile_cap(int M, int FL);
+
+// The general fused stage (spk::kFusedGen1 / kFusedGen2): every channel of
+// the power-of-two IFFT sizes 8..256 runs inside the forward kernel when
+// their tiles fit its LDS, no bin feeds more than two tile positions, and
+// they take at most kGenMaxRounds rounds of the workgroup's threads.  Tile
+// position i of channel c holds get_bin(first + j), j = (i + L/2) mod L
+// (fcfb.rs:164-172): bin m mod N directly when below K, else conj of bin
+// N - (m mod N) (fftutil.rs:8-22).  Each bin is scattered to its targets
+// through one or two index tables laid out like the scatter table of the
+// fixed-size stage.  Leaves the channel set unfused (SPK_OK) otherwise.
+constexpr int kGenMaxRounds = 8;
+spk_status fuse_general(spk_ctx* c) {
+  if (c->f32_in) return SPK_OK;  // the general stage has the s16 loader only
+  const int tile_cap = fwd_tile_cap(c->M, spk::kFusedGen1);
+  if (tile_cap < 0 || fwd_tile_cap(c->M, spk::kFusedGen2) != tile_cap) return SPK_OK;
+  const int M = c->M, R = c->fplan->R, T = M / R, N = c->N, K = c->K;
+  std::vector<uint32_t> ids;
+  for (const auto& kv : c->groups) {
+    const int L = kv.first;
+    if (!kv.second.generic && L >= 8 && L <= 256) ids.insert(ids.end(), kv.second.ids.begin(), kv.second.ids.end());
+  }
+  if (ids.empty()) return SPK_OK;
+  // LDS regions, in ascending L (the group map's order)
+  std::vector<spk::GenTile> tiles;
+  int off = 0;
+  for (uint32_t id : ids) {
+    const int L = c->chans[id].L;
+    tiles.push_back(spk::GenTile{c->chan_out_off[id], off, c->chan_w_off[id]});
+    off += L + (L >> 5);
+  }
+  // Idle lanes (padding a wave to one size class) run the tile code on a
+  // spare region at the end of the tile area, with zero weights, and write
+  // no output: no lane writes into another tile's region.
+  constexpr int kSpare = 256 + (256 >> 5);
+  if (off > tile_cap - kSpare) return SPK_OK;
+  const int spare = (int)tiles.size();
+  tiles.push_back(spk::GenTile{0, tile_cap - kSpare, (int)c->weights_live});
+  // targets of every bin 0..M
+  const uint16_t dummy = (uint16_t)fwd_lds_dummy(M, spk::kFusedGen1);
+  std::vector<uint16_t> tgt((size_t)2 * (M + 1), dummy);
+  std::vector<uint8_t> cnt((size_t)M + 1, 0);
+  for (size_t ti = 0; ti < ids.size(); ++ti) {
+    const Channel& ch = c->chans[ids[ti]];
+    const int L = ch.L;
+    for (int j = 0; j < L; ++j) {
+      long long m = (ch.first + j) % N;
+      if (m < 0) m += N;
+      const bool conj = m >= K;
+      const int src = conj ? (int)(N - m) : (int)m;
+      const int li = tiles[ti].lds + spk::lds_pad((j + L / 2) & (L - 1));
+      if (cnt[(size_t)src] >= 2) return SPK_OK;  // a bin with three targets: unfused
+      tgt[(size_t)cnt[(size_t)src]++ * (M + 1) + src] = (uint16_t)(li | (conj ? 0x8000 : 0));
+    }
+  }
+  int D = 1;
+  for (uint8_t k : cnt) D = std::max(D, (int)k);
+  // thread slots: per size class, whole waves of one class
+  std::vector<int> slots;
+  for (int cls = 0; cls < 6; ++cls) {
+    const int L = 8 << cls, TF = L < 16 ? 1 : L / 16;
+    const size_t start = slots.size();
+    for (size_t ti = 0; ti < ids.size(); ++ti)
+      if (c->chans[ids[ti]].L == L)
+        for (int f = 0; f < TF; ++f) slots.push_back(spk::gen_slot((int)ti, f, cls, false));
+    while ((slots.size() - start) % 64) slots.push_back(spk::gen_slot(spare, 0, cls, true));
+  }
+  const int rounds = (int)((slots.size() + T - 1) / T);
+  if (rounds > kGenMaxRounds) return SPK_OK;
+  slots.resize((size_t)rounds * T, spk::gen_slot(spare, 0, 0, true));
+  // lane-major index tables (the Welch partial-row order of the forward kernel)
+  std::vector<uint16_t> tab((size_t)D * M, dummy);
+  for (int d = 0; d < D; ++d)
+    for (int k = 0; k < M; ++k) {
+      const int idx = spk::welch_row_index(k, M, R);
+      const int q = idx / T, t = idx % T;
+      tab[(size_t)d * M + ((size_t)(q >> 3) * T + t) * 8 + (q & 7)] = tgt[(size_t)d * (M + 1) + k];
+    }
+  HIP_TRY(c, upload(c->fscat, tab));
+  HIP_TRY(c, upload(c->gtiles, tiles));
+  HIP_TRY(c, upload(c->gslots, slots));
+  for (int i = 0; i < 4; ++i)
+    if (!c->gentw[i].p) HIP_TRY(c, upload(c->gentw[i], pass_table(2 << i, 16, +1)));
+  c->fscat_nyq = tgt[(size_t)M];
+  c->fscat_nyq2 = tgt[(size_t)(M + 1) + M];
+  c->grounds = rounds;
+  c->fmode = D == 1 ? spk::kFusedGen1 : spk::kFusedGen2;
+  c->fused = true;
+  for (auto& kv : c->groups)
+    if (!kv.second.generic && kv.first >= 8 && kv.first <= 256) kv.second.fused = true;
+  return SPK_OK;
+}
+
 // Lay out channel outputs and descriptors, upload weights and tables.
 // Every captured graph and batch plan refers to the old buffers and is
 // dropped here.
@@ -476,6 +579,10 @@ spk_status rebuild_channels(spk_ctx* c) {
     c->live_chans = (int)recs.size();
     c->live_len = prefix;
   }
+  // 256 zeros after the last channel's weights: the idle lanes of the
+  // general fused stage read them (fuse_general)
+  c->weights_live = wall.size();
+  wall.resize(wall.size() + 256, 0.f);
   HIP_TRY(c, upload(c->weights, wall));
   HIP_TRY(c, c->chan_out.alloc((size_t)off * sizeof(cpx)));
   c->groups.clear();
@@ -527,9 +634,14 @@ spk_status rebuild_channels(spk_ctx* c) {
     HIP_TRY(c, upload(g.atw3, pass_table(p->R3, p->R1 * p->R2, -1)));
   }
   // The forward kernel runs the L = 256 channels itself when one 16-thread
-  // tile per channel fits in its workgroup.
+  // tile per channel fits in its workgroup (the fixed-size stage), or,
+  // otherwise, every channel of a power-of-two IFFT size up to 256 when
+  // their tiles fit its LDS (the general stage, fuse_general).
+  c->fused = false;
+  c->fmode = 0;
   const auto it = c->groups.find(kFusedL);
-  c->fused = !c->two_step && it != c->groups.end() && (int)it->second.ids.size() * 16 <= c->M / c->fplan->R;
+  c->fused = !c->two_step && c->groups.size() == 1 && it != c->groups.end() &&
+             (int)it->second.ids.size() * 16 <= c->M / c->fplan->R;
   if (c->fused) {
     // The fused stage takes channels inside bins [0, K) with no bin shared
     // (get_bin mirrors and overlapping channels go to the channel kernel).
@@ -567,7 +679,13 @@ spk_status rebuild_channels(spk_ctx* c) {
     if (c->fused) {
       HIP_TRY(c, upload(c->fscat, tab));
       c->fscat_nyq = nyq;
+      c->fmode = kFusedL;
+      c->groups.at(kFusedL).fused = true;
     }
+  }
+  if (!c->fused && !c->two_step) {
+    spk_status st = fuse_general(c);
+    if (st != SPK_OK) return st;
   }
   c->chans_dirty = false;
   return SPK_OK;
@@ -594,13 +712,15 @@ struct FwdVariant {
   hipError_t (*launch)(const spk::FwdParams&, int, hipStream_t);
   int (*occupancy)();
   int lds_dummy;  // LDS cpx index of the fused stage's dummy scatter slot
+  int tile_cap;   // cpx of LDS the general fused stage's tiles may use
 };
 #define SPK_FWD_VARIANT1(M, R, R1, R2, FL, F32) \
   {M, R, FL, F32, &launch_fwd<M, R, R1, R2, FL, F32>, &fwd_wgs_per_cu<M, R, R1, R2, FL, F32>, \
-   spk::FwdRealCfg<M, R, R1, R2, FL>::LDS_DUMMY}
-#define SPK_FWD_VARIANT(M, R, R1, R2)                                               \
-  SPK_FWD_VARIANT1(M, R, R1, R2, 0, false), SPK_FWD_VARIANT1(M, R, R1, R2, 256, false), \
-      SPK_FWD_VARIANT1(M, R, R1, R2, 0, true), SPK_FWD_VARIANT1(M, R, R1, R2, 256, true)
+   spk::FwdRealCfg<M, R, R1, R2, FL>::LDS_DUMMY, spk::FwdRealCfg<M, R, R1, R2, FL>::LDS_FLOATS / 2}
+#define SPK_FWD_VARIANT(M, R, R1, R2)                                                               \
+  SPK_FWD_VARIANT1(M, R, R1, R2, 0, false), SPK_FWD_VARIANT1(M, R, R1, R2, 256, false),                 \
+      SPK_FWD_VARIANT1(M, R, R1, R2, 0, true), SPK_FWD_VARIANT1(M, R, R1, R2, 256, true),               \
+      SPK_FWD_VARIANT1(M, R, R1, R2, spk::kFusedGen1, false), SPK_FWD_VARIANT1(M, R, R1, R2, spk::kFusedGen2, false)
 const FwdVariant kFwdVariants[] = {
     SPK_FWD_VARIANT(512, 8, 8, 8),      SPK_FWD_VARIANT(1024, 16, 8, 8),    SPK_FWD_VARIANT(2048, 16, 8, 16),
     SPK_FWD_VARIANT(4096, 16, 16, 16),  SPK_FWD_VARIANT(8192, 32, 16, 16),  SPK_FWD_VARIANT(16384, 32, 16, 32),
@@ -610,6 +730,14 @@ const FwdVariant* find_fwd_variant(int M, int FL, bool f32) {
   for (const auto& v : kFwdVariants)
     if (v.M == M && v.FL == FL && v.f32 == f32) return &v;
   return nullptr;
+}
+int fwd_lds_dummy(int M, int FL) {
+  const FwdVariant* v = find_fwd_variant(M, FL, false);
+  return v ? v->lds_dummy : -1;
+}
+int fwd_tile_cap(int M, int FL) {
+  const FwdVariant* v = find_fwd_variant(M, FL, false);
+  return v ? v->tile_cap : -1;
 }
 int fused_lds_dummy(int M) {
   const FwdVariant* v = find_fwd_variant(M, kFusedL, false);
@@ -699,7 +827,7 @@ int batch_R2(int L) {
 std::vector<int> batched_groups(const spk_ctx* c) {
   std::vector<int> ls;
   for (const auto& kv : c->groups)
-    if (!(c->fused && kv.first == kFusedL) && !kv.second.generic) ls.push_back(kv.first);
+    if (!kv.second.fused && !kv.second.generic) ls.push_back(kv.first);
   return ls;
 }
 
@@ -881,7 +1009,7 @@ hipError_t dispatch_cplx_rows(int N1, const spk::CplxRowParams& P, int nb, bool 
   return hipGetLastError();
 }
 
-spk_status enqueue_channels(spk_ctx* c, int nb, hipStream_t s, const ChanGroup* fg);
+spk_status enqueue_channels(spk_ctx* c, int nb, hipStream_t s);
 
 // Formats without a loader of their own: convert the call's nb * H raw
 // samples into c->conv and return it as the transform's source (the input
@@ -1074,7 +1202,7 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
     HIP_TRY(c, hipGetLastError());
   }
   if (ev) HIP_TRY(c, hipEventRecord(ev[1], s));
-  spk_status st = enqueue_channels(c, nb, s, nullptr);
+  spk_status st = enqueue_channels(c, nb, s);
   if (st != SPK_OK) return st;
   if (ev) HIP_TRY(c, hipEventRecord(ev[2], s));
   {
@@ -1111,14 +1239,14 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
 }
 
 // Channel stage for every channel not fused into the forward kernel.
-spk_status enqueue_channels(spk_ctx* c, int nb, hipStream_t s, const ChanGroup* fg) {
+spk_status enqueue_channels(spk_ctx* c, int nb, hipStream_t s) {
   // sizes without a power-of-two plan have their own kernel in either form
   for (auto& kv : c->groups)
     if (kv.second.generic) HIP_TRY(c, launch_generic(c, kv.second, nb, s));
   if (c->chan_mode == SPK_CHAN_BATCHED) return launch_batched(c, nb, s);
   for (auto& kv : c->groups) {
     ChanGroup& g = kv.second;
-    if (&g == fg || g.generic) continue;
+    if (g.fused || g.generic) continue;
     spk::ChanParams Q{};
     Q.spec = c->spec.as<cpx>();
     Q.spec_stride = c->spec_stride;
@@ -1138,7 +1266,8 @@ spk_status enqueue_channels(spk_ctx* c, int nb, hipStream_t s, const ChanGroup* 
 }
 
 // Workgroups of the real forward launch over nb blocks.
-int fwd_grid(const spk_ctx* c, int nb) { return std::min(nb, c->gmax[c->fused ? 1 : 0]); }
+int fwd_class(int fmode) { return fmode == 0 ? 0 : (fmode == kFusedL ? 1 : 2); }
+int fwd_grid(const spk_ctx* c, int nb) { return std::min(nb, c->gmax[fwd_class(c->fmode)]); }
 
 // Welch record parameters of the real-input call over nb blocks launched
 // with G workgroups at the given parity (its position slot, partial rows
@@ -1187,7 +1316,8 @@ spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hi
   P.spec_stride = c->spec_stride;
   // Per-block spectrum rows are needed when kept or read by a channel
   // kernel; otherwise each workgroup reuses one scratch row.
-  const size_t unfused_groups = c->groups.size() - (c->fused ? 1 : 0);
+  size_t unfused_groups = 0;
+  for (const auto& kv : c->groups) unfused_groups += kv.second.fused ? 0 : 1;
   P.keep_spec = (c->keep_spec || unfused_groups > 0) ? 1 : 0;
   P.tw2 = c->ftw2.as<cpx>();
   P.tw3 = c->ftw3.as<cpx>();
@@ -1203,9 +1333,8 @@ spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hi
   P.O = c->O;
   P.edump = c->edump.as<cpx>();
   P.eitems = c->eitems.as<int4>();
-  const ChanGroup* fg = nullptr;
-  if (c->fused) {
-    fg = &c->groups.at(kFusedL);
+  if (c->fmode == kFusedL) {
+    const ChanGroup* fg = &c->groups.at(kFusedL);
     P.fchans = fg->desc.as<spk::ChanDesc>();
     P.fnch = (int)fg->ids.size();
     P.fscat = c->fscat.as<uint4>();
@@ -1223,13 +1352,24 @@ spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hi
   P.ncounters = kXcdQueues;
   if (c->pend_nb > 0) P.fin = fin_params(c, c->pend_nb, c->pend_G, c->pend_parity);
   P.fin_ticket = c->fin_ticket.as<int>();
-  const FwdVariant* fv = find_fwd_variant(c->M, c->fused ? kFusedL : 0, c->f32_in);
+  if (spk::fused_general(c->fmode)) {
+    P.fscat = c->fscat.as<uint4>();
+    P.fscat_nyq = c->fscat_nyq;
+    P.fscat_nyq2 = c->fscat_nyq2;
+    P.weights = c->weights.as<float>();
+    P.fout = c->chan_out.as<cpx>();
+    P.gtiles = c->gtiles.as<spk::GenTile>();
+    P.gslots = c->gslots.as<int>();
+    P.grounds = c->grounds;
+    for (int i = 0; i < 4; ++i) P.gtw[i] = c->gentw[i].as<cpx>();
+  }
+  const FwdVariant* fv = find_fwd_variant(c->M, c->fmode, c->f32_in);
   const int G = fwd_grid(c, nb);
   HIP_TRY(c, fv->launch(P, G, s));
   if (ev) HIP_TRY(c, hipEventRecord(ev[1], s));
 
   {
-    spk_status st = enqueue_channels(c, nb, s, fg);
+    spk_status st = enqueue_channels(c, nb, s);
     if (st != SPK_OK) return st;
   }
   if (ev) HIP_TRY(c, hipEventRecord(ev[2], s));
@@ -1245,7 +1385,9 @@ spk_status enqueue_blocks(spk_ctx* c, const void* src, int nb, hipStream_t s, hi
 // launching the two kernels directly.
 bool graph_pays(const spk_ctx* c) {
   if (c->two_step) return true;
-  return c->groups.size() > (c->fused ? 1u : 0u);
+  for (const auto& kv : c->groups)
+    if (!kv.second.fused) return true;
+  return false;
 }
 
 // One launch sequence over nb blocks whose samples start at src.  With
@@ -1486,14 +1628,21 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
     *out = c;
     return SPK_OK;
   }
-  for (int f = 0; f < 2; ++f) {
-    const FwdVariant* v = find_fwd_variant(c->M, f ? kFusedL : 0, c->f32_in);
-    const int per_cu = v ? v->occupancy() : 0;
+  for (int f = 0; f < 3; ++f) {
+    // class 2: the two general fused instantiations (s16 loader only)
+    int per_cu = 1 << 30;
+    for (int fl : f == 0 ? std::vector<int>{0} : (f == 1 ? std::vector<int>{kFusedL}
+                                                         : std::vector<int>{spk::kFusedGen1, spk::kFusedGen2})) {
+      const FwdVariant* v = find_fwd_variant(c->M, fl, c->f32_in);
+      if (!v && f == 2) continue;
+      per_cu = std::min(per_cu, v ? v->occupancy() : 0);
+    }
+    if (f == 2 && per_cu == (1 << 30)) per_cu = c->gmax[0] / ncu;  // no general stage for this loader
     if (per_cu <= 0) return cleanup(SPK_ERR_DEVICE);
     c->gmax[f] = per_cu * ncu;
   }
-  const int gcap = std::max(c->gmax[0], c->gmax[1]);
-  const int gmin = std::min(c->gmax[0], c->gmax[1]);
+  const int gcap = std::max(c->gmax[0], std::max(c->gmax[1], c->gmax[2]));
+  const int gmin = std::min(c->gmax[0], std::min(c->gmax[1], c->gmax[2]));
   // The Welch kernels index blocks times workgroups in 32 bits.
   if ((long long)(c->max_blocks + c->A) * (gcap + 1) >= (1LL << 31)) return cleanup(SPK_ERR_UNSUPPORTED);
   // A workgroup spans at most ceil(ceil(max_blocks / G) / A) + 1 Welch groups.
@@ -2266,7 +2415,7 @@ spk_status spk_autotune(spk_ctx* c, uint32_t nblocks, spk_tune_result* out) {
   const float per_size = time_it([&]() -> spk_status {
     for (auto& kv : c->groups) {
       ChanGroup& g = kv.second;
-      if ((c->fused && kv.first == kFusedL) || g.generic) continue;  // same kernel in both forms
+      if (g.fused || g.generic) continue;  // same kernel in both forms
       spk::ChanParams Q{};
       Q.spec = c->spec.as<cpx>();
       Q.spec_stride = c->spec_stride;
@@ -2339,7 +2488,10 @@ spk_status spk_fused_channels(spk_ctx* c, uint32_t* n) {
     spk_status st = rebuild_channels(c);
     if (st != SPK_OK) return st;
   }
-  *n = c->fused ? (uint32_t)c->groups.at(kFusedL).ids.size() : 0u;
+  uint32_t nf = 0;
+  for (const auto& kv : c->groups)
+    if (kv.second.fused) nf += (uint32_t)kv.second.ids.size();
+  *n = nf;
   return SPK_OK;
   } catch (...) {
     return entry_exception(c);

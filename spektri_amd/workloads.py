@@ -282,7 +282,24 @@ def design_16k(nsamples: int | None = None) -> Workload:
     return wl
 
 
-WORKLOADS = {"plain_8k": plain_8k, "rx888_64k": rx888_64k, "rx888_256k": rx888_256k, "mixed_48k": mixed_48k, "uneven_64k": uneven_64k,
+def hfrx1(nsamples: int | None = None) -> Workload:
+    """The reference's HF receiver script (hfrx/hfrx1.sh:4-5,40-53): real s16
+    at 131.072 MS/s, N = 65536 (2 kHz bins), Welch over 24000 blocks, six
+    64 kHz channels (fs=64000, L = 32) at 80 kHz, 1.848, 3.696, 4.624,
+    7.056 and 14.264 MHz with the reference's raised-cosine weights;
+    rx888_64k's signal, 2^26 samples at F = 8 (1360 blocks)."""
+    base = rx888_64k(nsamples)
+    wl = Workload("hfrx1", 5, base.fft_size, base.fs, 8, 24000, base.nsamples)
+    wl.tones, wl.sweep_amp, wl.noise_sigma = base.tones, base.sweep_amp, base.noise_sigma
+    df = wl.fs / wl.fft_size
+    for fc in (80e3, 1848e3, 3696e3, 4624e3, 7056e3, 14264e3):
+        L = int(round(64000 / df / 4)) * 4            # fcfb.rs:225-248 (exact here)
+        c = int(round(fc / df / 4)) * 4
+        wl.channels.append((c - L // 2, L, None))
+    return wl
+
+
+WORKLOADS = {"plain_8k": plain_8k, "hfrx1": hfrx1, "rx888_64k": rx888_64k, "rx888_256k": rx888_256k, "mixed_48k": mixed_48k, "uneven_64k": uneven_64k,
              "wideband_256k": wideband_256k,
              "design_16k": design_16k}
 

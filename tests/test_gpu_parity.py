@@ -200,9 +200,10 @@ def test_back_to_back_calls_defer_welch_records(N, fused):
     F, A, calls = 4, 6, 6
     K, H = N // 2 + 1, N - N // 4
     x = np.clip(np.round(rng.standard_normal(calls * F * H) * 4000), -32768, 32767).astype(np.int16)
-    # a channel reaching below bin 0 cannot run inside the forward kernel
-    chans = [(300 if fused else -100, 256, rng.random(256).astype(np.float32)),
-             (K // 2, 256, rng.random(256).astype(np.float32))]
+    # channels of 512 bins run in the channel kernels
+    L = 256 if fused else 512
+    chans = [(300 if fused else -100, L, rng.random(L).astype(np.float32)),
+             (K // 2, L, rng.random(L).astype(np.float32))]
     dev = torch.from_numpy(x).to("cuda")
     seg = F * H
     res = {}

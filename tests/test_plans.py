@@ -121,3 +121,11 @@ def test_channel_plan_addresses(plan):
             u, q = divmod(e, rl)
             seen.add(t + T * u + q * pl)
     assert seen == set(range(L))
+
+
+@pytest.mark.parametrize("plan", [(32, 16, 16, 2, 1), (64, 16, 16, 4, 1), (128, 16, 16, 8, 1), (256, 16, 16, 16, 1)])
+def test_general_fused_tile_plans(plan):
+    """Tiles of the forward kernel's general fused stage: FL = 16 x FL/16,
+    16 points per thread (gen_tile / GenTileCfg); 8 and 16 points are one
+    thread's in-register DFT."""
+    test_channel_plan_addresses(plan)
