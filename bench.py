@@ -43,7 +43,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBPS = 8000.0
 NCOPIES = 3
-OTHER_CONFIGS = ("design_16k", "plain_8k", "uneven_64k", "wideband_256k")  # measured in this order
+OTHER_CONFIGS = ("design_16k", "plain_8k", "hfrx1", "uneven_64k", "wideband_256k")  # measured in this order
 
 
 def parse():
@@ -168,7 +168,7 @@ def fwd_kernel_name(wl, all_fused):
         return "cplx_row_kernel + cplx_col_kernel"
     if wl.fft_size > 65536:
         return "cplx_row_kernel + cplx_col_kernel + untangle_kernel"
-    return f"fwd_real_kernel<M={wl.fft_size // 2}, FL={256 if all_fused else 0}>"
+    return f"fwd_real_kernel<M={wl.fft_size // 2}>" + (" (channel stage fused)" if all_fused else "")
 
 
 def prepare_stream(wl, x, args, ncopies):

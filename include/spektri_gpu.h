@@ -260,10 +260,24 @@ typedef struct spk_tune_result {
 spk_status spk_autotune(spk_ctx* ctx, uint32_t nblocks, spk_tune_result* out);
 
 /* Number of channels whose gather, weighting, IFFT and discard run inside
- * the forward kernel (all channels of IFFT size 256 when they lie in
- * [0, K), do not overlap and fit the workgroup; 0 otherwise).  No
- * counterpart in the reference (plan introspection). */
+ * the forward kernel (real input up to N = 65536): every channel when all
+ * have IFFT size 256, lie in [0, K), do not overlap and fit the workgroup's
+ * 16-thread tiles; otherwise every channel of a power-of-two IFFT size from
+ * 8 to 256 when their tiles fit the kernel's LDS and no bin feeds more than
+ * two tile positions (overlapping channels, get_bin's mirror at DC and
+ * Nyquist); 0 otherwise.  No counterpart in the reference (plan
+ * introspection). */
 spk_status spk_fused_channels(spk_ctx* ctx, uint32_t* n);
+
+/* Captured launch sequences (SPK_FLAG_GRAPHS): executable graphs held now,
+ * graphs instantiated and graphs updated in place since the context was
+ * created.  A channel added, removed or reweighted does not drop the
+ * graphs: the next call with the same input pointer and block count
+ * captures its launch sequence again and updates the executable graph
+ * (hipGraphExecUpdate); only a changed launch topology (another kernel
+ * count or kernel) instantiates a new one.  No counterpart in the reference
+ * (plan introspection).  Any pointer may be NULL. */
+spk_status spk_graph_info(spk_ctx* ctx, uint32_t* live, uint32_t* instantiated, uint32_t* updated);
 
 /* ---- Differentiable channel weights (no counterpart in the reference; the
  * reference's weights are fixed raised cosines, fcfb.rs:197-203).  Packed

@@ -92,6 +92,8 @@ _SIGS = {
     "spk_enable_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "spk_last_timing": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float)]),
     "spk_fused_channels": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32)]),
+    "spk_graph_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                      ctypes.POINTER(ctypes.c_uint32)]),
     "spk_set_weights_device": (ctypes.c_int, [_P, _P]),
     "spk_set_channel_mode": (ctypes.c_int, [_P, ctypes.c_int]),
     "spk_autotune": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.POINTER(SpkTuneResult)]),

@@ -92,6 +92,10 @@ struct FinParams {
   int ncounters;
   unsigned char* qout;  // quantised records (qfmt 1: K bytes, 2: K big-endian u16) or null
   int qfmt;
+  // Forward kernel's record phase: S teams of T / S threads share a chunk of
+  // 4 T / S positions, team s summing the s-th range of the group's
+  // workgroups (a power of two; 1 = one team).
+  int split;
 };
 
 struct FwdParams {
@@ -380,29 +384,26 @@ __device__ __forceinline__ void quantize_bin(unsigned char* q, int qfmt, long lo
 }
 
 
-// One thread's share of a Welch group gi of a launch whose first block is
-// at stream position pos: partial-row positions i0 .. i0+3 (or the Nyquist
-// position M alone, M % 4 == 0) summed over the workgroups that processed
-// the group's blocks, plus the carry of a group begun in an earlier call.  A
-// complete group is written as records (bins by welch_row_bin), an
-// incomplete one as the next carry.
-__device__ __forceinline__ void welch_finalize_quad(const FinParams& F, long long pos, int gi, int i0) {
-  const bool nyq = i0 == F.M;
-  const int blk0_mod = (int)(pos % F.A);
+// Workgroups [w0, w1] of a forward launch over F.nblocks blocks that
+// processed blocks of Welch group gi (blk0_mod = stream position of the
+// launch's block 0 mod A): the w with floor(w nb / G) <= b < floor((w+1) nb / G).
+__device__ __forceinline__ void welch_group_wgs(const FinParams& F, int blk0_mod, int gi, int* w0, int* w1) {
   const int g_lo = gi * F.A - blk0_mod, g_hi = g_lo + F.A;
   const int b_lo = g_lo > 0 ? g_lo : 0;
   const int b_hi = g_hi < F.nblocks ? g_hi : F.nblocks;
-  float4 s = {0.f, 0.f, 0.f, 0.f};
-  if (gi == 0 && blk0_mod != 0) {
-    if (nyq) s.x = F.carry_in[i0];
-    else s = *reinterpret_cast<const float4*>(F.carry_in + i0);
-  }
-  // workgroup of block b: the w with floor(w nb / G) <= b < floor((w+1) nb / G)
   const unsigned nb = (unsigned)F.nblocks, G = (unsigned)F.G;
-  const int w0 = (int)(((unsigned)(b_lo + 1) * G + nb - 1) / nb) - 1;
-  const int w1 = (int)(((unsigned)b_hi * G + nb - 1) / nb) - 1;
+  *w0 = (int)(((unsigned)(b_lo + 1) * G + nb - 1) / nb) - 1;
+  *w1 = (int)(((unsigned)b_hi * G + nb - 1) / nb) - 1;
+}
+
+// s + the partial rows of workgroups wa..wb (in order) at positions
+// i0 .. i0+3 (or position M alone) of group gi.
+__device__ __forceinline__ float4 welch_rows_add(const FinParams& F, int blk0_mod, int gi, int i0, int wa, int wb,
+                                                 float4 s) {
+  const bool nyq = i0 == F.M;
+  const unsigned nb = (unsigned)F.nblocks, G = (unsigned)F.G;
 #pragma unroll 2
-  for (int wg = w0; wg <= w1; ++wg) {
+  for (int wg = wa; wg <= wb; ++wg) {
     const int lo = (int)((unsigned)wg * nb / G);
     const int slot = gi - (blk0_mod + lo) / F.A;
     const float* row = F.wpart + ((long long)wg * F.wslots + slot) * F.wstride;
@@ -416,7 +417,24 @@ __device__ __forceinline__ void welch_finalize_quad(const FinParams& F, long lon
       s.w += v.w;
     }
   }
-  if (g_hi > F.nblocks) {
+  return s;
+}
+
+// The carry of a group begun in an earlier call (group 0 only).
+__device__ __forceinline__ float4 welch_carry(const FinParams& F, int blk0_mod, int gi, int i0) {
+  float4 s = {0.f, 0.f, 0.f, 0.f};
+  if (gi == 0 && blk0_mod != 0) {
+    if (i0 == F.M) s.x = F.carry_in[i0];
+    else s = *reinterpret_cast<const float4*>(F.carry_in + i0);
+  }
+  return s;
+}
+
+// A finished sum: records acc / A (bins by welch_row_bin) of a complete
+// group, the next carry of an incomplete one.
+__device__ __forceinline__ void welch_emit(const FinParams& F, int blk0_mod, int gi, int i0, float4 s) {
+  const bool nyq = i0 == F.M;
+  if (gi * F.A - blk0_mod + F.A > F.nblocks) {
     if (nyq) F.carry_out[i0] = s.x;
     else *reinterpret_cast<float4*>(F.carry_out + i0) = s;
     return;
@@ -431,22 +449,20 @@ __device__ __forceinline__ void welch_finalize_quad(const FinParams& F, long lon
   }
 }
 
-// Persistent workgroups: workgroup w takes blocks [w nb / G, (w+1) nb / G)
-// in order, so its Welch sums stay in a private partial row (slot = Welch
-// group - first group of the workgroup) that is written with plain stores.
-// The rows of every group are added in the next launch (workgroups that are
-// done with their blocks take chunks of the previous call's groups) or by
-// welch_finalize_kernel when the records are asked for first.
-//
-// After the last pass every lane owns R bins of the real spectrum: lower
-// lanes pi + P3 q, upper lanes ksrc + P3 (R-1-q) (slot q).  A bin's
-// neighbours k +- 1 sit in the adjacent lanes of the same slot, so Welch
-// takes them by DPP wave shifts (lower lanes: left = lane-1, right =
-// lane+1; upper lanes the other way round).  Lanes where that is wrong (the
-// wave edges, and the wrap-around lanes of waves 0 and W-1) are finished
-// after a barrier from the edge table.  Channels of the fused stage are
-// gathered by scattering each owned bin into an LDS tile buffer through the
-// bin -> position map.
+// One thread's share of a Welch group gi of a launch whose first block is
+// at stream position pos: partial-row positions i0 .. i0+3 (or the Nyquist
+// position M alone, M % 4 == 0) summed over the workgroups that processed
+// the group's blocks, plus the carry of a group begun in an earlier call.  A
+// complete group is written as records (bins by welch_row_bin), an
+// incomplete one as the next carry.
+__device__ __forceinline__ void welch_finalize_quad(const FinParams& F, long long pos, int gi, int i0) {
+  const int blk0_mod = (int)(pos % F.A);
+  int w0, w1;
+  welch_group_wgs(F, blk0_mod, gi, &w0, &w1);
+  const float4 s = welch_rows_add(F, blk0_mod, gi, i0, w0, w1, welch_carry(F, blk0_mod, gi, i0));
+  welch_emit(F, blk0_mod, gi, i0, s);
+}
+
 // One thread's part of a general fused tile (fcfb.rs:154-188): weight the
 // gathered bins, inverse FFT of FL points in the tile's LDS region, keep
 // [FL/8, 7FL/8).  The tile's threads lie in one wave and only they touch its
@@ -492,6 +508,22 @@ __device__ __forceinline__ void gen_put(cpx* ldsc, unsigned entry, cpx X) {
   ldsc[entry & 0x7FFFu] = v;
 }
 
+// Persistent workgroups: workgroup w takes blocks [w nb / G, (w+1) nb / G)
+// in order, so its Welch sums stay in a private partial row (slot = Welch
+// group - first group of the workgroup) that is written with plain stores.
+// The rows of every group are added in the next launch (workgroups that are
+// done with their blocks take chunks of the previous call's groups) or by
+// welch_finalize_kernel when the records are asked for first.
+//
+// After the last pass every lane owns R bins of the real spectrum: lower
+// lanes pi + P3 q, upper lanes ksrc + P3 (R-1-q) (slot q).  A bin's
+// neighbours k +- 1 sit in the adjacent lanes of the same slot, so Welch
+// takes them by DPP wave shifts (lower lanes: left = lane-1, right =
+// lane+1; upper lanes the other way round).  Lanes where that is wrong (the
+// wave edges, and the wrap-around lanes of waves 0 and W-1) are finished
+// after a barrier from the edge table.  Channels of the fused stage are
+// gathered by scattering each owned bin into an LDS tile buffer through the
+// bin -> position map.
 template <int M, int R, int R1, int R2, int FL, bool F32>
 __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   using C = FwdRealCfg<M, R, R1, R2, FL>;
@@ -901,10 +933,17 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     // (its position slot is this launch's pos_next, already rewritten)
     const long long fpos = block0 - P.fin.nblocks;
     const int ngroups = (int)(fpos % P.fin.A + P.fin.nblocks - 1) / P.fin.A + 1;
-    constexpr int CPG = R / 4;  // chunks per group (M = R T positions + the Nyquist position)
+    // chunks of 4 T / S positions of one group (M = R T positions + the
+    // Nyquist position); S teams sum S ranges of the group's workgroups,
+    // then team 0 adds the teams' sums in order (S = 1: one team)
+    const int S = P.fin.split, TS = T / S;
+    const int CPG = (R / 4) * S;
     const int nchunks = ngroups * CPG;
+    const int blk0_mod = (int)(fpos % P.fin.A);
+    const int team = (int)threadIdx.x / TS, u = (int)threadIdx.x % TS;
+    float4* stage = reinterpret_cast<float4*>(lds);
     for (;;) {
-      __syncthreads();  // the previous ticket has been read
+      __syncthreads();  // the previous ticket (and stage) has been read
       if (threadIdx.x == 0) *ticket_slot = atomicAdd(P.fin_ticket, 1);
       __syncthreads();
       const int ticket = *ticket_slot;
@@ -913,8 +952,42 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
         break;
       }
       const int gi = ticket / CPG, c = ticket % CPG;
-      welch_finalize_quad(P.fin, fpos, gi, 4 * (c * T + (int)threadIdx.x));
-      if (c == CPG - 1 && threadIdx.x == 0) welch_finalize_quad(P.fin, fpos, gi, M);
+      const int i0 = 4 * (c * TS + u);
+      if (S == 1) {
+        welch_finalize_quad(P.fin, fpos, gi, i0);
+      } else {
+        int w0, w1;
+        welch_group_wgs(P.fin, blk0_mod, gi, &w0, &w1);
+        const int nw = w1 - w0 + 1;
+        const int wa = w0 + nw * team / S, wb = w0 + nw * (team + 1) / S - 1;
+        const float4 z = {0.f, 0.f, 0.f, 0.f};
+        stage[team * TS + u] =
+            welch_rows_add(P.fin, blk0_mod, gi, i0, wa, wb, team == 0 ? welch_carry(P.fin, blk0_mod, gi, i0) : z);
+        // the last chunk of a group also carries the Nyquist position
+        float* nstage = reinterpret_cast<float*>(stage + T);
+        const bool last = c == CPG - 1;
+        if (last && u == 0)
+          nstage[team] =
+              welch_rows_add(P.fin, blk0_mod, gi, M, wa, wb, team == 0 ? welch_carry(P.fin, blk0_mod, gi, M) : z).x;
+        __syncthreads();
+        if (team == 0) {
+          float4 acc = stage[u];
+          for (int k = 1; k < S; ++k) {
+            const float4 v = stage[k * TS + u];
+            acc.x += v.x;
+            acc.y += v.y;
+            acc.z += v.z;
+            acc.w += v.w;
+          }
+          welch_emit(P.fin, blk0_mod, gi, i0, acc);
+          if (last && u == 0) {
+            float n = nstage[0];
+            for (int k = 1; k < S; ++k) n += nstage[k];
+            welch_emit(P.fin, blk0_mod, gi, M, float4{n, 0.f, 0.f, 0.f});
+          }
+        }
+      }
+      if (S == 1 && c == CPG - 1 && threadIdx.x == 0) welch_finalize_quad(P.fin, fpos, gi, M);
     }
   }
 }

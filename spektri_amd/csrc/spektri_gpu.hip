@@ -341,7 +341,17 @@ struct spk_ctx {
   long long live_len = 0;    // samples per block over the live channels
   int qfmt = 0;              // 0 none, 1 U8, 2 U16
   bool use_graphs = false;
-  std::map<std::tuple<const void*, int, int, int>, hipGraphExec_t> graphs;
+  // Captured launch sequences per (input pointer, blocks, parity, pending
+  // blocks, pending grid).  A channel change marks them stale: the next call
+  // with a key captures its sequence again and updates the executable graph
+  // in place (hipGraphExecUpdate), instantiating anew only when the launch
+  // topology changed.
+  struct GraphEntry {
+    hipGraphExec_t exec = nullptr;
+    bool stale = false;
+  };
+  std::map<std::tuple<const void*, int, int, int, int>, GraphEntry> graphs;
+  uint32_t graphs_instantiated = 0, graphs_updated = 0;
   long long spec_stride = 0, wstride = 0;
   int wslots = 0, welch_out_cap = 0;
   long long wpart_half = 0;  // floats per parity of the partial rows (real input)
@@ -440,7 +450,7 @@ std::vector<float> raised_cosine_weights(int L) {
 int fused_lds_dummy(int M);
 
 void drop_graphs(spk_ctx* c) {
-  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
   c->graphs.clear();
 }
 // Graphs hold device pointers of the batch plans: free plans only with them.
@@ -546,13 +556,15 @@ spk_status fuse_general(spk_ctx* c) {
 }
 
 // Lay out channel outputs and descriptors, upload weights and tables.
-// Every captured graph and batch plan refers to the old buffers and is
-// dropped here.
+// Every captured graph and batch plan refers to the old buffers: the plans
+// are freed and the graphs marked stale (run_blocks updates a stale graph
+// before it is launched again).
 spk_status rebuild_channels(spk_ctx* c) {
   // Tables are uploaded with null-stream copies, which do not wait for the
   // context stream: no launch of an earlier call may still read them.
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  drop_graphs_and_plans(c);
+  for (auto& kv : c->graphs) kv.second.stale = true;
+  c->batches.clear();
   std::vector<float> wall;
   c->chan_out_off.assign(c->chans.size(), -1);
   c->chan_w_off.assign(c->chans.size(), -1);
@@ -1289,6 +1301,16 @@ spk::FinParams fin_params(const spk_ctx* c, int nb, int G, int parity) {
   F.out = c->welch_out.as<float>();
   F.qout = c->qout.as<unsigned char>();
   F.qfmt = c->qfmt;
+  // Teams per chunk of the forward kernel's record phase: enough chunks for
+  // the G workgroups when a call touches few Welch groups (long averages);
+  // from quantities fixed per launch shape, so a captured graph sums in the
+  // same order as an eager launch.
+  const int T = c->M / c->fplan->R;
+  const int ngroups = (c->A - 1 + nb - 1) / c->A + 1;
+  const int chunks = ngroups * (c->fplan->R / 4);
+  int split = 1;
+  while (split < 16 && 2 * split * chunks <= G && T / (2 * split) >= 64) split *= 2;
+  F.split = split;
   return F;
 }
 
@@ -1400,13 +1422,17 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
   }
   hipStream_t s = c->stream;
   if (c->use_graphs && !c->timing && graph_pays(c)) {
-    const auto key = std::make_tuple(src, nb, c->parity, c->pend_nb);
+    const auto key = std::make_tuple(src, nb, c->parity, c->pend_nb, c->pend_nb ? c->pend_G : 0);
     auto it = c->graphs.find(key);
-    if (it == c->graphs.end()) {
-      if (c->graphs.size() >= 8) drop_graphs(c);
+    if (it == c->graphs.end() || it->second.stale) {
+      if (it == c->graphs.end() && c->graphs.size() >= kMaxCachedPlans) {
+        drop_graphs(c);
+        it = c->graphs.end();
+      }
       if (c->chan_mode == SPK_CHAN_BATCHED && !c->batches.count(nb)) {
         spk_status st = build_batch(c, nb);  // uploads happen outside the capture
         if (st != SPK_OK) return st;
+        it = c->graphs.find(key);  // (build_batch may have dropped every graph)
       }
       hipGraph_t graph = nullptr;
       HIP_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1417,13 +1443,35 @@ spk_status run_blocks(spk_ctx* c, const void* src, int nb) {
         return st;
       }
       HIP_TRY(c, e);
-      hipGraphExec_t exec = nullptr;
-      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      bool updated = false;
+      if (it != c->graphs.end()) {
+        hipGraphNode_t err_node = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        e = hipGraphExecUpdate(it->second.exec, graph, &err_node, &res);
+        if (e == hipSuccess && res == hipGraphExecUpdateSuccess) {
+          updated = true;
+          it->second.stale = false;
+          ++c->graphs_updated;
+        } else {
+          (void)hipGetLastError();  // a different topology: instantiate anew
+          (void)hipGraphExecDestroy(it->second.exec);
+          c->graphs.erase(it);
+          it = c->graphs.end();
+        }
+      }
+      if (!updated) {
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+          (void)hipGraphDestroy(graph);
+          return hip_fail(c, e, "hipGraphInstantiate");
+        }
+        ++c->graphs_instantiated;
+        it = c->graphs.emplace(key, spk_ctx::GraphEntry{exec, false}).first;
+      }
       (void)hipGraphDestroy(graph);
-      HIP_TRY(c, e);
-      it = c->graphs.emplace(key, exec).first;
     }
-    HIP_TRY(c, hipGraphLaunch(it->second, s));
+    HIP_TRY(c, hipGraphLaunch(it->second.exec, s));
   } else {
     hipEvent_t* ev = nullptr;
     if (c->timing) {
@@ -2476,6 +2524,18 @@ spk_status spk_autotune(spk_ctx* c, uint32_t nblocks, spk_tune_result* out) {
     out->mode = c->chan_mode;
   }
   return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
+}
+
+spk_status spk_graph_info(spk_ctx* c, uint32_t* live, uint32_t* instantiated, uint32_t* updated) {
+  try {
+    if (!c) return SPK_ERR_INVALID;
+    if (live) *live = (uint32_t)c->graphs.size();
+    if (instantiated) *instantiated = c->graphs_instantiated;
+    if (updated) *updated = c->graphs_updated;
+    return SPK_OK;
   } catch (...) {
     return entry_exception(c);
   }

@@ -258,6 +258,12 @@ class DspState:
                 "rounds_per_item": r.rounds_per_item, "wgs_per_cu": r.wgs_per_cu,
                 "mode": "batched" if r.mode == _lib.CHAN_BATCHED else "per_size"}
 
+    def graph_info(self) -> dict:
+        """Executable graphs held, instantiated and updated in place so far."""
+        v = [ctypes.c_uint32() for _ in range(3)]
+        _lib.check(_lib.load().spk_graph_info(self._ctx, *[ctypes.byref(x) for x in v]), self._ctx)
+        return {"live": v[0].value, "instantiated": v[1].value, "updated": v[2].value}
+
     def fused_channels(self) -> int:
         """Channels whose stage runs inside the forward kernel."""
         n = ctypes.c_uint32()
