@@ -164,6 +164,10 @@ double spkport_run(const int16_t* x, int64_t nbuf_max, int N, int F, int A, int 
   const int O = N / 4, H = N - O, K = N / 2 + 1;
   const float scaling = 1.0f / 32767.0f;
   const float wv = scaling / (float)N; /* dsp.rs:70-73 */
+  if (nch < 0 || nch > 1 << 20 || N <= 0 || F <= 0 || A <= 0) {
+    *done = 0;
+    return 0.0;
+  }
   if (threads > 0) omp_set_num_threads(threads);
   fft_plan fwd;
   plan_init(&fwd, N, -1);

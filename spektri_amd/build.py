@@ -47,18 +47,25 @@ def build_lib(verbose: bool = True) -> str:
     return LIB
 
 
+HOST_BIN_SAN = os.path.join(HOST_DIR, "spektri-gpu-san")
+
+
 def build_host(verbose: bool = True) -> str:
     """The spektri-gpu command-line host (host/main.cpp), linked against the
-    library with a run path to it."""
-    tmp = HOST_BIN + ".tmp"
-    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
-           os.path.join(HOST_DIR, "main.cpp"), "-L", HERE, "-lspektri_gpu", "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
-    if verbose:
-        print("+", " ".join(cmd), flush=True)
-    proc = subprocess.run(cmd, capture_output=True, text=True)
-    if proc.returncode != 0:
-        raise RuntimeError(f"host build failed ({proc.returncode}):\n" + proc.stdout + proc.stderr)
-    os.replace(tmp, HOST_BIN)
+    library with a run path to it, and the same host built with AddressSanitizer
+    and UBSan (host code only, for the CPU tests; the library it loads is the
+    ordinary build)."""
+    for out, extra in ((HOST_BIN, ["-O2"]),
+                       (HOST_BIN_SAN, ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined"])):
+        tmp = out + ".tmp"
+        cmd = [os.environ.get("CXX", "g++"), *extra, "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+               os.path.join(HOST_DIR, "main.cpp"), "-L", HERE, "-lspektri_gpu", "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
+        if verbose:
+            print("+", " ".join(cmd), flush=True)
+        proc = subprocess.run(cmd, capture_output=True, text=True)
+        if proc.returncode != 0:
+            raise RuntimeError(f"host build failed ({proc.returncode}):\n" + proc.stdout + proc.stderr)
+        os.replace(tmp, out)
     return HOST_BIN
 
 

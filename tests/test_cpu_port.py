@@ -95,3 +95,17 @@ def test_design_16k_default_weights():
     wl = W.design_16k()
     x = W.make_signal_np(wl)
     check(wl, x, ("pocketfft+c",), 256, chan_ids=range(0, 256, 17))
+
+
+def test_port_under_sanitizers():
+    """spektri_port.c's entry points under AddressSanitizer and UBSan
+    (oracle/port_check.c, built by oracle/Makefile): no report, and the
+    whole-loop and post-FFT paths agree."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(P.PORT_LIB), "port_check_san")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/port_check_san not built")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, ASAN_OPTIONS="halt_on_error=1", UBSAN_OPTIONS="halt_on_error=1"))
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr

@@ -8,16 +8,21 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "spektri_amd", "host", "spektri-gpu")
+# the same host under AddressSanitizer and UBSan (host code only)
+BIN_SAN = os.path.join(ROOT, "spektri_amd", "host", "spektri-gpu-san")
 
 pytestmark = pytest.mark.skipif(not os.path.exists(BIN), reason="host binary not built (python -m spektri_amd.build)")
+BINS = [BIN] + ([BIN_SAN] if os.path.exists(BIN_SAN) else [])
 
 
-def run(*args):
-    return subprocess.run([BIN, *args], capture_output=True, text=True, timeout=30)
+def run(*args, binary=BIN):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1", UBSAN_OPTIONS="halt_on_error=1")
+    return subprocess.run([binary, *args], capture_output=True, text=True, timeout=60, env=env)
 
 
-def test_help_lists_the_reference_options():
-    r = run("--help")
+@pytest.mark.parametrize("binary", BINS)
+def test_help_lists_the_reference_options(binary):
+    r = run("--help", binary=binary)
     assert r.returncode == 0
     for opt in ("--samplerate", "--centerfreq", "--fftsize", "--fftbuf", "--averages", "--inputformat",
                 "--spectrumformat", "--filters", "--zmqbind"):
@@ -34,6 +39,7 @@ def test_help_lists_the_reference_options():
     (("-s", "abc", "-I", "u8"), "invalid value"),
     (("-x", "1"), "unknown option"),
 ])
-def test_invalid_options_exit_with_status_2(args, msg):
-    r = run(*args)
+@pytest.mark.parametrize("binary", BINS)
+def test_invalid_options_exit_with_status_2(args, msg, binary):
+    r = run(*args, binary=binary)
     assert r.returncode == 2 and msg in r.stderr, (r.returncode, r.stderr)
