@@ -195,10 +195,12 @@ def prepare_stream(wl, x, args, ncopies):
     return {"wl": wl, "st": st, "copies": copies, "tune": tune, "dev": x.device}
 
 
-def measure_stream(prep, steps, warmup, args):
+def measure_stream(prep, steps, warmup, args, keep=None):
     """Time ``steps`` passes of the path over the prepared stream (after
     ``warmup``) and break the step down per kernel; returns the figures of
-    one workload and releases its context and copies."""
+    one workload and releases its context and copies (or, with ``keep``,
+    appends them there to be released later: freeing gigabytes takes long
+    enough for the GPU clock to drop before the next measurement)."""
     import torch
 
     wl, st, copies, tune, dev = prep["wl"], prep["st"], prep["copies"], prep["tune"], prep["dev"]
@@ -247,8 +249,11 @@ def measure_stream(prep, steps, warmup, args):
     nb = st.last_blocks()
     assert nb == wl.nblocks
     fused = st.fused_channels()
-    st.close()
     torch.cuda.set_stream(torch.cuda.default_stream(dev))
+    if keep is None:
+        st.close()
+    else:
+        keep.append((st, copies))
     prep.clear()
     del copies
 
@@ -391,7 +396,7 @@ def main():
     wl = W.WORKLOADS[args.workload]()
     x = make_input(wl, dev)
     head_prep = prepare_stream(wl, x, args, ncopies)
-    configs, config_inputs = {}, {}
+    configs, config_inputs, kept = {}, {}, []
     for name in names:
         wl_c = W.WORKLOADS[name]()
         t0 = time.perf_counter()
@@ -402,12 +407,19 @@ def main():
             if args.cpu_seconds > 0:
                 # the first 64 blocks: the CPU leg repeats them as needed
                 config_inputs[name] = xc[:64 * wl_c.hop + wl_c.fft_size].cpu().numpy()
-            configs[name] = measure_stream(prepare_stream(wl_c, xc, args, ncopies), args.config_steps, 10, args)
+            last = name == names[-1]
+            configs[name] = measure_stream(prepare_stream(wl_c, xc, args, ncopies), args.config_steps, 10, args,
+                                           keep=kept if last else None)
             del xc
         configs[name]["config"] = wl_c.describe()
         configs[name]["wall_s"] = round(time.perf_counter() - t0, 1)
-        torch.cuda.empty_cache()
+        if name != names[-1]:
+            torch.cuda.empty_cache()
     head = measure_stream(head_prep, args.steps, args.warmup, args)
+    for st_k, _ in kept:
+        st_k.close()
+    kept.clear()
+    torch.cuda.empty_cache()
 
     out = {
         "metric": "input MS/s through FCFB+Welch, 2^16 FFT, 64 ch; % of HBM roofline",
