@@ -117,21 +117,23 @@ __device__ __forceinline__ void static_for(F&& f) {
 // ---------------------------------------------------------------------------
 // In-register DFT, natural order in and out.
 // ---------------------------------------------------------------------------
-template <int R, int SIGN>
+// With CONJ_HI the outputs R/2 .. R-1 come out conjugated: the last
+// butterflies form their imaginary parts as reversed differences, at no cost.
+template <int R, int SIGN, bool CONJ_HI = false>
 __device__ __forceinline__ void dft(cpx (&x)[R]) {
   static_assert((R & (R - 1)) == 0, "power of two radices only");
   if constexpr (R == 1) {
   } else if constexpr (R == 2) {
     cpx a = x[0], b = x[1];
     x[0] = cadd(a, b);
-    x[1] = csub(a, b);
+    x[1] = CONJ_HI ? cpx{a.x - b.x, b.y - a.y} : csub(a, b);
   } else if constexpr (R == 4) {
     cpx s02 = cadd(x[0], x[2]), d02 = csub(x[0], x[2]);
     cpx s13 = cadd(x[1], x[3]), d13 = mul_si<SIGN>(csub(x[1], x[3]));
     x[0] = cadd(s02, s13);
-    x[2] = csub(s02, s13);
+    x[2] = CONJ_HI ? cpx{s02.x - s13.x, s13.y - s02.y} : csub(s02, s13);
     x[1] = cadd(d02, d13);
-    x[3] = csub(d02, d13);
+    x[3] = CONJ_HI ? cpx{d02.x - d13.x, d13.y - d02.y} : csub(d02, d13);
   } else {
     // n = R1*n2 + n1, k = k2 + R2*k1
     constexpr int R1 = (R >= 16) ? 4 : 2;
@@ -155,7 +157,7 @@ __device__ __forceinline__ void dft(cpx (&x)[R]) {
       cpx b[R1];
 #pragma unroll
       for (int n1 = 0; n1 < R1; ++n1) b[n1] = a[n1][k2];
-      dft<R1, SIGN>(b);
+      dft<R1, SIGN, CONJ_HI>(b);  // outputs k2 + R2 k1, k1 >= R1/2 <=> index >= R/2
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) x[k2 + R2 * k1] = b[k1];
     }

@@ -359,6 +359,14 @@ __device__ __forceinline__ T& at32(T* base, unsigned idx) {
   return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * (unsigned)sizeof(T));
 }
 
+// 4 |c0 - (cm + cp) / 2|^2, the Welch term at the scale of the forward
+// kernel's partial rows.
+__device__ __forceinline__ float welch_term4(cpx c0, cpx cm, cpx cp) {
+  const float cr = __builtin_fmaf(2.f, c0.x, -(cm.x + cp.x));
+  const float ci = __builtin_fmaf(2.f, c0.y, -(cm.y + cp.y));
+  return __builtin_fmaf(ci, ci, cr * cr);
+}
+
 // Symmetric in the two neighbours.
 __device__ __forceinline__ float welch_term(cpx c0, cpx cm, cpx cp) {
   const float cr = c0.x - 0.5f * (cm.x + cp.x);
@@ -439,7 +447,8 @@ __device__ __forceinline__ void welch_emit(const FinParams& F, int blk0_mod, int
     else *reinterpret_cast<float4*>(F.carry_out + i0) = s;
     return;
   }
-  const float sv[4] = {s.x, s.y, s.z, s.w};
+  // (partial rows and carries hold 4x the sums: scaled back exactly)
+  const float sv[4] = {0.25f * s.x, 0.25f * s.y, 0.25f * s.z, 0.25f * s.w};
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     if (nyq && j > 0) break;
@@ -541,7 +550,10 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   const int lo = (int)((long long)wg * P.nblocks / G);
   const int hi = (int)((long long)(wg + 1) * P.nblocks / G);
   const long long block0 = *P.pos;
-  const long long grp0 = (block0 + lo) / P.A;
+  // Welch group of each block, carried from block to block in scalars (no
+  // 64-bit division per block): ph = stream block index mod A, wslot = its
+  // group - the group of this workgroup's first block.
+  int ph = (int)((block0 + lo) % P.A), wslot = 0;
   const float w = P.wscale;
   if (wg == 0) {
     if ((int)threadIdx.x < P.ncounters) P.chan_counters[threadIdx.x] = 0;
@@ -577,12 +589,16 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   // 1024-thread workgroups have no registers to spare for the prefetch
   // (the prefetched words would be spilled, which waits for the loads).
   constexpr bool PREFETCH = T < 1024;
+  // With the L = 256 fused stage the 1024-thread workgroups load the next
+  // block's input after the edge terms, while the channel tiles run: the
+  // spectrum registers are dead by then.
+  constexpr bool PF_FUSED = !PREFETCH && FL > kFusedGen2;
   Word vin[R];
-  if (PREFETCH && lo < hi) load_block(lo, vin);
+  if ((PREFETCH || PF_FUSED) && lo < hi) load_block(lo, vin);
 
 
   for (int g = lo; g < hi; ++g) {
-    if constexpr (!PREFETCH) load_block(g, vin);
+    if constexpr (!PREFETCH && !PF_FUSED) load_block(g, vin);
     // Per-block values that do not depend on g (lane maps, table reads) are
     // recomputed and reloaded every block: kept live across the loop they
     // would cost more registers than the 1024-thread workgroups have.
@@ -613,8 +629,9 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
         x[e].x = vin[e].x * w;
         x[e].y = vin[e].y * w;
       } else {
-        x[e].x = (float)(int16_t)vin[e] * w;
-        x[e].y = (float)(vin[e] >> 16) * w;
+        // the window scale w is applied in the untangle constants
+        x[e].x = (float)(int16_t)vin[e];
+        x[e].y = (float)(vin[e] >> 16);
       }
     }
     st_butterflies<M, R, T, R1, 1, -1>(x, t, nullptr);
@@ -622,7 +639,9 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     lds_exchange<R, SPLIT>(
         x, lds, [&](int e) { return st_out_addr<M, R, T, R1, 1>(t, e); },
         [&](int e) { return st_in_addr<M, R, T, R2>(t, e); });
-    st_butterflies<M, R, T, R2, R1, -1>(x, t, tw2);
+    // Outputs q >= R2/2 of pass 2 are the pass-3 inputs of the upper lanes
+    // (partial-transform index k + R1 q >= P3/2), which take them conjugated.
+    st_butterflies<M, R, T, R2, R1, -1, true>(x, t, tw2);
 
     // Pass 3: an upper lane must end with Z[ksrc + P3 (R-1-q)] in slot q,
     // i.e. the DFT of its twiddled inputs in reversed order.  That is
@@ -637,9 +656,8 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     // indices, loaded ahead of the pass-3 DFT.  The row was written by this
     // workgroup's previous block: it is read at agent scope, which bypasses
     // the CU's L1 and so sees those stores.
-    const long long gs = block0 + g;
-    const int slot = (int)(gs / P.A - grp0);
-    const bool fresh = (g == lo) || (gs % P.A == 0);
+    const int slot = wslot;
+    const bool fresh = (g == lo) || (ph == 0);
     const __amdgpu_buffer_rsrc_t prow = __builtin_amdgcn_make_buffer_rsrc(
         P.wpart + ((long long)wg * P.wslots + slot) * P.wstride, (short)0, (M + 1) * (int)sizeof(float),
         kBufferRsrcWord3);
@@ -668,8 +686,6 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     };
     load_group(std::integral_constant<int, 0>{});
 
-#pragma unroll
-    for (int e = 0; e < R; ++e) x[e].y *= sg;
     apply_pass_twiddles<R, P3>(x, tw3, kt);
     dft<R, -1>(x);
 #pragma unroll
@@ -704,8 +720,10 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     //   X.y = s ((A.y-B.y)/2 + u.y (A.y+B.y) - u.x (A.x-B.x)).
     // W^k = W^kt (table) * W^(P3 q) (constant per slot).
     const cpx wb = twu[kt];
-    const cpx wbs = {0.5f * sg * wb.x, 0.5f * sg * wb.y};
-    const cpx wh = {0.5f * wb.x, 0.5f * wb.y};
+    // (times the window scale, for the s16 loader, whose samples enter unscaled)
+    const float hw = F32 ? 0.5f : 0.5f * w;
+    const cpx wbs = {hw * sg * wb.x, hw * sg * wb.y};
+    const cpx wh = {hw * wb.x, hw * wb.y};
     const bool lane0 = lane == 0, lane32 = lane == 32;
     // byte address of the mirror lane for ds_bpermute
     const int mirror_addr = (lane ^ 32) << 2;
@@ -748,14 +766,14 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
         ub = wh.x * cy + wh.y * cx;
       }
       cpx X;
-      X.x = 0.5f * S1 + (ua * S2 + ub * D1);
-      X.y = 0.5f * D2 + (ub * S2 - ua * D1);
+      X.x = hw * S1 + (ua * S2 + ub * D1);
+      X.y = hw * D2 + (ub * S2 - ua * D1);
       if (P.keep_spec) at32(spec, (unsigned)(kb + q * kstep)) = X;
       if (es >= 0)
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, X), edr, evoff + q * (int)sizeof(cpx), 0, 0);
       if constexpr (q == 0) {
         if (t == 0) {  // Nyquist bin M = E - O of Z[0]
-          const cpx XN = {0.5f * (S1 - S2), 0.5f * (D2 + D1)};
+          const cpx XN = {hw * (S1 - S2), hw * (D2 + D1)};
           if (P.keep_spec) spec[M] = XN;
           ering[kEdgeNyq * R] = XN;
           if constexpr (FL > kFusedGen2) reinterpret_cast<cpx*>(lds)[P.fscat_nyq] = XN;
@@ -767,13 +785,14 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
       }
       // Welch neighbours by DPP: lanes l-1 and l+1 (left/right for lower
       // lanes, right/left for upper lanes; the term is symmetric).
-      // The term |X - (l + r)/2|^2 is formed as |l + r - 2 X|^2 / 4, which
-      // rounds identically (power-of-two scalings).
+      // The partial rows hold 4 |X - (l + r)/2|^2 = |l + r - 2 X|^2 per bin
+      // (a power-of-two scaling, removed exactly when records are formed).
       float dx = wave_shr1_any(X.x) + wave_shl1_any(X.x);
       float dy = wave_shr1_any(X.y) + wave_shl1_any(X.y);
       dx = __builtin_fmaf(-2.f, X.x, dx);
       dy = __builtin_fmaf(-2.f, X.y, dy);
-      const float acc = __builtin_fmaf(0.25f, __builtin_fmaf(dy, dy, dx * dx), pv[q]);
+      // (partial rows hold 4x the sums: one product fewer per bin)
+      const float acc = __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, pv[q]));
       __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(acc), prow, tvoff, q * T * (int)sizeof(float), 0);
       if constexpr (fused_general(FL)) {
         // general scatter: every target of this bin (a dummy slot when none)
@@ -807,7 +826,7 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     // partial-row entry when the Welch group ends.  The item table is read
     // before the barrier, the ring after it, and the terms are formed after
     // the fused channel stage so that the loads overlap it.
-    const bool slot_end = (g + 1 == hi) || ((gs + 1) % P.A == 0);
+    const bool slot_end = (g + 1 == hi) || (ph + 1 == P.A);
     int4 eit[NI];
     u32x2 ex[NI], el[NI], er[NI];
     // Threads past the last item repeat the last item's loads and keep
@@ -833,7 +852,7 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
         cpx L = __builtin_bit_cast(cpx, el[i]), Rn = __builtin_bit_cast(cpx, er[i]);
         if (fl & 1) L.y = -L.y;
         if (fl & 2) Rn.y = -Rn.y;
-        const float term = welch_term(X, L, Rn);
+        const float term = welch_term4(X, L, Rn);
         const float acc = fresh ? term : eacc[t + i * T] + term;
         eacc[t + i * T] = acc;
         if (slot_end && item_ok(i))
@@ -889,6 +908,22 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
       }
       __syncthreads();  // tile buffers and edge ring complete
       edge_issue();
+      int zero2 = 0;
+      asm volatile("" : "+s"(zero2));
+      const cpx* ftw2 = P.ftw2 + zero2;
+      static_assert(RF == FC::RF2, "one second-pass butterfly per thread");
+      // (loaded before the next block's input: the loads are waited for in order)
+      const PassTwiddles<FC::RF2> ftw = load_pass_twiddles<FC::RF2, FC::RF1>(ftw2, ftt & (FC::RF1 - 1));
+      if constexpr (PF_FUSED) {
+        edge_finish();
+        __builtin_amdgcn_sched_barrier(0);
+        // Branch-free (the last block loads its own input once more): with
+        // a skipped load on one path, the waits for the loads above would
+        // count on the shorter path and wait for this whole prefetch, and
+        // the registers would stay live across the loop.
+        load_block(g + 1 < hi ? g + 1 : g, vin);
+        __builtin_amdgcn_sched_barrier(0);
+      }
       float* tl = lds + fc * FC::STRIDE * 2;
       cpx gx[RF];
 #pragma unroll
@@ -900,10 +935,8 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
       tile_exchange<TF>(
           gx, tl, [&](int e) { return st_out_addr<FL, RF, TF, FC::RF1, 1>(ftt, e); },
           [&](int e) { return st_in_addr<FL, RF, TF, FC::RF2>(ftt, e); });
-      int zero2 = 0;
-      asm volatile("" : "+s"(zero2));
-      const cpx* ftw2 = P.ftw2 + zero2;
-      st_butterflies<FL, RF, TF, FC::RF2, FC::RF1, +1>(gx, ftt, ftw2);
+      apply_loaded_twiddles<FC::RF2>(gx, ftw);
+      dft<FC::RF2, +1>(gx);
       constexpr int PL = FL / FC::RF2;
       if (fact) {
         cpx* out = P.fout + fd.out_off + (long long)g * (3 * FL / 4);
@@ -914,7 +947,11 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
           if (idx >= FL / 8 && idx < (FL / 8) * 7) out[idx - FL / 8] = gx[e];
         }
       }
-      edge_finish();
+      if constexpr (!PF_FUSED) edge_finish();
+    }
+    if (++ph == P.A) {
+      ph = 0;
+      ++wslot;
     }
   }
 

@@ -75,8 +75,34 @@ __device__ __forceinline__ void apply_pass_twiddles(cpx (&v)[r], const cpx* __re
   }
 }
 
+// The table twiddles of one butterfly of pass (r, p) (r >= 8), loaded ahead
+// of their use: rows j = 1, 2, 3 and 4a as in apply_pass_twiddles.
+template <int r>
+struct PassTwiddles {
+  cpx lo[4], hi[r / 4];
+};
+template <int r, int p>
+__device__ __forceinline__ PassTwiddles<r> load_pass_twiddles(const cpx* __restrict__ tw, int k) {
+  static_assert(r >= 8, "table rows 1, 2, 3 and 4a");
+  PassTwiddles<r> w;
+#pragma unroll
+  for (int b = 1; b < 4; ++b) w.lo[b] = tw[(b - 1) * p + k];
+#pragma unroll
+  for (int a = 1; a < r / 4; ++a) w.hi[a] = tw[(4 * a - 1) * p + k];
+  return w;
+}
+template <int r>
+__device__ __forceinline__ void apply_loaded_twiddles(cpx (&v)[r], const PassTwiddles<r>& w) {
+#pragma unroll
+  for (int j = 1; j < r; ++j) {
+    const int a = j >> 2, b = j & 3;
+    const cpx t = (b == 0) ? w.hi[a] : (a == 0 ? w.lo[b] : cmul(w.lo[b], w.hi[a]));
+    v[j] = cmul(v[j], t);
+  }
+}
+
 // Twiddle then DFT for every butterfly a thread owns in pass (r, p).
-template <int M, int R, int T, int r, int p, int SIGN>
+template <int M, int R, int T, int r, int p, int SIGN, bool CONJ_HI = false>
 __device__ __forceinline__ void st_butterflies(cpx (&x)[R], int t, const cpx* __restrict__ tw) {
 #pragma unroll
   for (int u = 0; u < R / r; ++u) {
@@ -86,7 +112,7 @@ __device__ __forceinline__ void st_butterflies(cpx (&x)[R], int t, const cpx* __
 #pragma unroll
     for (int j = 0; j < r; ++j) v[j] = x[u * r + j];
     if constexpr (p > 1) apply_pass_twiddles<r, p>(v, tw, k);
-    dft<r, SIGN>(v);
+    dft<r, SIGN, CONJ_HI>(v);
 #pragma unroll
     for (int q = 0; q < r; ++q) x[u * r + q] = v[q];
   }
