@@ -636,12 +636,24 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     }
     st_butterflies<M, R, T, R1, 1, -1>(x, t, nullptr);
 
+    // The pass-2 and pass-3 twiddles are loaded inside the exchanges before
+    // them, while half of the data waits in LDS (for pass 2 when each
+    // thread has one butterfly, R2 = R).
+    PassTwiddles<R> ptw;
     lds_exchange<R, SPLIT>(
         x, lds, [&](int e) { return st_out_addr<M, R, T, R1, 1>(t, e); },
-        [&](int e) { return st_in_addr<M, R, T, R2>(t, e); });
+        [&](int e) { return st_in_addr<M, R, T, R2>(t, e); },
+        [&]() {
+          if constexpr (R2 == R) ptw = load_pass_twiddles<R, R1>(tw2, t & (R1 - 1));
+        });
     // Outputs q >= R2/2 of pass 2 are the pass-3 inputs of the upper lanes
     // (partial-transform index k + R1 q >= P3/2), which take them conjugated.
-    st_butterflies<M, R, T, R2, R1, -1, true>(x, t, tw2);
+    if constexpr (R2 == R) {
+      apply_loaded_twiddles<R>(x, ptw);
+      dft<R, -1, true>(x);
+    } else {
+      st_butterflies<M, R, T, R2, R1, -1, true>(x, t, tw2);
+    }
 
     // Pass 3: an upper lane must end with Z[ksrc + P3 (R-1-q)] in slot q,
     // i.e. the DFT of its twiddled inputs in reversed order.  That is
@@ -650,7 +662,8 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     // (column kt), and conjugate the DFT outputs.
     lds_exchange<R, SPLIT>(
         x, lds, [&](int e) { return st_out_addr<M, R, T, R2, R1>(t, e); },
-        [&](int e) { return lds_pad(ksrc) + pad_off(e * P3); });
+        [&](int e) { return lds_pad(ksrc) + pad_off(e * P3); },
+        [&]() { ptw = load_pass_twiddles<R, P3>(tw3, kt); });
     // Welch partial row of this block's group (lane-major: slot q of lane t
     // at q T + t), and the first slot group's partial sums and scatter
     // indices, loaded ahead of the pass-3 DFT.  The row was written by this
@@ -686,7 +699,7 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
     };
     load_group(std::integral_constant<int, 0>{});
 
-    apply_pass_twiddles<R, P3>(x, tw3, kt);
+    apply_loaded_twiddles<R>(x, ptw);
     dft<R, -1>(x);
 #pragma unroll
     for (int e = 0; e < R; ++e) x[e].y *= sg;

@@ -123,12 +123,18 @@ __device__ __forceinline__ void st_butterflies(cpx (&x)[R], int t, const cpx* __
 // SPLIT moves the real and imaginary planes one after the other so that
 // transforms twice the LDS size fit.  Starts with a barrier so earlier LDS
 // readers are done.
-template <int R, bool SPLIT, typename WA, typename RA>
-__device__ __forceinline__ void lds_exchange(cpx (&x)[R], float* lds, WA waddr, RA raddr) {
+// (mid(): called once the real plane is in LDS and the real parts are dead
+// in registers: a place to issue loads for the next pass.)
+struct NoMid {
+  __device__ void operator()() const {}
+};
+template <int R, bool SPLIT, typename WA, typename RA, typename MID = NoMid>
+__device__ __forceinline__ void lds_exchange(cpx (&x)[R], float* lds, WA waddr, RA raddr, MID mid = MID{}) {
   if constexpr (SPLIT) {
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < R; ++e) lds[waddr(e)] = x[e].x;
+    mid();
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < R; ++e) x[e].x = lds[raddr(e)];
@@ -143,6 +149,7 @@ __device__ __forceinline__ void lds_exchange(cpx (&x)[R], float* lds, WA waddr, 
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < R; ++e) l[waddr(e)] = x[e];
+    mid();
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < R; ++e) x[e] = l[raddr(e)];
