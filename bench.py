@@ -240,6 +240,32 @@ def measure_stream(prep, steps, warmup, args, keep=None):
     per_step = sorted(evs[i].elapsed_time(evs[i + 1]) for i in range(ns))
     step_stats = {"median_ms": round(per_step[ns // 2], 4), "min_ms": round(per_step[0], 4),
                   "max_ms": round(per_step[-1], 4), "steps": ns}
+    # Both channel-stage forms after tuning (tile shapes and work split
+    # chosen by the autotuner): per-step times, median and extremes, each
+    # form warmed up (graphs captured) first; the tuned form is restored.
+    forms = None
+    if tune is not None:
+        forms = {}
+        for batched in (False, True):
+            st.set_channel_mode(batched)
+            for i in range(5):
+                st.process_device(ptrs[i % ncopies], n)
+            fe = [torch.cuda.Event(enable_timing=True) for _ in range(ns + 1)]
+            fe[0].record(stream)
+            for i in range(ns):
+                st.process_device(ptrs[i % ncopies], n)
+                fe[i + 1].record(stream)
+            st.flush()
+            torch.cuda.synchronize()
+            per = sorted(fe[i].elapsed_time(fe[i + 1]) for i in range(ns))
+            forms["batched" if batched else "per_size"] = {
+                "median_ms": round(per[ns // 2], 4), "min_ms": round(per[0], 4), "max_ms": round(per[-1], 4),
+                "steps": ns}
+        st.set_channel_mode(tune["mode"] == "batched")
+        for i in range(5):
+            st.process_device(ptrs[i % ncopies], n)
+        st.flush()
+        torch.cuda.synchronize()
     # Per-kernel breakdown: the same steps launched eagerly with HIP events
     # around each launch (events are not recorded inside graphs).
     st.enable_timing(True)
@@ -299,6 +325,7 @@ def measure_stream(prep, steps, warmup, args, keep=None):
            "graphs": (not args.no_graphs) and (wl.complex or fused < len(wl.channels))}
     if tune is not None:
         out["channel_stage_tune"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tune.items()}
+        out["channel_stage_forms"] = forms
     return out
 
 

@@ -240,15 +240,22 @@ spk_status spk_last_timing(spk_ctx* ctx, float ms[3]);
  * one on rayon, fcfb.rs:66-100):
  *   SPK_CHAN_PER_SIZE  one launch per distinct IFFT size (the default);
  *   SPK_CHAN_BATCHED   one persistent launch for every channel of every
- *                      size, with XCD-local block chunks.  */
+ *                      size, with XCD-local block chunks; IFFT sizes
+ *                      without a power-of-two plan join it up to L = 2048
+ *                      (above that they keep their own launch).  */
 #define SPK_CHAN_PER_SIZE 0
 #define SPK_CHAN_BATCHED 1
 spk_status spk_set_channel_mode(spk_ctx* ctx, int mode);
 
-/* Time both launch forms on the context's current spectra for nblocks
- * blocks (0 = max_blocks), search the batched form's block chunk, rounds
- * per work item and workgroups per CU, keep the fastest settings, and
- * select the faster form.  Synchronous; overwrites channel outputs. */
+/* On the context's current spectra for nblocks blocks (0 = max_blocks):
+ * time every tile shape of every power-of-two IFFT size in its per-size
+ * launch and keep the fastest for that form; time the per-size form; for
+ * the batched form, with each size's fastest shape and with its fastest
+ * shape of at most 16 points per thread (half the LDS per workgroup), search
+ * the block chunk, rounds per work item and workgroups per CU; keep the
+ * fastest settings and select the batched form if it is at least 2 %
+ * faster.  Synchronous; overwrites channel outputs.  No counterpart in the
+ * reference. */
 typedef struct spk_tune_result {
   float per_size_ms;
   float batched_ms;
@@ -258,6 +265,21 @@ typedef struct spk_tune_result {
   int mode;            /* SPK_CHAN_* selected */
 } spk_tune_result;
 spk_status spk_autotune(spk_ctx* ctx, uint32_t nblocks, spk_tune_result* out);
+
+/* Tile shape of the channel transform of power-of-two IFFT size `bins` in
+ * one launch form (SPK_CHAN_*): shape s is a radix plan with
+ * points_per_thread points per thread, so a (channel, block) tile takes
+ * bins / points_per_thread threads.  Sizes 16 to 4096 have two or three
+ * shapes.  spk_set_tile_shape fixes one (SPK_ERR_INVALID for a shape the
+ * size does not have); the next call lays the channel tables out again.
+ * In the batched form a shape of more than 16 points per thread halves the
+ * workgroups per CU for every size (twice the LDS each).
+ * spk_tile_shape reports the shape in use (the form's default until
+ * spk_autotune or spk_set_tile_shape chose one).  No counterpart in the
+ * reference: launch geometry for the autotuner and its tests. */
+spk_status spk_set_tile_shape(spk_ctx* ctx, uint32_t bins, int form, uint32_t shape);
+spk_status spk_tile_shape(spk_ctx* ctx, uint32_t bins, int form, uint32_t* shape, uint32_t* threads_per_tile,
+                          uint32_t* points_per_thread);
 
 /* Number of channels whose gather, weighting, IFFT and discard run inside
  * the forward kernel (real input up to N = 65536): every channel when all

@@ -97,6 +97,9 @@ _SIGS = {
     "spk_set_weights_device": (ctypes.c_int, [_P, _P]),
     "spk_set_channel_mode": (ctypes.c_int, [_P, ctypes.c_int]),
     "spk_autotune": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.POINTER(SpkTuneResult)]),
+    "spk_set_tile_shape": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32]),
+    "spk_tile_shape": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
+                                      ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
     "spk_channels_output_packed": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "spk_channels_adjoint": (ctypes.c_int, [_P, _P, _P, _P]),
     "spk_channels_adjoint_noalloc": (ctypes.c_int, [_P, _P, _P, _P]),

@@ -17,12 +17,9 @@
 // made in double on the host).  A stage costs p complex multiply-adds per
 // output; the sum over stages is the sum of L's prime factors.
 #pragma once
-#include "fft_device.h"
+#include "kernels.h"
 
 namespace spk {
-
-constexpr int kGenMaxStages = 16;
-constexpr int kGenMaxL = 4096;  // two LDS buffers of L cpx per tile stay within 64 KiB
 
 struct GenChanParams {
   const cpx* spec;
@@ -39,8 +36,6 @@ struct GenChanParams {
   int tile_threads;    // threads per (channel, block) tile; divides the workgroup
 };
 
-constexpr int kGenThreads = 256;
-
 // Dynamic LDS: tiles_per_workgroup * 2 * L cpx.
 __global__ __launch_bounds__(kGenThreads) void chan_generic_kernel(GenChanParams P) {
   extern __shared__ __align__(16) float gen_lds[];
@@ -49,7 +44,6 @@ __global__ __launch_bounds__(kGenThreads) void chan_generic_kernel(GenChanParams
   const int tl = threadIdx.x / T, t = threadIdx.x % T;
   const int tpw = kGenThreads / T;
   cpx* a = reinterpret_cast<cpx*>(gen_lds) + (long long)tl * 2 * L;
-  cpx* b = a + L;
 
   const long long ntiles = (long long)P.nch * P.nblocks;
   long long tile = (long long)blockIdx.x * tpw + tl;
@@ -58,57 +52,8 @@ __global__ __launch_bounds__(kGenThreads) void chan_generic_kernel(GenChanParams
   const int c = (int)(tile % P.nch);
   const int g = (int)(tile / P.nch);
   const ChanDesc d = P.chans[c];
-  const cpx* X = P.spec + (long long)g * P.spec_stride;
-  const float* W = P.weights + d.w_off;
-
-  // Gather x weights in natural input order i (get_bin, fftutil.rs:8-22).
-  for (int i = t; i < L; i += T) {
-    int jj = i + L / 2;
-    if (jj >= L) jj -= L;
-    int m = d.f_mod + jj;
-    if (m >= P.N) m -= P.N;
-    const bool mirror = P.real_input && m >= P.K;
-    cpx v = X[mirror ? P.N - m : m];
-    if (mirror) v.y = -v.y;
-    a[i] = cscale(v, W[jj]);
-  }
-
-  int m = 1;
-  for (int s = 0; s < P.nstages; ++s) {
-    __syncthreads();
-    const int p = P.radix[s];
-    const int Lp = L / p;
-    const int mp = m * p;
-    const int scale = L / mp;  // w_mp = w_L^scale
-    for (int o = t; o < L; o += T) {
-      const int k = o % m;
-      const int r = o / m;
-      const int q = r % p;
-      const int base = (r / p) * m + k;
-      const int step = (k + q * m) * scale;  // < L
-      cpx acc = a[base];
-      int e = step;
-      for (int j = 1; j < p; ++j) {
-        const cpx v = a[base + j * Lp];
-        const cpx w = P.tw[e];
-        acc.x += v.x * w.x - v.y * w.y;
-        acc.y += v.x * w.y + v.y * w.x;
-        e += step;
-        if (e >= L) e -= L;
-      }
-      b[o] = acc;
-    }
-    cpx* tmp = a;
-    a = b;
-    b = tmp;
-    m = mp;
-  }
-  __syncthreads();
-  if (active) {
-    const int lo = L / 8, n = 6 * (L / 8);
-    cpx* out = P.out + d.out_off + (long long)g * n;
-    for (int i = t; i < n; i += T) out[i] = a[lo + i];
-  }
+  gen_chan_tile(P.spec, P.spec_stride, P.N, P.K, P.real_input, P.weights, d, g, active, a, a + L, t, T, L,
+                P.nstages, P.radix, P.tw, P.out);
 }
 
 // Wide channels (kGenMaxL < L <= kGenBigMaxL, any multiple of 4, and the

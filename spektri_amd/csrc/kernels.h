@@ -1144,6 +1144,144 @@ __global__ __launch_bounds__(kChanThreads) void chan_kernel(ChanParams P) {
 }
 
 // ---------------------------------------------------------------------------
+// One tile of the mixed-radix channel transform for IFFT sizes without a
+// power-of-two plan (fcfb.rs:154-188 with L any multiple of 4): gather x
+// weights into a (natural input order i, get_bin of fftutil.rs:8-22), one
+// Stockham stage per prime factor of L ping-ponging between a and b, and the
+// outputs y[L/8 .. (L/8)*7) stored when active.  Stage (p, m) computes every
+// output o = m (c p + q) + k (k < m, q < p) directly as
+//   out[o] = sum_j in[c m + k + j L/p] * w_L^(j (k + q m) L/(m p)),
+// merging the inter-stage twiddle and the p-point DFT kernel into one table
+// entry per term (tw = w_L^e, e < L, inverse sign).  T threads (t < T) share
+// the tile; the stages hold workgroup barriers, so every thread of the
+// workgroup makes the same calls.
+// ---------------------------------------------------------------------------
+constexpr int kGenMaxStages = 16;
+constexpr int kGenMaxL = 4096;  // two LDS buffers of L cpx per tile stay within 64 KiB
+constexpr int kGenThreads = 256;
+
+__device__ __forceinline__ void gen_chan_tile(const cpx* __restrict__ spec, long long spec_stride, int N, int K,
+                                              int real_input, const float* __restrict__ weights, const ChanDesc& d,
+                                              int g, bool active, cpx* a, cpx* b, int t, int T, int L, int nstages,
+                                              const int* radix, const cpx* __restrict__ tw,
+                                              cpx* __restrict__ out_base) {
+  const cpx* X = spec + (long long)g * spec_stride;
+  const float* W = weights + d.w_off;
+  for (int i = t; i < L; i += T) {
+    int jj = i + L / 2;
+    if (jj >= L) jj -= L;
+    int m = d.f_mod + jj;
+    if (m >= N) m -= N;
+    const bool mirror = real_input && m >= K;
+    cpx v = X[mirror ? N - m : m];
+    if (mirror) v.y = -v.y;
+    a[i] = cscale(v, W[jj]);
+  }
+  int m = 1;
+  for (int s = 0; s < nstages; ++s) {
+    __syncthreads();
+    const int p = radix[s];
+    const int Lp = L / p;
+    const int mp = m * p;
+    const int scale = L / mp;  // w_mp = w_L^scale
+    for (int o = t; o < L; o += T) {
+      const int k = o % m;
+      const int r = o / m;
+      const int q = r % p;
+      const int base = (r / p) * m + k;
+      const int step = (k + q * m) * scale;  // < L
+      cpx acc = a[base];
+      int e = step;
+      for (int j = 1; j < p; ++j) {
+        const cpx v = a[base + j * Lp];
+        const cpx w = tw[e];
+        acc.x += v.x * w.x - v.y * w.y;
+        acc.y += v.x * w.y + v.y * w.x;
+        e += step;
+        if (e >= L) e -= L;
+      }
+      b[o] = acc;
+    }
+    cpx* tmp = a;
+    a = b;
+    b = tmp;
+    m = mp;
+  }
+  __syncthreads();
+  if (active) {
+    const int lo = L / 8, n = 6 * (L / 8);
+    cpx* out = out_base + d.out_off + (long long)g * n;
+    for (int i = t; i < n; i += T) out[i] = a[lo + i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tile shapes of the power-of-two channel transform: shape S of size L is
+// the radix plan (R points per thread, passes R1 x R2 x R3, R1 = R), so a
+// tile takes L / R threads.  Shape 0 is the default of the batched kernel;
+// kPerSizeDefaultShape<L> that of the per-size launch.  A workgroup of the
+// batched kernel holds 256 R complex values in LDS: chan_batched_kernel<4096>
+// serves the shapes with R <= 16 (four workgroups per CU),
+// chan_batched_kernel<8192> those with R <= 32 (two per CU).  The autotuner
+// picks one shape per size and form.
+// ---------------------------------------------------------------------------
+template <int L, int S>
+struct TileShape {
+  static constexpr bool valid = false;
+};
+#define SPK_SHAPE(L_, S_, R_, R1_, R2_, R3_)                                              \
+  template <>                                                                            \
+  struct TileShape<L_, S_> {                                                             \
+    static constexpr bool valid = true;                                                  \
+    static constexpr int R = R_, R1 = R1_, R2 = R2_, R3 = R3_;                           \
+    static_assert(R1 == R && R1 * R2 * R3 == L_ && R % R2 == 0 && R % R3 == 0, "plan"); \
+  };
+SPK_SHAPE(8, 0, 4, 4, 2, 1)
+SPK_SHAPE(16, 0, 4, 4, 4, 1)
+SPK_SHAPE(16, 1, 8, 8, 2, 1)
+SPK_SHAPE(32, 0, 8, 8, 4, 1)
+SPK_SHAPE(32, 1, 4, 4, 4, 2)
+SPK_SHAPE(32, 2, 16, 16, 2, 1)
+SPK_SHAPE(64, 0, 8, 8, 8, 1)
+SPK_SHAPE(64, 1, 16, 16, 4, 1)
+SPK_SHAPE(64, 2, 4, 4, 4, 4)
+SPK_SHAPE(128, 0, 16, 16, 8, 1)
+SPK_SHAPE(128, 1, 8, 8, 8, 2)
+SPK_SHAPE(128, 2, 32, 32, 4, 1)
+SPK_SHAPE(256, 0, 16, 16, 16, 1)
+SPK_SHAPE(256, 1, 8, 8, 8, 4)
+SPK_SHAPE(256, 2, 32, 32, 8, 1)
+SPK_SHAPE(512, 0, 16, 16, 16, 2)
+SPK_SHAPE(512, 1, 32, 32, 16, 1)
+SPK_SHAPE(512, 2, 8, 8, 8, 8)
+SPK_SHAPE(1024, 0, 16, 16, 16, 4)
+SPK_SHAPE(1024, 1, 32, 32, 32, 1)
+SPK_SHAPE(2048, 0, 16, 16, 16, 8)
+SPK_SHAPE(2048, 1, 32, 32, 32, 2)
+SPK_SHAPE(4096, 0, 16, 16, 16, 16)
+SPK_SHAPE(4096, 1, 32, 32, 32, 4)
+SPK_SHAPE(8192, 0, 32, 32, 16, 16)
+#undef SPK_SHAPE
+constexpr int kMaxShapes = 3;
+template <int L>
+constexpr int shape_count() {
+  return TileShape<L, 2>::valid ? 3 : (TileShape<L, 1>::valid ? 2 : 1);
+}
+// The per-size launch's default: the shape with 32 points per thread from
+// L = 512 to 2048, shape 0 otherwise.
+template <int L>
+constexpr int kPerSizeDefaultShape = (L >= 512 && L <= 2048) ? 1 : 0;
+constexpr int batch_max_points(int MAXL) { return MAXL > 4096 ? 32 : 16; }
+template <int L, int S, int MAXL>
+constexpr bool batch_shape_ok() {
+  if constexpr (TileShape<L, S>::valid && L <= MAXL) {
+    return TileShape<L, S>::R <= batch_max_points(MAXL);
+  } else {
+    return false;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Batched persistent channel stage: every channel of every size in one
 // launch.  The host cuts the (block, channel) tiles into items: an item
 // holds tiles of one IFFT size class from one chunk of consecutive blocks,
@@ -1154,7 +1292,7 @@ __global__ __launch_bounds__(kChanThreads) void chan_kernel(ChanParams P) {
 // one L2.  Items are taken in order through one atomic counter per queue,
 // which welch_finalize_kernel resets for the next launch.
 // ---------------------------------------------------------------------------
-constexpr int kMaxClasses = 11;
+constexpr int kMaxClasses = 16;
 struct BatchItem {
   int cls, ch0, nch, g0, t0, t1;
 };
@@ -1170,36 +1308,46 @@ struct BatchParams {
   int* counters;        // [X]
   int X;
   int cls_L[kMaxClasses];
+  // power-of-two classes: tile shape and pass tables; mixed-radix classes
+  // (cls_shape < 0): threads per tile, stage radices, and w_L in tw2
+  int cls_shape[kMaxClasses];
   const cpx* tw2[kMaxClasses];
   const cpx* tw3[kMaxClasses];
+  int gen_T[kMaxClasses];
+  int gen_nst[kMaxClasses];
+  int gen_radix[kMaxClasses][kGenMaxStages];
 };
 
-// Radix plans of the batched kernel: at most 16 points per thread up to
-// L = 4096 (4096 points per workgroup in LDS), 32 for L = 8192.
-template <int L>
-struct BatchPlan;
-template <> struct BatchPlan<8> { static constexpr int R = 4, R1 = 4, R2 = 2, R3 = 1; };
-template <> struct BatchPlan<16> { static constexpr int R = 4, R1 = 4, R2 = 4, R3 = 1; };
-template <> struct BatchPlan<32> { static constexpr int R = 8, R1 = 8, R2 = 4, R3 = 1; };
-template <> struct BatchPlan<64> { static constexpr int R = 8, R1 = 8, R2 = 8, R3 = 1; };
-template <> struct BatchPlan<128> { static constexpr int R = 16, R1 = 16, R2 = 8, R3 = 1; };
-template <> struct BatchPlan<256> { static constexpr int R = 16, R1 = 16, R2 = 16, R3 = 1; };
-template <> struct BatchPlan<512> { static constexpr int R = 16, R1 = 16, R2 = 16, R3 = 2; };
-template <> struct BatchPlan<1024> { static constexpr int R = 16, R1 = 16, R2 = 16, R3 = 4; };
-template <> struct BatchPlan<2048> { static constexpr int R = 16, R1 = 16, R2 = 16, R3 = 8; };
-template <> struct BatchPlan<4096> { static constexpr int R = 16, R1 = 16, R2 = 16, R3 = 16; };
-template <> struct BatchPlan<8192> { static constexpr int R = 32, R1 = 32, R2 = 16, R3 = 16; };
-
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
-template <int L>
+// LDS of the batched kernel (floats): 256 threads x R points of the largest
+// shape it serves, padded per tile.
+template <int L, int S, int MAXL>
 constexpr int batch_lds_floats() {
-  constexpr int T = L / BatchPlan<L>::R;
-  return (kChanThreads / T) * lds_padded_size(L) * 2;
+  if constexpr (batch_shape_ok<L, S, MAXL>()) {
+    constexpr int T = L / TileShape<L, S>::R;
+    return (kChanThreads / T) * lds_padded_size(L) * 2;
+  } else {
+    return 0;
+  }
+}
+template <int L, int MAXL>
+constexpr int batch_lds_floats_all() {
+  return cmax(batch_lds_floats<L, 0, MAXL>(), cmax(batch_lds_floats<L, 1, MAXL>(), batch_lds_floats<L, 2, MAXL>()));
+}
+// LDS floats of chan_batched_kernel<MAXL> (the host sizes mixed-radix tiles by it).
+template <int MAXL>
+constexpr int batch_kernel_lds_floats() {
+  return cmax(cmax(cmax(batch_lds_floats_all<8, MAXL>(), batch_lds_floats_all<16, MAXL>()),
+                   cmax(batch_lds_floats_all<32, MAXL>(), batch_lds_floats_all<64, MAXL>())),
+              cmax(cmax(cmax(batch_lds_floats_all<128, MAXL>(), batch_lds_floats_all<256, MAXL>()),
+                        cmax(batch_lds_floats_all<512, MAXL>(), batch_lds_floats_all<1024, MAXL>())),
+                   cmax(cmax(batch_lds_floats_all<2048, MAXL>(), batch_lds_floats_all<4096, MAXL>()),
+                        batch_lds_floats_all<8192, MAXL>())));
 }
 
-template <int L>
+template <int L, int S>
 __device__ __forceinline__ void batch_item(const BatchParams& P, const BatchItem& bi, float* lds_all) {
-  using BP = BatchPlan<L>;
+  using BP = TileShape<L, S>;
   constexpr int T = L / BP::R;
   constexpr int TPW = kChanThreads / T;
   constexpr int LP = lds_padded_size(L);
@@ -1225,18 +1373,46 @@ __device__ __forceinline__ void batch_item(const BatchParams& P, const BatchItem
   }
 }
 
+// A mixed-radix class: gen_T threads per tile, two buffers of L values per
+// tile (the host checks they fit the kernel's LDS).
+__device__ __forceinline__ void batch_gen_item(const BatchParams& P, const BatchItem& bi, float* lds_all) {
+  const int cls = bi.cls, L = P.cls_L[cls], T = P.gen_T[cls];
+  const int tpw = kChanThreads / T;
+  const int tl = (int)threadIdx.x / T, t = (int)threadIdx.x % T;
+  cpx* a = reinterpret_cast<cpx*>(lds_all) + (long long)tl * 2 * L;
+  const int rounds = (bi.t1 - bi.t0 + tpw - 1) / tpw;
+  for (int r = 0; r < rounds; ++r) {
+    int tile = bi.t0 + r * tpw + tl;
+    const bool active = tile < bi.t1;
+    if (!active) tile = bi.t0;
+    const ChanDesc d = P.descs[bi.ch0 + tile % bi.nch];
+    __syncthreads();  // the previous round's tiles are done with the buffers
+    gen_chan_tile(P.spec, P.spec_stride, P.N, P.K, P.real_input, P.weights, d, bi.g0 + tile / bi.nch, active, a,
+                  a + L, t, T, L, P.gen_nst[cls], P.gen_radix[cls], P.tw2[cls], P.out);
+  }
+}
+
+template <int L, int MAXL>
+__device__ __forceinline__ void batch_item_shape(const BatchParams& P, const BatchItem& bi, float* lds_all) {
+  switch (P.cls_shape[bi.cls]) {
+    case 1:
+      if constexpr (batch_shape_ok<L, 1, MAXL>()) batch_item<L, 1>(P, bi, lds_all);
+      break;
+    case 2:
+      if constexpr (batch_shape_ok<L, 2, MAXL>()) batch_item<L, 2>(P, bi, lds_all);
+      break;
+    default:
+      if constexpr (batch_shape_ok<L, 0, MAXL>()) batch_item<L, 0>(P, bi, lds_all);
+      break;
+  }
+}
+
 // MAXL: largest IFFT size the instantiation serves (4096 or 8192).
 // (A minimum-occupancy launch bound of four workgroups per CU made this
 // kernel 1.7x slower on gfx950; LDS already limits it to four.)
 template <int MAXL>
 __global__ __launch_bounds__(kChanThreads) void chan_batched_kernel(BatchParams P) {
-  constexpr int LDSF = cmax(
-      cmax(cmax(batch_lds_floats<8>(), batch_lds_floats<16>()), cmax(batch_lds_floats<32>(), batch_lds_floats<64>())),
-      cmax(cmax(cmax(batch_lds_floats<128>(), batch_lds_floats<256>()),
-                cmax(batch_lds_floats<512>(), batch_lds_floats<1024>())),
-           cmax(cmax(batch_lds_floats<2048>(), batch_lds_floats<4096>()),
-                MAXL > 4096 ? batch_lds_floats<8192>() : 0)));
-  __shared__ float lds_all[LDSF];
+  __shared__ float lds_all[batch_kernel_lds_floats<MAXL>()];
   __shared__ int s_item;
   const int x = (int)blockIdx.x % P.X;
   const int base = P.item_off[x], n = P.item_off[x + 1] - base;
@@ -1247,20 +1423,22 @@ __global__ __launch_bounds__(kChanThreads) void chan_batched_kernel(BatchParams 
     const int it = s_item;
     if (it >= n) break;
     const BatchItem bi = P.items[base + it];
+    if (P.cls_shape[bi.cls] < 0) {
+      batch_gen_item(P, bi, lds_all);
+      continue;
+    }
     switch (P.cls_L[bi.cls]) {
-      case 8: batch_item<8>(P, bi, lds_all); break;
-      case 16: batch_item<16>(P, bi, lds_all); break;
-      case 32: batch_item<32>(P, bi, lds_all); break;
-      case 64: batch_item<64>(P, bi, lds_all); break;
-      case 128: batch_item<128>(P, bi, lds_all); break;
-      case 256: batch_item<256>(P, bi, lds_all); break;
-      case 512: batch_item<512>(P, bi, lds_all); break;
-      case 1024: batch_item<1024>(P, bi, lds_all); break;
-      case 2048: batch_item<2048>(P, bi, lds_all); break;
-      case 4096: batch_item<4096>(P, bi, lds_all); break;
-      default:
-        if constexpr (MAXL > 4096) batch_item<8192>(P, bi, lds_all);
-        break;
+      case 8: batch_item_shape<8, MAXL>(P, bi, lds_all); break;
+      case 16: batch_item_shape<16, MAXL>(P, bi, lds_all); break;
+      case 32: batch_item_shape<32, MAXL>(P, bi, lds_all); break;
+      case 64: batch_item_shape<64, MAXL>(P, bi, lds_all); break;
+      case 128: batch_item_shape<128, MAXL>(P, bi, lds_all); break;
+      case 256: batch_item_shape<256, MAXL>(P, bi, lds_all); break;
+      case 512: batch_item_shape<512, MAXL>(P, bi, lds_all); break;
+      case 1024: batch_item_shape<1024, MAXL>(P, bi, lds_all); break;
+      case 2048: batch_item_shape<2048, MAXL>(P, bi, lds_all); break;
+      case 4096: batch_item_shape<4096, MAXL>(P, bi, lds_all); break;
+      default: batch_item_shape<8192, MAXL>(P, bi, lds_all); break;
     }
   }
 }

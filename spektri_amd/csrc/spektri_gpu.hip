@@ -118,6 +118,7 @@ struct ChanGroup {  // channels sharing one IFFT size
   DevBuf gtw;              // w_L^e, e = 0..L-1, inverse sign
   DevBuf pk_off;           // long long per channel: packed-layout offset (samples per block before it)
   bool fused = false;      // computed inside the forward kernel
+  int ps_shape = 0;        // tile shape of the per-size launch (tw2/tw3 are its tables)
 };
 
 // Output samples per block of a channel of IFFT size L: the reference keeps
@@ -172,6 +173,61 @@ const ChanPlan* find_chan_plan(int L) {
   for (const auto& p : kChanPlans)
     if (p.L == L) return &p;
   return nullptr;
+}
+
+// Tile shapes of the power-of-two channel transform (spk::TileShape): the
+// radix plan of shape S of IFFT size L.  The batched kernel serves every
+// shape: chan_batched_kernel<4096> those of at most 16 points per thread,
+// chan_batched_kernel<8192> (twice the LDS) the rest.
+struct ShapeInfo {
+  int R = 0, R1 = 0, R2 = 0, R3 = 0;
+  bool big() const { return R > spk::batch_max_points(4096); }
+};
+template <int L, int S>
+void shape_entry(std::vector<ShapeInfo>& v) {
+  if constexpr (spk::TileShape<L, S>::valid) {
+    using TS = spk::TileShape<L, S>;
+    static_assert(spk::batch_shape_ok<L, S, 8192>(), "every shape is served by chan_batched_kernel<8192>");
+    v.push_back(ShapeInfo{TS::R, TS::R1, TS::R2, TS::R3});
+  }
+}
+template <int L>
+std::vector<ShapeInfo> shapes_of() {
+  std::vector<ShapeInfo> v;
+  shape_entry<L, 0>(v);
+  shape_entry<L, 1>(v);
+  shape_entry<L, 2>(v);
+  return v;
+}
+std::vector<ShapeInfo> tile_shapes(int L) {
+  switch (L) {
+    case 8: return shapes_of<8>();
+    case 16: return shapes_of<16>();
+    case 32: return shapes_of<32>();
+    case 64: return shapes_of<64>();
+    case 128: return shapes_of<128>();
+    case 256: return shapes_of<256>();
+    case 512: return shapes_of<512>();
+    case 1024: return shapes_of<1024>();
+    case 2048: return shapes_of<2048>();
+    case 4096: return shapes_of<4096>();
+    case 8192: return shapes_of<8192>();
+    default: return {};
+  }
+}
+int default_shape(int L, bool batched);
+int shape_of(const std::map<int, int>& m, int L, bool batched) {
+  const auto it = m.find(L);
+  return it == m.end() ? default_shape(L, batched) : it->second;
+}
+int default_shape(int L, bool batched) {
+  if (batched) return 0;
+  switch (L) {
+    case 512: return spk::kPerSizeDefaultShape<512>;
+    case 1024: return spk::kPerSizeDefaultShape<1024>;
+    case 2048: return spk::kPerSizeDefaultShape<2048>;
+    default: return 0;
+  }
 }
 // Pass radices of the mixed-radix forward transform of Nc points: eights,
 // then a four or a two, then odd primes up to 31 (the instantiated
@@ -263,10 +319,15 @@ const FormatInfo* format_info(uint32_t fmt) {
 // and freed only together with every graph (drop_graphs_and_plans).
 struct BatchPlan {
   std::vector<int> cls_L;
+  std::vector<int> cls_shape;          // tile shape, or -1 for a mixed-radix class
   DevBuf descs, items, item_off;
-  std::vector<DevBuf> tw2, tw3;
+  std::vector<DevBuf> tw2, tw3;        // pass tables of the power-of-two classes
+  std::vector<const cpx*> twp2, twp3;  // per class: the tables the kernel reads
+  std::vector<int> gen_T, gen_nst;     // mixed-radix classes: threads per tile, stages
+  std::vector<std::vector<int>> gen_radix;
   int nb = -1;     // block count the items were cut for
   int maxL = 0;
+  bool big = false;  // chan_batched_kernel<8192>: a size above 4096 or a shape above 16 points per thread
   size_t nitems = 0;
 };
 constexpr size_t kMaxCachedPlans = 8;  // graphs and batch plans kept at once
@@ -329,6 +390,9 @@ struct spk_ctx {
   // channel stage launch form and batched tuning (chunk blocks, rounds per item, workgroups per CU)
   int chan_mode = SPK_CHAN_PER_SIZE;
   int tune_chunk = 8, tune_rounds = 2, tune_wgs_per_cu = 0;
+  // Tile shape per power-of-two IFFT size (absent: the form's default), for
+  // the per-size launch and the batched kernel (spk_set_tile_shape, spk_autotune).
+  std::map<int, int> shape_ps, shape_b;
   std::map<int, BatchPlan> batches;  // per block count
   DevBuf chan_counters;      // kXcdQueues ints
   DevBuf pos;                // device stream position (blocks consumed), two long long slots
@@ -640,8 +704,10 @@ spk_status rebuild_channels(spk_ctx* c) {
       HIP_TRY(c, upload(g.gtw, tw));
       continue;
     }
-    HIP_TRY(c, upload(g.tw2, pass_table(p->R2, p->R1, +1)));
-    HIP_TRY(c, upload(g.tw3, pass_table(p->R3, p->R1 * p->R2, +1)));
+    g.ps_shape = shape_of(c->shape_ps, g.L, false);
+    const ShapeInfo sh = tile_shapes(g.L).at((size_t)g.ps_shape);
+    HIP_TRY(c, upload(g.tw2, pass_table(sh.R2, sh.R1, +1)));
+    HIP_TRY(c, upload(g.tw3, pass_table(sh.R3, sh.R1 * sh.R2, +1)));
     HIP_TRY(c, upload(g.atw2, pass_table(p->R2, p->R1, -1)));
     HIP_TRY(c, upload(g.atw3, pass_table(p->R3, p->R1 * p->R2, -1)));
   }
@@ -764,21 +830,44 @@ void launch_chan(const spk::ChanParams& P, hipStream_t s) {
   hipLaunchKernelGGL((spk::chan_kernel<L, R, R1, R2, R3>), dim3(grid), dim3(spk::kChanThreads), 0, s, P);
 }
 
-hipError_t dispatch_chan(int L, const spk::ChanParams& P, hipStream_t s) {
-  switch (L) {
-    case 8: launch_chan<8, 4, 4, 2, 1>(P, s); break;
-    case 16: launch_chan<16, 4, 4, 4, 1>(P, s); break;
-    case 32: launch_chan<32, 8, 8, 4, 1>(P, s); break;
-    case 64: launch_chan<64, 8, 8, 8, 1>(P, s); break;
-    case 128: launch_chan<128, 16, 16, 8, 1>(P, s); break;
-    case 256: launch_chan<256, 16, 16, 16, 1>(P, s); break;
-    case 512: launch_chan<512, 32, 32, 16, 1>(P, s); break;
-    case 1024: launch_chan<1024, 32, 32, 32, 1>(P, s); break;
-    case 2048: launch_chan<2048, 32, 32, 32, 2>(P, s); break;
-    case 4096: launch_chan<4096, 16, 16, 16, 16>(P, s); break;
-    case 8192: launch_chan<8192, 32, 32, 16, 16>(P, s); break;
-    default: return hipErrorInvalidValue;
+template <int L, int S>
+bool launch_chan_shape(const spk::ChanParams& P, hipStream_t s) {
+  if constexpr (spk::TileShape<L, S>::valid) {
+    using TS = spk::TileShape<L, S>;
+    launch_chan<L, TS::R, TS::R1, TS::R2, TS::R3>(P, s);
+    return true;
+  } else {
+    return false;
   }
+}
+template <int L>
+bool launch_chan_any(int S, const spk::ChanParams& P, hipStream_t s) {
+  switch (S) {
+    case 0: return launch_chan_shape<L, 0>(P, s);
+    case 1: return launch_chan_shape<L, 1>(P, s);
+    case 2: return launch_chan_shape<L, 2>(P, s);
+    default: return false;
+  }
+}
+// Per-size launch of IFFT size L in tile shape S (the tables in P must be
+// that shape's).
+hipError_t dispatch_chan(int L, int S, const spk::ChanParams& P, hipStream_t s) {
+  bool ok = false;
+  switch (L) {
+    case 8: ok = launch_chan_any<8>(S, P, s); break;
+    case 16: ok = launch_chan_any<16>(S, P, s); break;
+    case 32: ok = launch_chan_any<32>(S, P, s); break;
+    case 64: ok = launch_chan_any<64>(S, P, s); break;
+    case 128: ok = launch_chan_any<128>(S, P, s); break;
+    case 256: ok = launch_chan_any<256>(S, P, s); break;
+    case 512: ok = launch_chan_any<512>(S, P, s); break;
+    case 1024: ok = launch_chan_any<1024>(S, P, s); break;
+    case 2048: ok = launch_chan_any<2048>(S, P, s); break;
+    case 4096: ok = launch_chan_any<4096>(S, P, s); break;
+    case 8192: ok = launch_chan_any<8192>(S, P, s); break;
+    default: break;
+  }
+  if (!ok) return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
@@ -815,32 +904,37 @@ int batched_wgs_per_cu() {
   return n;
 }
 
-int batch_T(int L) {
-  switch (L) {
-    case 8: case 16: return L / 4;
-    case 32: case 64: return L / 8;
-    case 8192: return L / 32;
-    default: return L / 16;
-  }
+// Threads per tile of the mixed-radix channel transform: about two outputs
+// per thread and stage, 32 to 256.
+int gen_tile_threads(int L) {
+  int T = 32;
+  while (T < 256 && 2 * T < L) T *= 2;
+  return T;
 }
-int batch_R1(int L) { return L == 8 || L == 16 ? 4 : (L <= 64 ? 8 : (L == 8192 ? 32 : 16)); }
-int batch_R2(int L) {
-  switch (L) {
-    case 8: return 2;
-    case 16: return 4;
-    case 32: return 4;
-    case 64: return 8;
-    case 128: return 8;
-    default: return 16;
-  }
+// A mixed-radix size whose tiles (two buffers of L values each) fit the
+// batched kernel's LDS.
+bool gen_fits_batch(int L) {
+  if (L > spk::kGenMaxL) return false;
+  const long long bytes = (long long)(spk::kChanThreads / gen_tile_threads(L)) * 2 * L * (long long)sizeof(cpx);
+  return bytes <= (long long)spk::batch_kernel_lds_floats<4096>() * (long long)sizeof(float);
 }
-
-// Channels that the batched kernel serves: every group except the fused one.
+// Channels that the batched kernel serves: every power-of-two group except
+// the fused one, and the mixed-radix groups whose tiles fit its LDS (up to
+// spk::kMaxClasses classes; other groups keep their own launch).
 std::vector<int> batched_groups(const spk_ctx* c) {
-  std::vector<int> ls;
-  for (const auto& kv : c->groups)
-    if (!kv.second.fused && !kv.second.generic) ls.push_back(kv.first);
+  std::vector<int> ls, gen;
+  for (const auto& kv : c->groups) {
+    if (kv.second.fused) continue;
+    if (!kv.second.generic) ls.push_back(kv.first);
+    else if (gen_fits_batch(kv.first)) gen.push_back(kv.first);
+  }
+  for (int L : gen)
+    if ((int)ls.size() < spk::kMaxClasses) ls.push_back(L);
   return ls;
+}
+bool in_batch(const spk_ctx* c, int L) {
+  const std::vector<int> ls = batched_groups(c);
+  return std::find(ls.begin(), ls.end(), L) != ls.end();
 }
 
 // One launch of spk::chan_generic_kernel for a group without a
@@ -882,8 +976,7 @@ hipError_t launch_generic(const spk_ctx* c, const ChanGroup& g, int nb, hipStrea
   Q.L = g.L;
   Q.nstages = (int)g.radix.size();
   for (size_t i = 0; i < g.radix.size(); ++i) Q.radix[i] = g.radix[i];
-  int T = 32;
-  while (T < 256 && 2 * T < g.L) T *= 2;  // about two outputs per thread and stage
+  const int T = gen_tile_threads(g.L);
   Q.tile_threads = T;
   const int tpw = spk::kGenThreads / T;
   const long long tiles = (long long)Q.nch * nb;
@@ -901,14 +994,22 @@ spk_status build_batch(spk_ctx* c, int nb) {
   std::vector<int> ls = batched_groups(c);
   std::sort(ls.begin(), ls.end(), std::greater<int>());  // big tiles first within a chunk
   B.cls_L = ls;
-  B.maxL = ls.empty() ? 0 : ls.front();
+  B.maxL = 0;
+  B.big = false;
   std::vector<spk::ChanDesc> descs;
-  std::vector<int> ch0(ls.size()), nch(ls.size());
+  std::vector<int> ch0(ls.size()), nch(ls.size()), ttile(ls.size());
+  const size_t nc = ls.size();
   B.tw2.clear();
   B.tw3.clear();
-  B.tw2.resize(ls.size());
-  B.tw3.resize(ls.size());
-  for (size_t k = 0; k < ls.size(); ++k) {
+  B.tw2.resize(nc);
+  B.tw3.resize(nc);
+  B.cls_shape.assign(nc, -1);
+  B.twp2.assign(nc, nullptr);
+  B.twp3.assign(nc, nullptr);
+  B.gen_T.assign(nc, 0);
+  B.gen_nst.assign(nc, 0);
+  B.gen_radix.assign(nc, std::vector<int>());
+  for (size_t k = 0; k < nc; ++k) {
     const ChanGroup& g = c->groups.at(ls[k]);
     ch0[k] = (int)descs.size();
     nch[k] = (int)g.ids.size();
@@ -918,11 +1019,26 @@ spk_status build_batch(spk_ctx* c, int nb) {
       if (fm < 0) fm += c->N;
       descs.push_back(spk::ChanDesc{c->chan_out_off[id], (int)fm, c->chan_w_off[id]});
     }
-    const int L = ls[k], R1 = batch_R1(L), R2 = batch_R2(L), R = L / batch_T(L);
-    const int R3 = L / (R1 * R2);
-    (void)R;
-    HIP_TRY(c, upload(B.tw2[k], pass_table(R2, R1, +1)));
-    HIP_TRY(c, upload(B.tw3[k], pass_table(R3, R1 * R2, +1)));
+    const int L = ls[k];
+    if (g.generic) {
+      ttile[k] = gen_tile_threads(L);
+      B.gen_T[k] = ttile[k];
+      B.gen_nst[k] = (int)g.radix.size();
+      B.gen_radix[k] = g.radix;
+      B.twp2[k] = g.gtw.as<cpx>();
+      B.twp3[k] = g.gtw.as<cpx>();
+      continue;
+    }
+    B.maxL = std::max(B.maxL, L);
+    const int S = shape_of(c->shape_b, L, true);
+    const ShapeInfo sh = tile_shapes(L).at((size_t)S);
+    B.big = B.big || L > 4096 || sh.big();
+    B.cls_shape[k] = S;
+    ttile[k] = L / sh.R;
+    HIP_TRY(c, upload(B.tw2[k], pass_table(sh.R2, sh.R1, +1)));
+    HIP_TRY(c, upload(B.tw3[k], pass_table(sh.R3, sh.R1 * sh.R2, +1)));
+    B.twp2[k] = B.tw2[k].as<cpx>();
+    B.twp3[k] = B.tw3[k].as<cpx>();
   }
   std::vector<std::vector<spk::BatchItem>> q(kXcdQueues);
   const int chunk = std::max(1, c->tune_chunk);
@@ -930,7 +1046,7 @@ spk_status build_batch(spk_ctx* c, int nb) {
     const int nbk = std::min(chunk, nb - g0);
     auto& qx = q[k % kXcdQueues];
     for (size_t cl = 0; cl < ls.size(); ++cl) {
-      const int tpw = spk::kChanThreads / batch_T(ls[cl]);
+      const int tpw = spk::kChanThreads / ttile[cl];
       const int per = tpw * std::max(1, c->tune_rounds);
       const int ntiles = nch[cl] * nbk;
       for (int t0 = 0; t0 < ntiles; t0 += per)
@@ -974,10 +1090,14 @@ spk_status launch_batched(spk_ctx* c, int nb, hipStream_t s) {
   P.X = kXcdQueues;
   for (size_t k = 0; k < B.cls_L.size(); ++k) {
     P.cls_L[k] = B.cls_L[k];
-    P.tw2[k] = B.tw2[k].as<cpx>();
-    P.tw3[k] = B.tw3[k].as<cpx>();
+    P.cls_shape[k] = B.cls_shape[k];
+    P.tw2[k] = B.twp2[k];
+    P.tw3[k] = B.twp3[k];
+    P.gen_T[k] = B.gen_T[k];
+    P.gen_nst[k] = B.gen_nst[k];
+    for (size_t i = 0; i < B.gen_radix[k].size(); ++i) P.gen_radix[k][i] = B.gen_radix[k][i];
   }
-  const bool big = B.maxL > 4096;
+  const bool big = B.big;
   int per_cu = big ? batched_wgs_per_cu<8192>() : batched_wgs_per_cu<4096>();
   if (c->tune_wgs_per_cu > 0) per_cu = std::min(per_cu, c->tune_wgs_per_cu);
   int G = std::max(1, per_cu) * c->ncu;
@@ -1252,10 +1372,12 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
 
 // Channel stage for every channel not fused into the forward kernel.
 spk_status enqueue_channels(spk_ctx* c, int nb, hipStream_t s) {
-  // sizes without a power-of-two plan have their own kernel in either form
+  // sizes without a power-of-two plan: their own kernel, unless the batched
+  // kernel takes them
+  const bool batched = c->chan_mode == SPK_CHAN_BATCHED;
   for (auto& kv : c->groups)
-    if (kv.second.generic) HIP_TRY(c, launch_generic(c, kv.second, nb, s));
-  if (c->chan_mode == SPK_CHAN_BATCHED) return launch_batched(c, nb, s);
+    if (kv.second.generic && !(batched && in_batch(c, kv.first))) HIP_TRY(c, launch_generic(c, kv.second, nb, s));
+  if (batched) return launch_batched(c, nb, s);
   for (auto& kv : c->groups) {
     ChanGroup& g = kv.second;
     if (g.fused || g.generic) continue;
@@ -1272,7 +1394,7 @@ spk_status enqueue_channels(spk_ctx* c, int nb, hipStream_t s) {
     Q.out = c->chan_out.as<cpx>();
     Q.tw2 = g.tw2.as<cpx>();
     Q.tw3 = g.tw3.as<cpx>();
-    HIP_TRY(c, dispatch_chan(g.L, Q, s));
+    HIP_TRY(c, dispatch_chan(g.L, g.ps_shape, Q, s));
   }
   return SPK_OK;
 }
@@ -2459,61 +2581,125 @@ spk_status spk_autotune(spk_ctx* c, uint32_t nblocks, spk_tune_result* out) {
     }
     return best;
   };
-  // the plain per-size launch
+  auto finish = [&](spk_status st) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
+  };
+  auto per_size_params = [&](const ChanGroup& g, const DevBuf& t2, const DevBuf& t3) {
+    spk::ChanParams Q{};
+    Q.spec = c->spec.as<cpx>();
+    Q.spec_stride = c->spec_stride;
+    Q.N = c->N;
+    Q.K = c->K;
+    Q.real_input = c->complex_in ? 0 : 1;
+    Q.weights = c->weights.as<float>();
+    Q.chans = g.desc.as<spk::ChanDesc>();
+    Q.nch = (int)g.ids.size();
+    Q.nblocks = nb;
+    Q.out = c->chan_out.as<cpx>();
+    Q.tw2 = t2.as<cpx>();
+    Q.tw3 = t3.as<cpx>();
+    return Q;
+  };
+  // 1. Tile shape of every power-of-two size (fcfb.rs:66-100 runs one
+  // filter per channel; here one launch per size and shape): each shape's
+  // per-size launch over the size's channels is timed; the per-size form
+  // takes the fastest, the batched kernel the fastest it serves.
+  // (best_small: the fastest shape of at most 16 points per thread, which
+  // keeps chan_batched_kernel<4096> and four workgroups per CU)
+  std::map<int, int> best_small, best_any;
+  for (auto& kv : c->groups) {
+    ChanGroup& g = kv.second;
+    if (g.fused || g.generic) continue;
+    const std::vector<ShapeInfo> shapes = tile_shapes(g.L);
+    if (shapes.size() < 2) continue;
+    int best_ps = -1, best_b = -1;
+    float t_ps = 1e30f, t_b = 1e30f;
+    for (size_t S = 0; S < shapes.size(); ++S) {
+      DevBuf t2, t3;
+      HIP_TRY(c, upload(t2, pass_table(shapes[S].R2, shapes[S].R1, +1)));
+      HIP_TRY(c, upload(t3, pass_table(shapes[S].R3, shapes[S].R1 * shapes[S].R2, +1)));
+      const spk::ChanParams Q = per_size_params(g, t2, t3);
+      const float ms = time_it([&]() -> spk_status {
+        return dispatch_chan(g.L, (int)S, Q, s) == hipSuccess ? SPK_OK : SPK_ERR_DEVICE;
+      });
+      HIP_TRY(c, hipStreamSynchronize(s));  // before the tables are freed
+      if (ms < 0.f) return finish(fail(c, SPK_ERR_DEVICE, "autotune launch failed"));
+      if (ms < t_ps) {
+        t_ps = ms;
+        best_ps = (int)S;
+      }
+      if (!shapes[S].big() && ms < t_b) {
+        t_b = ms;
+        best_b = (int)S;
+      }
+    }
+    c->shape_ps[g.L] = best_ps;
+    best_any[g.L] = best_ps;
+    best_small[g.L] = best_b >= 0 ? best_b : best_ps;
+    g.ps_shape = best_ps;
+    HIP_TRY(c, upload(g.tw2, pass_table(shapes[(size_t)best_ps].R2, shapes[(size_t)best_ps].R1, +1)));
+    HIP_TRY(c, upload(g.tw3, pass_table(shapes[(size_t)best_ps].R3,
+                                        shapes[(size_t)best_ps].R1 * shapes[(size_t)best_ps].R2, +1)));
+  }
+  // 2. The per-size form with those shapes: every group the batched kernel
+  // would take, in its own launch.
   const float per_size = time_it([&]() -> spk_status {
     for (auto& kv : c->groups) {
       ChanGroup& g = kv.second;
-      if (g.fused || g.generic) continue;  // same kernel in both forms
-      spk::ChanParams Q{};
-      Q.spec = c->spec.as<cpx>();
-      Q.spec_stride = c->spec_stride;
-      Q.N = c->N;
-      Q.K = c->K;
-      Q.real_input = c->complex_in ? 0 : 1;
-      Q.weights = c->weights.as<float>();
-      Q.chans = g.desc.as<spk::ChanDesc>();
-      Q.nch = (int)g.ids.size();
-      Q.nblocks = nb;
-      Q.out = c->chan_out.as<cpx>();
-      Q.tw2 = g.tw2.as<cpx>();
-      Q.tw3 = g.tw3.as<cpx>();
-      if (dispatch_chan(g.L, Q, s) != hipSuccess) return SPK_ERR_DEVICE;
+      if (g.fused || !in_batch(c, g.L)) continue;
+      if (g.generic) {
+        if (launch_generic(c, g, nb, s) != hipSuccess) return SPK_ERR_DEVICE;
+        continue;
+      }
+      if (dispatch_chan(g.L, g.ps_shape, per_size_params(g, g.tw2, g.tw3), s) != hipSuccess) return SPK_ERR_DEVICE;
     }
     return SPK_OK;
   });
+  // 3. The batched form: the shapes of either LDS tier (every size's
+  // fastest shape, or its fastest of at most 16 points per thread), then
+  // the work split (chunk of blocks per item, tile rounds per item,
+  // workgroups per CU).
   const int chunks[] = {2, 4, 8, 16};
   const int rounds[] = {1, 2, 4};
   const int wpcu[] = {0, 2};
   float best = 1e30f;
   int bc = c->tune_chunk, br = c->tune_rounds, bw = c->tune_wgs_per_cu;
-  for (int ch : chunks)
-    for (int rd : rounds)
-      for (int w : wpcu) {
-        c->tune_chunk = ch;
-        c->tune_rounds = rd;
-        c->tune_wgs_per_cu = w;
-        c->batches.clear();
-        const float ms = time_it([&]() { return launch_batched(c, nb, s); });
-        if (ms < 0.f) {
-          (void)hipEventDestroy(e0);
-          (void)hipEventDestroy(e1);
-          return fail(c, SPK_ERR_DEVICE, "autotune launch failed");
+  std::map<int, int> bshape = c->shape_b;
+  for (const std::map<int, int>* tier : {&best_small, &best_any}) {
+    if (tier == &best_any && best_any == best_small) break;
+    for (const auto& kv : *tier) c->shape_b[kv.first] = kv.second;
+    for (int ch : chunks)
+      for (int rd : rounds)
+        for (int w : wpcu) {
+          c->tune_chunk = ch;
+          c->tune_rounds = rd;
+          c->tune_wgs_per_cu = w;
+          c->batches.clear();
+          const float ms = time_it([&]() { return launch_batched(c, nb, s); });
+          if (ms < 0.f) return finish(fail(c, SPK_ERR_DEVICE, "autotune launch failed"));
+          if (ms < best) {
+            best = ms;
+            bc = ch;
+            br = rd;
+            bw = w;
+            bshape = c->shape_b;
+          }
         }
-        if (ms < best) {
-          best = ms;
-          bc = ch;
-          br = rd;
-          bw = w;
-        }
-      }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  }
+  c->shape_b = bshape;
+  finish(SPK_OK);
   HIP_TRY(c, hipMemsetAsync(c->chan_counters.p, 0, c->chan_counters.bytes, s));
   HIP_TRY(c, hipStreamSynchronize(s));
   c->tune_chunk = bc;
   c->tune_rounds = br;
   c->tune_wgs_per_cu = bw;
-  c->chan_mode = best < per_size ? SPK_CHAN_BATCHED : SPK_CHAN_PER_SIZE;
+  // The batched form must win by 2 %: measured in place, right after the
+  // forward kernel, its launch has cost up to ~3 % more than alone (the
+  // per-size launches cost the same in place), and the timings here vary by
+  // about 1.5 % from run to run.
+  c->chan_mode = best < 0.98f * per_size ? SPK_CHAN_BATCHED : SPK_CHAN_PER_SIZE;
   drop_graphs_and_plans(c);
   if (out) {
     out->per_size_ms = per_size;
@@ -2523,6 +2709,37 @@ spk_status spk_autotune(spk_ctx* c, uint32_t nblocks, spk_tune_result* out) {
     out->wgs_per_cu = (uint32_t)bw;
     out->mode = c->chan_mode;
   }
+  return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
+}
+
+spk_status spk_set_tile_shape(spk_ctx* c, uint32_t bins, int form, uint32_t shape) {
+  try {
+  if (!c || (form != SPK_CHAN_PER_SIZE && form != SPK_CHAN_BATCHED)) return SPK_ERR_INVALID;
+  const std::vector<ShapeInfo> shapes = tile_shapes((int)bins);
+  if (shape >= shapes.size()) return fail(c, SPK_ERR_INVALID, "no such tile shape for this IFFT size");
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  (form == SPK_CHAN_BATCHED ? c->shape_b : c->shape_ps)[(int)bins] = (int)shape;
+  drop_graphs_and_plans(c);
+  c->chans_dirty = true;  // the per-size tables are uploaded again
+  return SPK_OK;
+  } catch (...) {
+    return entry_exception(c);
+  }
+}
+
+spk_status spk_tile_shape(spk_ctx* c, uint32_t bins, int form, uint32_t* shape, uint32_t* threads_per_tile,
+                          uint32_t* points_per_thread) {
+  try {
+  if (!c || (form != SPK_CHAN_PER_SIZE && form != SPK_CHAN_BATCHED)) return SPK_ERR_INVALID;
+  const std::vector<ShapeInfo> shapes = tile_shapes((int)bins);
+  if (shapes.empty()) return fail(c, SPK_ERR_INVALID, "no tile shapes for this IFFT size");
+  const int S = shape_of(form == SPK_CHAN_BATCHED ? c->shape_b : c->shape_ps, (int)bins, form == SPK_CHAN_BATCHED);
+  if (shape) *shape = (uint32_t)S;
+  if (threads_per_tile) *threads_per_tile = (uint32_t)(bins / (uint32_t)shapes[(size_t)S].R);
+  if (points_per_thread) *points_per_thread = (uint32_t)shapes[(size_t)S].R;
   return SPK_OK;
   } catch (...) {
     return entry_exception(c);

@@ -250,13 +250,34 @@ class DspState:
         _lib.check(_lib.load().spk_set_channel_mode(self._ctx, mode), self._ctx)
 
     def autotune(self, nblocks: int = 0) -> dict:
-        """Time both forms on the current spectra, keep the fastest batched
-        settings and select the faster form."""
+        """Time every tile shape per IFFT size and both forms on the current
+        spectra, keep the fastest shapes and batched settings and select the
+        faster form."""
         r = _lib.SpkTuneResult()
         _lib.check(_lib.load().spk_autotune(self._ctx, nblocks, ctypes.byref(r)), self._ctx)
-        return {"per_size_ms": r.per_size_ms, "batched_ms": r.batched_ms, "chunk_blocks": r.chunk_blocks,
-                "rounds_per_item": r.rounds_per_item, "wgs_per_cu": r.wgs_per_cu,
-                "mode": "batched" if r.mode == _lib.CHAN_BATCHED else "per_size"}
+        res = {"per_size_ms": r.per_size_ms, "batched_ms": r.batched_ms, "chunk_blocks": r.chunk_blocks,
+               "rounds_per_item": r.rounds_per_item, "wgs_per_cu": r.wgs_per_cu,
+               "mode": "batched" if r.mode == _lib.CHAN_BATCHED else "per_size"}
+        shapes = {}
+        for L in sorted({b.bins for _, b in self.channels}):
+            if L & (L - 1) or L < 8 or L > 8192:
+                continue
+            shapes[L] = {f: self.tile_shape(L, f == "batched")["points_per_thread"] for f in ("per_size", "batched")}
+        if shapes:
+            res["points_per_thread"] = shapes
+        return res
+
+    def set_tile_shape(self, bins: int, batched: bool, shape: int):
+        """Fix tile shape `shape` of IFFT size `bins` for one launch form."""
+        form = _lib.CHAN_BATCHED if batched else _lib.CHAN_PER_SIZE
+        _lib.check(_lib.load().spk_set_tile_shape(self._ctx, bins, form, shape), self._ctx)
+
+    def tile_shape(self, bins: int, batched: bool) -> dict:
+        """Tile shape in use for IFFT size `bins` in one launch form."""
+        form = _lib.CHAN_BATCHED if batched else _lib.CHAN_PER_SIZE
+        v = [ctypes.c_uint32() for _ in range(3)]
+        _lib.check(_lib.load().spk_tile_shape(self._ctx, bins, form, *[ctypes.byref(x) for x in v]), self._ctx)
+        return {"shape": v[0].value, "threads_per_tile": v[1].value, "points_per_thread": v[2].value}
 
     def graph_info(self) -> dict:
         """Executable graphs held, instantiated and updated in place so far."""
