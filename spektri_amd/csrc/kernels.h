@@ -367,6 +367,16 @@ __device__ __forceinline__ float welch_term4(cpx c0, cpx cm, cpx cp) {
   return __builtin_fmaf(ci, ci, cr * cr);
 }
 
+// Outputs of the channel kernels (not the forward kernel's fused stage) are
+// written once and not read again by the path: stored non-temporally, so
+// they do not push the spectrum rows, which the channel gathers read many
+// times over, out of L2.  (In the fused stage, whose gathers come from LDS,
+// non-temporal stores were slower.)
+__device__ __forceinline__ void store_out(cpx* p, cpx v) {
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+}
+
 // Symmetric in the two neighbours.
 __device__ __forceinline__ float welch_term(cpx c0, cpx cm, cpx cp) {
   const float cr = c0.x - 0.5f * (cm.x + cp.x);
@@ -1116,7 +1126,7 @@ __device__ __forceinline__ void chan_tile(const cpx* __restrict__ spec, long lon
     for (int e = 0; e < R; ++e) {
       const int u = e / RL, q = e % RL;
       const int idx = t + T * u + q * PL;
-      if (idx >= L / 8 && idx < (L / 8) * 7) out[idx - L / 8] = x[e];
+      if (idx >= L / 8 && idx < (L / 8) * 7) store_out(out + (idx - L / 8), x[e]);
     }
   }
 }
@@ -1211,7 +1221,7 @@ __device__ __forceinline__ void gen_chan_tile(const cpx* __restrict__ spec, long
   if (active) {
     const int lo = L / 8, n = 6 * (L / 8);
     cpx* out = out_base + d.out_off + (long long)g * n;
-    for (int i = t; i < n; i += T) out[i] = a[lo + i];
+    for (int i = t; i < n; i += T) store_out(out + i, a[lo + i]);
   }
 }
 

@@ -2660,8 +2660,9 @@ spk_status spk_autotune(spk_ctx* c, uint32_t nblocks, spk_tune_result* out) {
   // 3. The batched form: the shapes of either LDS tier (every size's
   // fastest shape, or its fastest of at most 16 points per thread), then
   // the work split (chunk of blocks per item, tile rounds per item,
-  // workgroups per CU).
-  const int chunks[] = {2, 4, 8, 16};
+  // workgroups per CU).  One-block chunks keep the fewest spectrum rows per
+  // XCD live, which suits channel sets that read each row many times over.
+  const int chunks[] = {1, 2, 4, 8, 16};
   const int rounds[] = {1, 2, 4};
   const int wpcu[] = {0, 2};
   float best = 1e30f;
