@@ -351,7 +351,7 @@ def measure_design(wl, steps, warmup):
     def step():
         opt.zero_grad()
         out = bank.apply(w)
-        loss = loss_fn(bank.split(out))
+        loss = loss_fn(out)
         loss.backward()
         opt.step()
         return loss

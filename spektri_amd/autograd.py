@@ -24,9 +24,10 @@ from .dsp import DspParams, DspState
 class _StageFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, weights, bank):
-        bank._run(weights)
+        out = torch.empty(bank.output_floats // 2, 2, dtype=torch.float32, device=weights.device)
+        bank._run(weights, out)
         ctx.bank = bank
-        return bank._out.clone()
+        return out
 
     @staticmethod
     def backward(ctx, gout):
@@ -62,9 +63,8 @@ class ChannelBank:
         self.weight_count = sum(self.layout)
         self.nblocks = nblocks
         self.output_floats = 2 * nblocks * sum(6 * (L // 8) for L in self.layout)
-        self._out = torch.empty(self.output_floats // 2, 2, dtype=torch.float32, device=samples.device)
 
-    def _run(self, weights: torch.Tensor):
+    def _run(self, weights: torch.Tensor, out: torch.Tensor):
         w = weights.detach().contiguous().to(torch.float32)
         if w.numel() != self.weight_count or not w.is_cuda:
             raise ValueError(f"weights must be a device tensor of {self.weight_count} floats")
@@ -75,7 +75,7 @@ class ChannelBank:
         st.process_device(self.samples.data_ptr(), self.samples.numel())
         if st.last_blocks() != self.nblocks:
             raise _lib.SpektriError(_lib.SPK_OK, "unexpected block count")
-        st.channels_output_packed(self._out.data_ptr(), self.output_floats)
+        st.channels_output_packed(out.data_ptr(), self.output_floats)
         st.synchronize()
 
     def apply(self, weights: torch.Tensor) -> torch.Tensor:

@@ -92,9 +92,10 @@ struct FinParams {
   int ncounters;
   unsigned char* qout;  // quantised records (qfmt 1: K bytes, 2: K big-endian u16) or null
   int qfmt;
-  // Forward kernel's record phase: S teams of T / S threads share a chunk of
-  // 4 T / S positions, team s summing the s-th range of the group's
-  // workgroups (a power of two; 1 = one team).
+  // Teams per chunk (a power of two; 1 = one team): in the forward kernel's
+  // record phase S teams of T / S threads share a chunk of 4 T / S
+  // positions, in welch_finalize_kernel S teams of 256 / S threads; team s
+  // sums the s-th range of the group's workgroups.
   int split;
 };
 
@@ -1053,14 +1054,6 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
 }
 
 // ---------------------------------------------------------------------------
-// Welch terms of the edge lanes (spektri/src/dsp/spectrum.rs:95-111): the
-// bins whose neighbours live in another wave or wrap across slots, read from
-// the per-block edge dump.  One thread per (workgroup, slot, edge bin) sums
-// the terms of the workgroup's blocks in that Welch group and writes the
-// partial row entry the forward kernel left out.  Items: lanes 0, 31, 32, 63
-// of every wave and slot, wave 0's lane 33, and the Nyquist bin.
-// ---------------------------------------------------------------------------
-// ---------------------------------------------------------------------------
 // Channel stage, one launch per IFFT size L (spektri/src/dsp/fcfb.rs:154-188):
 //   B[i] = W[j] get_bin(first + j), j = (i + L/2) mod L,
 //   y = unnormalised inverse FFT_L(B), output y[L/8 .. 7L/8).
@@ -1457,7 +1450,15 @@ __global__ __launch_bounds__(kChanThreads) void chan_batched_kernel(BatchParams 
 // launch touches (group 0 is the group of the launch's first block).  Row
 // y = 0 also does the next call's housekeeping the fields ask for (input
 // history, queue counters, position slot).
-__global__ void welch_finalize_kernel(FinParams F) {
+// Grid: x = chunks of blockDim.x / F.split quads, y = Welch groups.  Each
+// workgroup holds F.split teams; team k sums the k-th of F.split contiguous
+// ranges of the group's partial rows over the chunk, and team 0 adds the
+// teams' sums in order (a walk over every row by one thread is latency-bound
+// when a group spans many workgroups).  The last chunk also carries the
+// Nyquist position.
+__global__ __launch_bounds__(256) void welch_finalize_kernel(FinParams F) {
+  __shared__ float4 stage[256];
+  __shared__ float nstage[16];
   const int gi = (int)blockIdx.y;
   const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const long long pos = *F.pos;
@@ -1466,11 +1467,46 @@ __global__ void welch_finalize_kernel(FinParams F) {
     if (tid < F.ncounters) F.chan_counters[tid] = 0;
     if (tid == 0 && F.pos_next) *F.pos_next = pos + F.nblocks;
   }
-  const int i0 = 4 * tid;
   const int blk0_mod = (int)(pos % F.A);
-  // rows beyond the groups this launch touches (the grid is sized for the worst case)
-  if (i0 > F.M || gi > (blk0_mod + F.nblocks - 1) / F.A) return;
-  welch_finalize_quad(F, pos, gi, i0);
+  // groups beyond the ones this launch touches (the grid is sized for the worst case)
+  if (gi > (blk0_mod + F.nblocks - 1) / F.A) return;
+  const int S = F.split, TS = (int)blockDim.x / S;
+  const int team = (int)threadIdx.x / TS, u = (int)threadIdx.x % TS;
+  const int i0 = 4 * ((int)blockIdx.x * TS + u);
+  const bool valid = i0 < F.M;
+  const bool nyq_chunk = blockIdx.x == gridDim.x - 1;
+  int w0, w1;
+  welch_group_wgs(F, blk0_mod, gi, &w0, &w1);
+  const int nw = w1 - w0 + 1;
+  const int wa = w0 + nw * team / S, wb = w0 + nw * (team + 1) / S - 1;
+  const float4 z = {0.f, 0.f, 0.f, 0.f};
+  const float4 s = valid ? welch_rows_add(F, blk0_mod, gi, i0, wa, wb, team == 0 ? welch_carry(F, blk0_mod, gi, i0) : z)
+                         : z;
+  if (S == 1) {
+    if (valid) welch_emit(F, blk0_mod, gi, i0, s);
+    if (nyq_chunk && threadIdx.x == 0) welch_finalize_quad(F, pos, gi, F.M);
+    return;
+  }
+  stage[threadIdx.x] = s;
+  if (nyq_chunk && u == 0)
+    nstage[team] = welch_rows_add(F, blk0_mod, gi, F.M, wa, wb, team == 0 ? welch_carry(F, blk0_mod, gi, F.M) : z).x;
+  __syncthreads();
+  if (team == 0) {
+    float4 acc = stage[u];
+    for (int k = 1; k < S; ++k) {
+      const float4 v = stage[k * TS + u];
+      acc.x += v.x;
+      acc.y += v.y;
+      acc.z += v.z;
+      acc.w += v.w;
+    }
+    if (valid) welch_emit(F, blk0_mod, gi, i0, acc);
+    if (nyq_chunk && u == 0) {
+      float n = nstage[0];
+      for (int k = 1; k < S; ++k) n += nstage[k];
+      welch_emit(F, blk0_mod, gi, F.M, float4{n, 0.f, 0.f, 0.f});
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

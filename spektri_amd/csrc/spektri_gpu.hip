@@ -1440,10 +1440,16 @@ spk::FinParams fin_params(const spk_ctx* c, int nb, int G, int parity) {
 // once the launches before this one on the stream have run).
 spk_status flush_records(spk_ctx* c) {
   if (c->two_step || c->pend_nb == 0) return SPK_OK;
-  const spk::FinParams F = fin_params(c, c->pend_nb, c->pend_G, c->pend_parity);
+  spk::FinParams F = fin_params(c, c->pend_nb, c->pend_G, c->pend_parity);
   // groups touched: at most (A - 1 + nb - 1) / A + 1; rows past the actual count exit
   const int gmax = (c->A - 1 + c->pend_nb - 1) / c->A + 1;
-  hipLaunchKernelGGL(spk::welch_finalize_kernel, dim3((c->M / 4 + 1 + 255) / 256, gmax), dim3(256), 0, c->stream, F);
+  // teams per workgroup: at least 4 partial rows per team (a group spans at
+  // most A G / nb + 2 workgroups of the launch)
+  const long long rows = std::min<long long>(c->pend_G, (long long)c->A * c->pend_G / c->pend_nb + 2);
+  F.split = 1;
+  while (F.split < 16 && 4LL * 2 * F.split <= rows) F.split *= 2;
+  const int TS = 256 / F.split;
+  hipLaunchKernelGGL(spk::welch_finalize_kernel, dim3((c->M / 4 + TS - 1) / TS, gmax), dim3(256), 0, c->stream, F);
   HIP_TRY(c, hipGetLastError());
   c->pend_nb = 0;
   return SPK_OK;

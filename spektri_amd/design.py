@@ -46,37 +46,61 @@ def tone_terms(wl):
 
 
 class DesignLoss:
-    """loss(ys) for per-channel outputs ys[c] of shape [nblocks * 3L/4, 2]."""
+    """loss(ys) for per-channel outputs ys[c] of shape [nblocks * 3L/4, 2],
+    or for the packed output tensor (every channel's output in channel order,
+    as ChannelBank.apply returns it), from which the rows are gathered with
+    one index per output length."""
 
     def __init__(self, wl, nblocks: int, device, dtype):
         self.rows = tone_terms(wl)
         if not self.rows:
             raise ValueError("no loss terms")
         self.device, self.dtype = device, dtype
-        self.nout = {}
-        bases = []
-        for c, t, j, kind, a in self.rows:
+        # Rows of one output length are projected together, as a few
+        # whole-tensor operations over [rows, samples] (a loop over the rows
+        # would launch thousands of small kernels per training step).
+        offs = np.cumsum([0] + [nblocks * (3 * L // 4) for _, L, _ in wl.channels])
+        by_len = {}
+        for k, (c, t, j, kind, a) in enumerate(self.rows):
             L = wl.channels[c][1]
+            by_len.setdefault(L, []).append(k)
+        self.groups = []
+        order = []
+        for L, ks in by_len.items():
             n_out = nblocks * (3 * L // 4)
             n = np.arange(n_out, dtype=np.int64)
+            js = np.array([self.rows[k][2] for k in ks], dtype=np.int64)
             # theta / 2 pi = (j - L/2) n / L, reduced exactly in integers
-            ph = np.mod((j - L // 2) * n, L).astype(np.float64) / L
-            bases.append((np.cos(2 * np.pi * ph), np.sin(2 * np.pi * ph)))
-            self.nout[c] = n_out
-        self.cos = [torch.tensor(b[0], dtype=dtype, device=device) for b in bases]
-        self.sin = [torch.tensor(b[1], dtype=dtype, device=device) for b in bases]
+            ph = np.mod(np.outer(js - L // 2, n), L).astype(np.float64) / L
+            chans = [self.rows[k][0] for k in ks]
+            uniq = sorted(set(chans))
+            pos = {c: i for i, c in enumerate(uniq)}
+            pidx = torch.tensor(offs[chans][:, None] + n[None, :], device=device)
+            self.groups.append((uniq, torch.tensor([pos[c] for c in chans], device=device), pidx,
+                                torch.tensor(np.cos(2 * np.pi * ph), dtype=dtype, device=device),
+                                torch.tensor(np.sin(2 * np.pi * ph), dtype=dtype, device=device)))
+            order += ks
+        # position of row k in the concatenated group results
+        self.inv = torch.tensor(np.argsort(np.array(order)), device=device)
         self.kind = torch.tensor([r[3] for r in self.rows], device=device)
         self.amp = torch.tensor([r[4] for r in self.rows], dtype=dtype, device=device)
 
     def projections(self, ys):
         pr, pi = [], []
-        for k, (c, t, j, kind, a) in enumerate(self.rows):
-            y = ys[c].to(self.dtype)
-            cs, sn = self.cos[k], self.sin[k]
+        packed = isinstance(ys, torch.Tensor)
+        for uniq, idx, pidx, cs, sn in self.groups:
+            if packed:
+                # (per-channel views of a packed tensor would each give a
+                # full-size gradient in the backward pass)
+                # gather (whose backward is a scatter-add, not a sort)
+                y = torch.gather(ys, 0, pidx.reshape(-1, 1).expand(-1, 2)).view(*pidx.shape, 2).to(self.dtype)
+            else:
+                y = torch.stack([ys[c].to(self.dtype) for c in uniq])[idx]
+            yr, yi = y[..., 0], y[..., 1]
             # y e^{-i theta} = (yr cos + yi sin) + i (yi cos - yr sin)
-            pr.append((y[:, 0] * cs + y[:, 1] * sn).mean())
-            pi.append((y[:, 1] * cs - y[:, 0] * sn).mean())
-        return torch.stack(pr), torch.stack(pi)
+            pr.append((yr * cs + yi * sn).mean(1))
+            pi.append((yi * cs - yr * sn).mean(1))
+        return torch.cat(pr)[self.inv], torch.cat(pi)[self.inv]
 
     def __call__(self, ys):
         pr, pi = self.projections(ys)

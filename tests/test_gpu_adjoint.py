@@ -159,7 +159,7 @@ def test_design_gradient_fd_and_adam():
     loss32 = D.DesignLoss(wl, nb, dev, torch.float32)
     w0 = D.initial_weights(wl, dev).requires_grad_(True)
     y = bank.apply(w0)
-    loss = loss32(bank.split(y))
+    loss = loss32(y)  # packed outputs (the FP64 reference below takes per-channel ones)
     loss.backward()
     g_gpu = w0.grad.detach().cpu().double().numpy()
 
@@ -203,7 +203,7 @@ def test_design_gradient_fd_and_adam():
     losses = []
     for _ in range(200):
         opt.zero_grad()
-        lv = loss32(bank.split(bank.apply(w)))
+        lv = loss32(bank.apply(w))
         lv.backward()
         opt.step()
         losses.append(float(lv.detach()))
