@@ -145,8 +145,9 @@ struct FwdParams {
   // General fused stage (FL = kFusedGen1 / kFusedGen2, below): the scatter
   // tables of fscat hold fused_targets(FL) tables of M/8 uint4 one after the
   // other, entries LDS cpx index | conj << 15; the Nyquist bin's second
-  // target; per-tile descriptors; per (round, thread) slots (see gen_slot);
-  // pass-2 twiddles of the tiles of 32, 64, 128 and 256 points.
+  // target; per (round, thread) slots (see gen_slot) and, beside them, the
+  // descriptors of the slots' tiles; pass-2 twiddles of the tiles of 32, 64,
+  // 128 and 256 points.
   int fscat_nyq2;
   const struct GenTile* gtiles;
   const int* gslots;
@@ -488,10 +489,9 @@ __device__ __forceinline__ void welch_finalize_quad(const FinParams& F, long lon
 // [FL/8, 7FL/8).  The tile's threads lie in one wave and only they touch its
 // region until the next block's first exchange barrier.
 template <int FL>
-__device__ __forceinline__ void gen_tile(const FwdParams& P, cpx* ldsc, int tile, int ftt, bool act, int g) {
+__device__ __forceinline__ void gen_tile(const FwdParams& P, cpx* ldsc, const GenTile& gd, int ftt, bool act, int g) {
   using TC = GenTileCfg<FL>;
   constexpr int RF = TC::RF, TF = TC::TF;
-  const GenTile gd = P.gtiles[tile];
   const float* wts = P.weights + gd.w_off;
   cpx* tl = ldsc + gd.lds;
   cpx gx[RF];
@@ -600,16 +600,17 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
   // 1024-thread workgroups have no registers to spare for the prefetch
   // (the prefetched words would be spilled, which waits for the loads).
   constexpr bool PREFETCH = T < 1024;
-  // With the L = 256 fused stage the 1024-thread workgroups load the next
-  // block's input after the edge terms, while the channel tiles run: the
-  // spectrum registers are dead by then.
-  constexpr bool PF_FUSED = !PREFETCH && FL > kFusedGen2;
+  // The 1024-thread workgroups load the next block's input late in the
+  // block, once the spectrum registers are dead: after the edge terms and
+  // before the channel tiles (L = 256 stage), after the tiles (general stage
+  // with one target per bin; with two, the registers would spill), or while
+  // the edge terms wait for their loads (no fused channels).
+  constexpr bool PF_LATE = !PREFETCH && FL != kFusedGen2;
   Word vin[R];
-  if ((PREFETCH || PF_FUSED) && lo < hi) load_block(lo, vin);
-
+  if ((PREFETCH || PF_LATE) && lo < hi) load_block(lo, vin);
 
   for (int g = lo; g < hi; ++g) {
-    if constexpr (!PREFETCH && !PF_FUSED) load_block(g, vin);
+    if constexpr (!PREFETCH && !PF_LATE) load_block(g, vin);
     // Per-block values that do not depend on g (lane maps, table reads) are
     // recomputed and reloaded every block: kept live across the loop they
     // would cost more registers than the 1024-thread workgroups have.
@@ -884,35 +885,50 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
                                                 (unsigned)(eit[i].w & 0x0FFFFFFF) * (unsigned)sizeof(float), 0, 0);
       }
     };
+    // Branch-free prefetch of the next block (the last block loads its own
+    // input once more): with a skipped load on one path the waits for the
+    // loads before it would count on the shorter path.
+    auto prefetch_next = [&]() {
+      __builtin_amdgcn_sched_barrier(0);
+      load_block(g + 1 < hi ? g + 1 : g, vin);
+      __builtin_amdgcn_sched_barrier(0);
+    };
     if constexpr (FL == 0) {
       edge_prep();
       __syncthreads();  // edge ring complete
       edge_issue();
+      if constexpr (PF_LATE) prefetch_next();
       edge_finish();
     }
 
     // General fused stage: the thread's tile slot of every round (the
     // size class is the same for every lane of a wave).
     if constexpr (fused_general(FL)) {
+      cpx* ldsc = reinterpret_cast<cpx*>(lds);
       edge_prep();
       __syncthreads();  // tile buffers and edge ring complete
       edge_issue();
       for (int r = 0; r < P.grounds; ++r) {
         const int sl = P.gslots[r * T + t];
-        const int cls = __builtin_amdgcn_readfirstlane(sl & 7);
-        const int tile = (sl >> 8) & 0x7FFFFF, ftt = (sl >> 3) & 31;
+        const GenTile gd = P.gtiles[r * T + t];
+        // waves of idle slots only (padding) skip the round: the tile code
+        // has no workgroup barrier
+        if (__ballot(sl >= 0) == 0) continue;
+        const int cls = __builtin_amdgcn_readfirstlane(sl & 7), ftt = (sl >> 3) & 31;
         const bool act = sl >= 0;
-        cpx* ldsc = reinterpret_cast<cpx*>(lds);
         switch (cls) {
-          case 0: gen_tile<8>(P, ldsc, tile, ftt, act, g); break;
-          case 1: gen_tile<16>(P, ldsc, tile, ftt, act, g); break;
-          case 2: gen_tile<32>(P, ldsc, tile, ftt, act, g); break;
-          case 3: gen_tile<64>(P, ldsc, tile, ftt, act, g); break;
-          case 4: gen_tile<128>(P, ldsc, tile, ftt, act, g); break;
-          default: gen_tile<256>(P, ldsc, tile, ftt, act, g); break;
+          case 0: gen_tile<8>(P, ldsc, gd, ftt, act, g); break;
+          case 1: gen_tile<16>(P, ldsc, gd, ftt, act, g); break;
+          case 2: gen_tile<32>(P, ldsc, gd, ftt, act, g); break;
+          case 3: gen_tile<64>(P, ldsc, gd, ftt, act, g); break;
+          case 4: gen_tile<128>(P, ldsc, gd, ftt, act, g); break;
+          default: gen_tile<256>(P, ldsc, gd, ftt, act, g); break;
         }
       }
       edge_finish();
+      // (after the tiles: their loads, issued behind a prefetch, would be
+      // waited for together with it)
+      if constexpr (PF_LATE) prefetch_next();
     }
 
     // Fused channel stage (fcfb.rs:154-188): weight the gathered bins,
@@ -938,7 +954,7 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
       static_assert(RF == FC::RF2, "one second-pass butterfly per thread");
       // (loaded before the next block's input: the loads are waited for in order)
       const PassTwiddles<FC::RF2> ftw = load_pass_twiddles<FC::RF2, FC::RF1>(ftw2, ftt & (FC::RF1 - 1));
-      if constexpr (PF_FUSED) {
+      if constexpr (PF_LATE) {
         edge_finish();
         __builtin_amdgcn_sched_barrier(0);
         // Branch-free (the last block loads its own input once more): with
@@ -971,7 +987,7 @@ __global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
           if (idx >= FL / 8 && idx < (FL / 8) * 7) out[idx - FL / 8] = gx[e];
         }
       }
-      if constexpr (!PF_FUSED) edge_finish();
+      if constexpr (!PF_LATE) edge_finish();
     }
     if (++ph == P.A) {
       ph = 0;

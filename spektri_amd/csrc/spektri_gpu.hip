@@ -605,7 +605,11 @@ spk_status fuse_general(spk_ctx* c) {
       tab[(size_t)d * M + ((size_t)(q >> 3) * T + t) * 8 + (q & 7)] = tgt[(size_t)d * (M + 1) + k];
     }
   HIP_TRY(c, upload(c->fscat, tab));
-  HIP_TRY(c, upload(c->gtiles, tiles));
+  // per slot: its tile's descriptor, loaded beside the slot rather than
+  // after it
+  std::vector<spk::GenTile> slot_tiles(slots.size());
+  for (size_t i = 0; i < slots.size(); ++i) slot_tiles[i] = tiles[(size_t)((slots[i] >> 8) & 0x7FFFFF)];
+  HIP_TRY(c, upload(c->gtiles, slot_tiles));
   HIP_TRY(c, upload(c->gslots, slots));
   for (int i = 0; i < 4; ++i)
     if (!c->gentw[i].p) HIP_TRY(c, upload(c->gentw[i], pass_table(2 << i, 16, +1)));
