@@ -544,8 +544,16 @@ __device__ __forceinline__ void gen_put(cpx* ldsc, unsigned entry, cpx X) {
 // after a barrier from the edge table.  Channels of the fused stage are
 // gathered by scattering each owned bin into an LDS tile buffer through the
 // bin -> position map.
+// Waves per SIMD the compiler must leave room for (at most 512 / n VGPRs
+// per wave): four for the 128- and 256-thread workgroups of M = 2048 and
+// 4096 with s16 input, whose fused stages otherwise take a few registers
+// more than 128 and so one workgroup per CU less than their LDS admits
+// (with the f32 loader the same bound spills); one otherwise.
+__host__ __device__ constexpr int fwd_min_waves(int M, bool f32) { return !f32 && M <= 4096 && M >= 2048 ? 4 : 1; }
+
 template <int M, int R, int R1, int R2, int FL, bool F32>
-__global__ __launch_bounds__(M / R) void fwd_real_kernel(FwdParams P) {
+__global__ __launch_bounds__(M / R) __attribute__((amdgpu_waves_per_eu(fwd_min_waves(M, F32))))
+void fwd_real_kernel(FwdParams P) {
   using C = FwdRealCfg<M, R, R1, R2, FL>;
   using FC = FusedCfg<FL>;
   constexpr int T = C::T;
