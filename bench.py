@@ -54,7 +54,9 @@ def parse():
     ap.add_argument("--workload", default="rx888_64k")
     ap.add_argument("--configs", default="all",
                     help="other workloads measured before the headline: 'all', 'none' or a comma list")
-    ap.add_argument("--config-steps", type=int, default=20, help="timed steps per other workload")
+    ap.add_argument("--config-steps", type=int, default=20, help="timed steps per other workload (at least)")
+    ap.add_argument("--config-seconds", type=float, default=0.3,
+                    help="each other workload warms up for at least this long and its timed steps fill at least as much")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU work for the headline baseline sample (0 disables every CPU leg)")
     ap.add_argument("--config-cpu-seconds", type=float, default=4.0,
@@ -195,12 +197,16 @@ def prepare_stream(wl, x, args, ncopies):
     return {"wl": wl, "st": st, "copies": copies, "tune": tune, "dev": x.device}
 
 
-def measure_stream(prep, steps, warmup, args, keep=None):
+def measure_stream(prep, steps, warmup, args, keep=None, min_seconds=0.0):
     """Time ``steps`` passes of the path over the prepared stream (after
     ``warmup``) and break the step down per kernel; returns the figures of
     one workload and releases its context and copies (or, with ``keep``,
     appends them there to be released later: freeing gigabytes takes long
-    enough for the GPU clock to drop before the next measurement)."""
+    enough for the GPU clock to drop before the next measurement).  With
+    ``min_seconds`` the warm-up also lasts at least that long and the timed
+    region takes at least as many steps as fill it (short steps would
+    otherwise be timed on a clock that has not ramped up, over a region of
+    a few milliseconds)."""
     import torch
 
     wl, st, copies, tune, dev = prep["wl"], prep["st"], prep["copies"], prep["tune"], prep["dev"]
@@ -211,10 +217,16 @@ def measure_stream(prep, steps, warmup, args, keep=None):
     st.set_stream(stream.cuda_stream)
     torch.cuda.set_stream(stream)
     ptrs, n = [c.data_ptr() for c in copies], wl.nsamples
-    for i in range(warmup):
+    w0, i = time.perf_counter(), 0
+    while i < warmup or time.perf_counter() - w0 < min_seconds:
         st.process_device(ptrs[i % ncopies], n)
+        i += 1
+        if min_seconds > 0 and i % 32 == 0:
+            torch.cuda.synchronize()  # host time follows the GPU's
     st.flush()
     torch.cuda.synchronize()
+    if min_seconds > 0:
+        steps = max(steps, int(min_seconds / ((time.perf_counter() - w0) / i)) + 1)
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     t0.record(stream)
@@ -315,7 +327,7 @@ def measure_stream(prep, steps, warmup, args, keep=None):
     roof["frac"] = round(roof["achieved"] / HBM_PEAK_GBPS, 4)
     if roof["traffic"] is not None:
         roof["traffic_from"] = f"profiles/traffic_{wl.name}.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
-    out = {"value": round(value, 1), "ms_per_step": round(ms_step, 4), "roofline": roof,
+    out = {"value": round(value, 1), "ms_per_step": round(ms_step, 4), "steps": steps, "roofline": roof,
            "path": {"algorithmic_bytes_per_step": int(path_bytes), "achieved_GBps": round(path_gbps, 1),
                     "frac": round(path_gbps / HBM_PEAK_GBPS, 4), "bytes_per_input_sample": round(b_per_sample, 3),
                     "roofline_MSps": round(HBM_PEAK_GBPS * 1e9 / b_per_sample / 1e6, 1)},
@@ -436,7 +448,7 @@ def main():
                 config_inputs[name] = xc[:64 * wl_c.hop + wl_c.fft_size].cpu().numpy()
             last = name == names[-1]
             configs[name] = measure_stream(prepare_stream(wl_c, xc, args, ncopies), args.config_steps, 10, args,
-                                           keep=kept if last else None)
+                                           keep=kept if last else None, min_seconds=args.config_seconds)
             del xc
         configs[name]["config"] = wl_c.describe()
         configs[name]["wall_s"] = round(time.perf_counter() - t0, 1)
