@@ -29,15 +29,20 @@ struct CplxRowParams {
 
 // Workgroup w serves XCD w mod 8; block g goes to XCD g mod 8, so the 64
 // row transforms of one block (which read interleaved samples of the same
-// cache lines) share one L2.  Grid: 8 * ceil(nblocks / 8) * (64 / TPW).
+// cache lines) share one L2.  A workgroup of WGT threads holds TPW = WGT / T
+// consecutive rows n2 and loads them together: element f of its share is
+// sample (n1 = f / TPW, row f mod TPW), so a wave reads runs of TPW
+// adjacent samples (a 128-byte line holds 16 consecutive n2 of one n1)
+// rather than 64 samples 512 bytes apart, and the rows are staged in LDS
+// before the first pass.  Grid: 8 * ceil(nblocks / 8) * (64 / TPW).
 // S16: a complex sample is a 32-bit pair of s16 (cs16le input, and real
 // s16le input read as x[2n] + i x[2n+1]), converted in the load.
-template <int N1, int R, int R1, int R2, int R3, bool S16>
-__global__ __launch_bounds__(kChanThreads) void cplx_row_kernel(CplxRowParams P) {
+template <int N1, int R, int R1, int R2, int R3, bool S16, int WGT = kChanThreads>
+__global__ __launch_bounds__(WGT) void cplx_row_kernel(CplxRowParams P) {
   constexpr int T = N1 / R;
   static_assert(R1 * R2 * R3 == N1, "radix plan must cover N1");
-  static_assert(T <= kChanThreads && kChanThreads % T == 0, "tile must divide the workgroup");
-  constexpr int TPW = kChanThreads / T;
+  static_assert(T <= WGT && WGT % T == 0, "tile must divide the workgroup");
+  constexpr int TPW = WGT / T;
   static_assert(64 % TPW == 0, "row tiles divide the 64 rows");
   constexpr int TILES = 64 / TPW;  // workgroups per block
   constexpr int LP = lds_padded_size(N1);
@@ -47,15 +52,17 @@ __global__ __launch_bounds__(kChanThreads) void cplx_row_kernel(CplxRowParams P)
   const int g = 8 * (r / TILES) + x;
   if (g >= P.nblocks) return;  // whole workgroup
   const int tl = threadIdx.x / T, t = threadIdx.x % T;
-  const int n2 = (r % TILES) * TPW + tl;
+  const int n20 = (r % TILES) * TPW;  // the workgroup's first row
+  const int n2 = n20 + tl;
   float* lds = lds_all + tl * LP * 2;
 
   const long long s0 = (long long)g * P.H - P.O;  // stream index of sample 0 of the block
-  cpx v[R];
+  cpx* rows = reinterpret_cast<cpx*>(lds_all);
 #pragma unroll
-  for (int e = 0; e < R; ++e) {
-    const int n1 = st_in_index<N1, R, T, R1>(t, e);
-    const long long si = s0 + 64LL * n1 + n2;
+  for (int k = 0; k < R; ++k) {
+    const int f = k * WGT + (int)threadIdx.x;
+    const int n1 = f / TPW, j = f % TPW;
+    const long long si = s0 + 64LL * n1 + n20 + j;
     cpx a;
     if constexpr (S16) {
       const int w = si < 0 ? static_cast<const int*>(P.hist)[P.O + si] : static_cast<const int*>(P.src)[si];
@@ -63,8 +70,12 @@ __global__ __launch_bounds__(kChanThreads) void cplx_row_kernel(CplxRowParams P)
     } else {
       a = si < 0 ? static_cast<const cpx*>(P.hist)[P.O + si] : static_cast<const cpx*>(P.src)[si];
     }
-    v[e] = cscale(a, P.wscale);
+    rows[j * LP + lds_pad(n1)] = cscale(a, P.wscale);
   }
+  __syncthreads();
+  cpx v[R];
+#pragma unroll
+  for (int e = 0; e < R; ++e) v[e] = reinterpret_cast<const cpx*>(lds)[lds_pad(st_in_index<N1, R, T, R1>(t, e))];
   st_butterflies<N1, R, T, R1, 1, -1>(v, t, nullptr);
   tile_exchange<T>(  // wave-local when the row's T threads lie in one wave
       v, lds, [&](int e) { return st_out_addr<N1, R, T, R1, 1>(t, e); },

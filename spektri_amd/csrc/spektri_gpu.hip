@@ -1121,12 +1121,14 @@ spk_status launch_batched(spk_ctx* c, int nb, hipStream_t s) {
 // graph; every per-call quantity the kernels need is in device memory.
 template <int N1, int R, int R1, int R2, int R3>
 void launch_cplx_rows(const spk::CplxRowParams& P, int nb, bool s16, hipStream_t s) {
-  constexpr int TPW = spk::kChanThreads / (N1 / R);
+  // at least four rows per workgroup (rows of 4096 points: 1024 threads)
+  constexpr int WGT = N1 / R >= spk::kChanThreads ? 4 * (N1 / R) : spk::kChanThreads;
+  constexpr int TPW = WGT / (N1 / R);
   const int grid = 8 * ((nb + 7) / 8) * (64 / TPW);
   if (s16)
-    hipLaunchKernelGGL((spk::cplx_row_kernel<N1, R, R1, R2, R3, true>), dim3(grid), dim3(spk::kChanThreads), 0, s, P);
+    hipLaunchKernelGGL((spk::cplx_row_kernel<N1, R, R1, R2, R3, true, WGT>), dim3(grid), dim3(WGT), 0, s, P);
   else
-    hipLaunchKernelGGL((spk::cplx_row_kernel<N1, R, R1, R2, R3, false>), dim3(grid), dim3(spk::kChanThreads), 0, s, P);
+    hipLaunchKernelGGL((spk::cplx_row_kernel<N1, R, R1, R2, R3, false, WGT>), dim3(grid), dim3(WGT), 0, s, P);
 }
 
 hipError_t dispatch_cplx_rows(int N1, const spk::CplxRowParams& P, int nb, bool s16, hipStream_t s) {
