@@ -97,6 +97,9 @@ struct FinParams {
   // positions, in welch_finalize_kernel S teams of 256 / S threads; team s
   // sums the s-th range of the group's workgroups.
   int split;
+  // Chunks per take of the forward kernel's record phase (a power of two):
+  // short averages make many small chunks, taken several at a time.
+  int take;
 };
 
 struct FwdParams {
@@ -422,7 +425,7 @@ __device__ __forceinline__ float4 welch_rows_add(const FinParams& F, int blk0_mo
                                                  float4 s) {
   const bool nyq = i0 == F.M;
   const unsigned nb = (unsigned)F.nblocks, G = (unsigned)F.G;
-#pragma unroll 2
+#pragma unroll 4
   for (int wg = wa; wg <= wb; ++wg) {
     const int lo = (int)((unsigned)wg * nb / G);
     const int slot = gi - (blk0_mod + lo) / F.A;
@@ -1011,9 +1014,10 @@ void fwd_real_kernel(FwdParams P) {
 
   // The previous call's Welch records: its partial rows are complete (they
   // were written by the previous launch), so the workgroups that finish
-  // their blocks first take them, one chunk (4 T positions of one group) at
-  // a time.  Every workgroup ends with exactly one failed take, so the take
-  // numbered chunks + G - 1 is the launch's last and resets the counter.
+  // their blocks first take them, P.fin.take chunks (of 4 T / S positions of
+  // one group) at a time.  Every workgroup ends with exactly one failed take,
+  // so the take numbered takes + G - 1 is the launch's last and resets the
+  // counter.
   if (P.fin.nblocks > 0) {
     // (its position slot is this launch's pos_next, already rewritten)
     const long long fpos = block0 - P.fin.nblocks;
@@ -1024,6 +1028,7 @@ void fwd_real_kernel(FwdParams P) {
     const int S = P.fin.split, TS = T / S;
     const int CPG = (R / 4) * S;
     const int nchunks = ngroups * CPG;
+    const int Q = P.fin.take, ntakes = (nchunks + Q - 1) / Q;
     const int blk0_mod = (int)(fpos % P.fin.A);
     const int team = (int)threadIdx.x / TS, u = (int)threadIdx.x % TS;
     float4* stage = reinterpret_cast<float4*>(lds);
@@ -1032,47 +1037,51 @@ void fwd_real_kernel(FwdParams P) {
       if (threadIdx.x == 0) *ticket_slot = atomicAdd(P.fin_ticket, 1);
       __syncthreads();
       const int ticket = *ticket_slot;
-      if (ticket >= nchunks) {
-        if (threadIdx.x == 0 && ticket == nchunks + G - 1) *P.fin_ticket = 0;
+      if (ticket >= ntakes) {
+        if (threadIdx.x == 0 && ticket == ntakes + G - 1) *P.fin_ticket = 0;
         break;
       }
-      const int gi = ticket / CPG, c = ticket % CPG;
-      const int i0 = 4 * (c * TS + u);
-      if (S == 1) {
-        welch_finalize_quad(P.fin, fpos, gi, i0);
-      } else {
-        int w0, w1;
-        welch_group_wgs(P.fin, blk0_mod, gi, &w0, &w1);
-        const int nw = w1 - w0 + 1;
-        const int wa = w0 + nw * team / S, wb = w0 + nw * (team + 1) / S - 1;
-        const float4 z = {0.f, 0.f, 0.f, 0.f};
-        stage[team * TS + u] =
-            welch_rows_add(P.fin, blk0_mod, gi, i0, wa, wb, team == 0 ? welch_carry(P.fin, blk0_mod, gi, i0) : z);
-        // the last chunk of a group also carries the Nyquist position
-        float* nstage = reinterpret_cast<float*>(stage + T);
-        const bool last = c == CPG - 1;
-        if (last && u == 0)
-          nstage[team] =
-              welch_rows_add(P.fin, blk0_mod, gi, M, wa, wb, team == 0 ? welch_carry(P.fin, blk0_mod, gi, M) : z).x;
-        __syncthreads();
-        if (team == 0) {
-          float4 acc = stage[u];
-          for (int k = 1; k < S; ++k) {
-            const float4 v = stage[k * TS + u];
-            acc.x += v.x;
-            acc.y += v.y;
-            acc.z += v.z;
-            acc.w += v.w;
-          }
-          welch_emit(P.fin, blk0_mod, gi, i0, acc);
-          if (last && u == 0) {
-            float n = nstage[0];
-            for (int k = 1; k < S; ++k) n += nstage[k];
-            welch_emit(P.fin, blk0_mod, gi, M, float4{n, 0.f, 0.f, 0.f});
+      const int cend = min(nchunks, (ticket + 1) * Q);
+      for (int ch = ticket * Q; ch < cend; ++ch) {
+        if (S > 1 && ch > ticket * Q) __syncthreads();  // the previous chunk's stage has been read
+        const int gi = ch / CPG, c = ch % CPG;
+        const int i0 = 4 * (c * TS + u);
+        if (S == 1) {
+          welch_finalize_quad(P.fin, fpos, gi, i0);
+        } else {
+          int w0, w1;
+          welch_group_wgs(P.fin, blk0_mod, gi, &w0, &w1);
+          const int nw = w1 - w0 + 1;
+          const int wa = w0 + nw * team / S, wb = w0 + nw * (team + 1) / S - 1;
+          const float4 z = {0.f, 0.f, 0.f, 0.f};
+          stage[team * TS + u] =
+              welch_rows_add(P.fin, blk0_mod, gi, i0, wa, wb, team == 0 ? welch_carry(P.fin, blk0_mod, gi, i0) : z);
+          // the last chunk of a group also carries the Nyquist position
+          float* nstage = reinterpret_cast<float*>(stage + T);
+          const bool last = c == CPG - 1;
+          if (last && u == 0)
+            nstage[team] =
+                welch_rows_add(P.fin, blk0_mod, gi, M, wa, wb, team == 0 ? welch_carry(P.fin, blk0_mod, gi, M) : z).x;
+          __syncthreads();
+          if (team == 0) {
+            float4 acc = stage[u];
+            for (int k = 1; k < S; ++k) {
+              const float4 v = stage[k * TS + u];
+              acc.x += v.x;
+              acc.y += v.y;
+              acc.z += v.z;
+              acc.w += v.w;
+            }
+            welch_emit(P.fin, blk0_mod, gi, i0, acc);
+            if (last && u == 0) {
+              float n = nstage[0];
+              for (int k = 1; k < S; ++k) n += nstage[k];
+              welch_emit(P.fin, blk0_mod, gi, M, float4{n, 0.f, 0.f, 0.f});
+            }
           }
         }
+        if (S == 1 && c == CPG - 1 && threadIdx.x == 0) welch_finalize_quad(P.fin, fpos, gi, M);
       }
-      if (S == 1 && c == CPG - 1 && threadIdx.x == 0) welch_finalize_quad(P.fin, fpos, gi, M);
     }
   }
 }

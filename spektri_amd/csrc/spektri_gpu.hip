@@ -1437,8 +1437,14 @@ spk::FinParams fin_params(const spk_ctx* c, int nb, int G, int parity) {
   const int ngroups = (c->A - 1 + nb - 1) / c->A + 1;
   const int chunks = ngroups * (c->fplan->R / 4);
   int split = 1;
-  while (split < 16 && 2 * split * chunks <= G && T / (2 * split) >= 64) split *= 2;
+  while (split < 16 && 2 * split * chunks <= G && T / (2 * split) >= 16) split *= 2;
   F.split = split;
+  // Chunks per take: short averages make many chunks of few partial rows
+  // each; take several at a time, keeping at least four takes per workgroup.
+  const long long nchunks = (long long)chunks * split;
+  int take = 1;
+  while (take < 16 && nchunks >= 8LL * take * G) take *= 2;
+  F.take = take;
   return F;
 }
 
