@@ -167,14 +167,30 @@ struct WelchDirectParams {
   const float* carry_in;
   float* carry_out;
   float* out;
-  const int* hist_src;
-  int* hist_dst;
+  // input history for the next call: the last hist_words 32-bit words of
+  // this call's input, or (hist_shorts != 0: s16 samples with an odd
+  // overlap or offset) its last hist_shorts 16-bit samples
+  const void* hist_src;
+  void* hist_dst;
   int hist_words;
+  int hist_shorts;
   int* chan_counters;
   int ncounters;
   unsigned char* qout;
   int qfmt;
 };
+
+__device__ __forceinline__ void copy_history(const WelchDirectParams& F, int i0, int stride) {
+  if (F.hist_shorts) {
+    const unsigned short* src = static_cast<const unsigned short*>(F.hist_src);
+    unsigned short* dst = static_cast<unsigned short*>(F.hist_dst);
+    for (int i = i0; i < F.hist_shorts; i += stride) dst[i] = src[i];
+  } else {
+    const int* src = static_cast<const int*>(F.hist_src);
+    int* dst = static_cast<int*>(F.hist_dst);
+    for (int i = i0; i < F.hist_words; i += stride) dst[i] = src[i];
+  }
+}
 
 // Real input outside the forward kernel's plans goes through the
 // complex pipeline: the complex FFT Z of the M = N/2 packed pairs
@@ -221,7 +237,7 @@ __global__ void welch_direct_real_kernel(WelchDirectParams F) {
   const int k0 = 4 * tid;
   const long long pos = *F.pos;
   if (gi == 0) {
-    for (int i = tid; i < F.hist_words; i += (int)(gridDim.x * blockDim.x)) F.hist_dst[i] = F.hist_src[i];
+    copy_history(F, tid, (int)(gridDim.x * blockDim.x));
     if (tid < F.ncounters) F.chan_counters[tid] = 0;
     if (tid == 0) *F.pos_next = pos + F.nblocks;
   }
@@ -288,7 +304,7 @@ __global__ void welch_direct_kernel(WelchDirectParams F) {
   const int k0 = 4 * tid;
   const long long pos = *F.pos;
   if (gi == 0) {
-    for (int i = tid; i < F.hist_words; i += (int)(gridDim.x * blockDim.x)) F.hist_dst[i] = F.hist_src[i];
+    copy_history(F, tid, (int)(gridDim.x * blockDim.x));
     if (tid < F.ncounters) F.chan_counters[tid] = 0;
     if (tid == 0) *F.pos_next = pos + F.nblocks;
   }

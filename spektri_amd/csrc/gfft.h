@@ -29,7 +29,9 @@ struct GfftPassParams {
   long long in_stride;
   const void* src;         // first pass: this call's samples (pairs)
   const void* hist;        // first pass: the previous call's last Oc pairs
-  int Hc, Oc;              // hop and overlap in pairs
+  int Hc, Oc;              // hop and overlap in pairs (in samples when samples != 0)
+  int samples;             // real input whose hop and overlap are odd (N = 4 mod 8):
+                           // a pair is two samples read one by one
   float wscale;            // window s / N (dsp.rs:70-73)
   cpx* out;
   long long out_stride;
@@ -40,10 +42,23 @@ struct GfftPassParams {
 
 // LOAD: 0 = rows of the previous pass, 1 = s16 pairs, 2 = f32 pairs.
 template <int LOAD>
+__device__ __forceinline__ float gfft_sample(const GfftPassParams& P, long long s) {
+  const bool from_hist = s < 0;
+  const long long idx = from_hist ? s + P.Oc : s;
+  if constexpr (LOAD == 1)
+    return (float)(from_hist ? static_cast<const short*>(P.hist) : static_cast<const short*>(P.src))[idx];
+  else
+    return (from_hist ? static_cast<const float*>(P.hist) : static_cast<const float*>(P.src))[idx];
+}
+template <int LOAD>
 __device__ __forceinline__ cpx gfft_load(const GfftPassParams& P, int b, int n) {
   if constexpr (LOAD == 0) {
     return P.in[(long long)b * P.in_stride + n];
   } else {
+    if (P.samples) {  // main.rs:173-181 framing in samples; a pair may straddle the history
+      const long long s = (long long)b * P.Hc - P.Oc + 2LL * n;
+      return cpx{gfft_sample<LOAD>(P, s) * P.wscale, gfft_sample<LOAD>(P, s + 1) * P.wscale};
+    }
     const long long g = (long long)b * P.Hc - P.Oc + n;  // main.rs:173-181 framing, in pairs
     const bool from_hist = g < 0;
     const long long idx = from_hist ? g + P.Oc : g;
@@ -120,6 +135,7 @@ struct Gfft4Params {
   const void* src;         // A: this call's samples (pairs)
   const void* hist;        // A: the previous call's last Oc pairs
   int Hc, Oc;
+  int samples;             // as GfftPassParams::samples
   float wscale;
   const cpx* Y;            // B: column results
   cpx* out;                // A: Y rows (stride Nc); B: spectrum rows
@@ -238,6 +254,7 @@ __global__ __launch_bounds__(kG4Threads) void gfft4_kernel(Gfft4Params P) {
   Q.hist = P.hist;
   Q.Hc = P.Hc;
   Q.Oc = P.Oc;
+  Q.samples = P.samples;
   Q.wscale = P.wscale;
   cpx vals[UL];
 #pragma unroll

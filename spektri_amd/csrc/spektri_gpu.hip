@@ -1216,12 +1216,16 @@ bool gfft4_split(int Nc, int* N1, int* N2) {
 // last writes the spectrum rows.
 spk_status enqueue_gfft(spk_ctx* c, const void* src, int nb, hipStream_t s) {
   const int pk = c->complex_in ? 1 : 2;
+  // real input with an odd hop and overlap (N = 4 mod 8): framed in samples
+  const int samples = !c->complex_in && (c->H % 2 || c->O % 2) ? 1 : 0;
+  const int Hc = samples ? c->H : c->H / pk, Oc = samples ? c->O : c->O / pk;
   if (c->gfft4) {
     spk::Gfft4Params P{};
     P.src = src;
     P.hist = c->hist.p;
-    P.Hc = c->H / pk;
-    P.Oc = c->O / pk;
+    P.Hc = Hc;
+    P.Oc = Oc;
+    P.samples = samples;
     P.wscale = c->wscale;
     P.tw_full = c->gtw.as<cpx>();
     P.Nc = c->Nc;
@@ -1271,8 +1275,9 @@ spk_status enqueue_gfft(spk_ctx* c, const void* src, int nb, hipStream_t s) {
   spk::GfftPassParams P{};
   P.src = src;
   P.hist = c->hist.p;
-  P.Hc = c->H / pk;
-  P.Oc = c->O / pk;
+  P.Hc = Hc;
+  P.Oc = Oc;
+  P.samples = samples;
   P.wscale = c->wscale;
   P.tw = c->gtw.as<cpx>();
   P.Nc = c->Nc;
@@ -1357,10 +1362,12 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
     W.carry_in = c->carry.as<float>() + (long long)c->parity * c->spec_stride;
     W.carry_out = c->carry.as<float>() + (long long)(c->parity ^ 1) * c->spec_stride;
     W.out = c->welch_out.as<float>();
-    W.hist_src = reinterpret_cast<const int*>(reinterpret_cast<const char*>(src) +
-                                              ((long long)nb * c->H - c->O) * c->bps);
-    W.hist_dst = c->hist.as<int>();
+    const long long hist_off = ((long long)nb * c->H - c->O) * c->bps;
+    W.hist_src = reinterpret_cast<const char*>(src) + hist_off;
+    W.hist_dst = c->hist.p;
     W.hist_words = c->O * c->bps / 4;
+    // (s16 samples with an odd overlap or start: copied 16 bits at a time)
+    W.hist_shorts = (c->O * c->bps) % 4 != 0 || hist_off % 4 != 0 ? c->O * c->bps / 2 : 0;
     W.chan_counters = c->chan_counters.as<int>();
     W.ncounters = kXcdQueues;
     W.qout = c->qout.as<unsigned char>();
@@ -1669,13 +1676,14 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   // real N beyond the forward kernel's plans: the complex pipeline on N/2 pairs
   const bool big_real = !cplx && pow2 && find_fwd_plan((int)N / 2) == nullptr && N > 65536 && N <= 524288;
   // every other size the reference accepts (rustfft plans any N) goes
-  // through the mixed-radix passes: N a multiple of 8 (whole sample pairs
-  // in the hop and the overlap), at most 2^20, and the transform length
+  // through the mixed-radix passes: at least 16 and at most 2^20, real N a
+  // multiple of 4 (N/2 sample pairs; with N = 4 mod 8 the hop and overlap
+  // are odd and the blocks are framed in samples), and the transform length
   // without a prime factor above 31
   const bool planned = pow2 && (cplx ? (N >= 1024 && N <= 262144) : (big_real || find_fwd_plan((int)N / 2) != nullptr));
   if (!planned) {
     const int Nc = cplx ? (int)N : (int)N / 2;
-    if (N % 8 != 0 || N < 16 || N > (1u << 20) || gfft_radices(Nc).empty()) {
+    if ((!cplx && N % 4 != 0) || N < 16 || N > (1u << 20) || gfft_radices(Nc).empty()) {
       delete c;
       return SPK_ERR_UNSUPPORTED;
     }

@@ -35,9 +35,10 @@ def test_null_and_invalid_arguments_return_codes():
     out = ctypes.c_void_p()
     cfg = _lib.SpkConfig(0, _lib.FMT_S16LE, 0.0, 8, 16, 0)  # no FFT size
     assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 1
-    # sizes rustfft plans but this library does not: N not a multiple of 8,
-    # a transform length with a prime factor above 31 (real N = 296: 148 =
-    # 4 * 37 pairs; complex N = 8 * 37), N above 2^20
+    # sizes rustfft plans but this library does not: a transform length
+    # with a prime factor above 31 (real N = 1004: 502 = 2 * 251 pairs,
+    # complex 1004 = 4 * 251; real N = 296: 148 = 4 * 37 pairs; complex
+    # N = 8 * 37), N above 2^20
     for N, fmt in [(1004, _lib.FMT_S16LE), (1000 + 4, _lib.FMT_CF32LE), (296, _lib.FMT_S16LE),
                    (8 * 37, _lib.FMT_CF32LE), (1 << 21, _lib.FMT_U8), (1 << 21, _lib.FMT_CF32LE)]:
         cfg = _lib.SpkConfig(N, fmt, 0.0, 8, 16, 0)
@@ -106,19 +107,21 @@ def _largest_prime(n: int) -> int:
 def test_fft_size_acceptance_rule(cplx):
     """spk_create accepts (and then, without a GPU, fails at the first HIP
     call with SPK_ERR_DEVICE) exactly the sizes it documents: powers of two
-    in the planned ranges, and every other multiple of 8 from 16 to 2^20
-    whose transform length (N/2 real, N complex) has no prime factor above
-    31; every other reference-valid size is SPK_ERR_UNSUPPORTED."""
+    in the planned ranges, and every other size from 16 to 2^20 (real: a
+    multiple of 4) whose transform length (N/2 real, N complex) has no prime
+    factor above 31; every other reference-valid size is
+    SPK_ERR_UNSUPPORTED."""
     lib = _lib.load()
     out = ctypes.c_void_p()
     fmt = _lib.FMT_CF32LE if cplx else _lib.FMT_S16LE
-    sizes = list(range(4, 4100, 4)) + [48000, 61440, 96000, 100000, 65536 * 3, 2 ** 19, 2 ** 20, 2 ** 20 + 8,
-                                       2 ** 21, 8 * 37 * 41, 8 * 31 * 31 * 31]
+    sizes = list(range(4, 4100, 4)) + list(range(5, 1200, 7)) + list(range(6, 1200, 8)) + [
+        48000, 61440, 96000, 100000, 65536 * 3, 2 ** 19, 2 ** 20, 2 ** 20 + 8, 2 ** 20 - 1, 2 ** 21, 8 * 37 * 41,
+        8 * 31 * 31 * 31, 3 * 5 * 7 * 11 * 13, 2 * 3 ** 9]
     for N in sizes:
         Nc = N if cplx else N // 2
         pow2 = N & (N - 1) == 0
         planned = pow2 and ((1024 <= N <= 262144) if cplx else (1024 <= N <= 524288))
-        mixed = N % 8 == 0 and 16 <= N <= 2 ** 20 and _largest_prime(Nc) <= 31
+        mixed = (cplx or N % 4 == 0) and 16 <= N <= 2 ** 20 and _largest_prime(Nc) <= 31
         want = 3 if (planned or mixed) else 2
         cfg = _lib.SpkConfig(N, fmt, 0.0, 8, 16, 0)
         assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == want, (N, cplx)

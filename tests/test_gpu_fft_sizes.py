@@ -40,6 +40,15 @@ CASES = [
     ("cf32le", 96000),    # four-step lines above 256: 300 x 320
     ("cs16le", 6000),     # s16 pairs read in place: 8 2 3 5 5 5
     ("cs8", 2 * 17 * 19 * 8),  # converted complex format, 8 2 17 19
+    # N = 4 mod 8 real: odd hop and overlap, blocks framed in samples (pairs
+    # straddle the history), s16 read in place and a converted format
+    ("s16le", 1020),      # real, Nc 510: 2 3 5 17
+    ("u8", 4620),         # real, Nc 2310: 2 3 5 7 11
+    # complex sizes that are not multiples of 8
+    ("cf32le", 1020),     # 4 3 5 17
+    ("cf32le", 1014),     # 2 3 13 13
+    ("cf32le", 1001),     # odd: 7 11 13
+    ("cs16le", 1020),     # s16 pairs read in place
 ]
 
 FORMATS = {"s16le": O.FMT_S16LE, "u8": O.FMT_U8, "f32le": O.FMT_F32LE, "cf32le": O.FMT_CF32LE,
