@@ -60,7 +60,7 @@ typedef enum spk_status {
 typedef struct spk_config {
   uint32_t fft_size;          /* N: powers of two 1024..524288 (real input) and 1024..262144
                                  (complex input) run in their planned kernels; any other N
-                                 from 16 to 2^20 (real input: a multiple of 4) whose
+                                 from 16 to 2^20 (real input: even) whose
                                  transform length (N/2 real, N complex) has no prime factor
                                  above 31 runs in mixed-radix passes; other sizes return
                                  SPK_ERR_UNSUPPORTED */

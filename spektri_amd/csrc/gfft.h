@@ -30,8 +30,8 @@ struct GfftPassParams {
   const void* src;         // first pass: this call's samples (pairs)
   const void* hist;        // first pass: the previous call's last Oc pairs
   int Hc, Oc;              // hop and overlap in pairs (in samples when samples != 0)
-  int samples;             // real input whose hop and overlap are odd (N = 4 mod 8):
-                           // a pair is two samples read one by one
+  int samples;             // real input whose hop or overlap is odd (N not a multiple
+                           // of 8): a pair is two samples read one by one
   float wscale;            // window s / N (dsp.rs:70-73)
   cpx* out;
   long long out_stride;

@@ -1216,7 +1216,7 @@ bool gfft4_split(int Nc, int* N1, int* N2) {
 // last writes the spectrum rows.
 spk_status enqueue_gfft(spk_ctx* c, const void* src, int nb, hipStream_t s) {
   const int pk = c->complex_in ? 1 : 2;
-  // real input with an odd hop and overlap (N = 4 mod 8): framed in samples
+  // real input with an odd hop or overlap (N not a multiple of 8): framed in samples
   const int samples = !c->complex_in && (c->H % 2 || c->O % 2) ? 1 : 0;
   const int Hc = samples ? c->H : c->H / pk, Oc = samples ? c->O : c->O / pk;
   if (c->gfft4) {
@@ -1676,14 +1676,14 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   // real N beyond the forward kernel's plans: the complex pipeline on N/2 pairs
   const bool big_real = !cplx && pow2 && find_fwd_plan((int)N / 2) == nullptr && N > 65536 && N <= 524288;
   // every other size the reference accepts (rustfft plans any N) goes
-  // through the mixed-radix passes: at least 16 and at most 2^20, real N a
-  // multiple of 4 (N/2 sample pairs; with N = 4 mod 8 the hop and overlap
-  // are odd and the blocks are framed in samples), and the transform length
-  // without a prime factor above 31
+  // through the mixed-radix passes: at least 16 and at most 2^20, real N
+  // even (N/2 sample pairs; when the hop or the overlap is odd the blocks
+  // are framed in samples), and the transform length without a prime
+  // factor above 31
   const bool planned = pow2 && (cplx ? (N >= 1024 && N <= 262144) : (big_real || find_fwd_plan((int)N / 2) != nullptr));
   if (!planned) {
     const int Nc = cplx ? (int)N : (int)N / 2;
-    if ((!cplx && N % 4 != 0) || N < 16 || N > (1u << 20) || gfft_radices(Nc).empty()) {
+    if ((!cplx && N % 2 != 0) || N < 16 || N > (1u << 20) || gfft_radices(Nc).empty()) {
       delete c;
       return SPK_ERR_UNSUPPORTED;
     }

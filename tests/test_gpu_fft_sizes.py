@@ -40,10 +40,13 @@ CASES = [
     ("cf32le", 96000),    # four-step lines above 256: 300 x 320
     ("cs16le", 6000),     # s16 pairs read in place: 8 2 3 5 5 5
     ("cs8", 2 * 17 * 19 * 8),  # converted complex format, 8 2 17 19
-    # N = 4 mod 8 real: odd hop and overlap, blocks framed in samples (pairs
-    # straddle the history), s16 read in place and a converted format
+    # real N not a multiple of 8: odd hop or overlap, blocks framed in
+    # samples (pairs straddle the history), s16 read in place and converted
+    # formats; N = 2 mod 4 has an odd number of pairs
     ("s16le", 1020),      # real, Nc 510: 2 3 5 17
     ("u8", 4620),         # real, Nc 2310: 2 3 5 7 11
+    ("s16le", 2310),      # real N = 2 mod 4, odd Nc 1155: 3 5 7 11
+    ("f32le", 1050),      # real N = 2 mod 4, odd Nc 525: 3 5 5 7
     # complex sizes that are not multiples of 8
     ("cf32le", 1020),     # 4 3 5 17
     ("cf32le", 1014),     # 2 3 13 13
