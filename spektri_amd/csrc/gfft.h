@@ -30,8 +30,9 @@ struct GfftPassParams {
   const void* src;         // first pass: this call's samples (pairs)
   const void* hist;        // first pass: the previous call's last Oc pairs
   int Hc, Oc;              // hop and overlap in pairs (in samples when samples != 0)
-  int samples;             // real input whose hop or overlap is odd (N not a multiple
-                           // of 8): a pair is two samples read one by one
+  int samples;             // real input framed in samples: 1 = a pair is two samples
+                           // read one by one (hop or overlap odd), 2 = one sample per
+                           // point, imaginary part 0 (odd N)
   float wscale;            // window s / N (dsp.rs:70-73)
   cpx* out;
   long long out_stride;
@@ -56,7 +57,8 @@ __device__ __forceinline__ cpx gfft_load(const GfftPassParams& P, int b, int n) 
     return P.in[(long long)b * P.in_stride + n];
   } else {
     if (P.samples) {  // main.rs:173-181 framing in samples; a pair may straddle the history
-      const long long s = (long long)b * P.Hc - P.Oc + 2LL * n;
+      const long long s = (long long)b * P.Hc - P.Oc + (P.samples == 2 ? n : 2LL * n);
+      if (P.samples == 2) return cpx{gfft_sample<LOAD>(P, s) * P.wscale, 0.f};
       return cpx{gfft_sample<LOAD>(P, s) * P.wscale, gfft_sample<LOAD>(P, s + 1) * P.wscale};
     }
     const long long g = (long long)b * P.Hc - P.Oc + n;  // main.rs:173-181 framing, in pairs

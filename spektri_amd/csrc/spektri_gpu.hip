@@ -376,6 +376,7 @@ struct spk_ctx {
   // sizes without a two-step plan: mixed-radix passes (gfft.h) over the
   // Nc-point transform, radices in pass order, twiddles W_Nc^m
   bool gfft = false;
+  bool real_odd = false;     // odd real N: an N-point complex transform of (x, 0), no untangle
   std::vector<int> gfft_radices;
   DevBuf gtw;
   // four-step form (gfft4_kernel): Nc = N1 N2, stage radices per line length,
@@ -1216,8 +1217,9 @@ bool gfft4_split(int Nc, int* N1, int* N2) {
 // last writes the spectrum rows.
 spk_status enqueue_gfft(spk_ctx* c, const void* src, int nb, hipStream_t s) {
   const int pk = c->complex_in ? 1 : 2;
-  // real input with an odd hop or overlap (N not a multiple of 8): framed in samples
-  const int samples = !c->complex_in && (c->H % 2 || c->O % 2) ? 1 : 0;
+  // real input with an odd hop or overlap (N not a multiple of 8): framed in
+  // samples, two per pair (1), or one per point for odd N (2)
+  const int samples = c->real_odd ? 2 : (!c->complex_in && (c->H % 2 || c->O % 2) ? 1 : 0);
   const int Hc = samples ? c->H : c->H / pk, Oc = samples ? c->O : c->O / pk;
   if (c->gfft4) {
     spk::Gfft4Params P{};
@@ -1334,7 +1336,7 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
                      0, s, Q);
   HIP_TRY(c, hipGetLastError());
   }
-  if (!c->complex_in) {
+  if (!c->complex_in && !c->real_odd) {
     spk::UntangleParams U{};
     U.spec = c->spec.as<cpx>();
     U.spec_stride = c->spec_stride;
@@ -1676,14 +1678,15 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   // real N beyond the forward kernel's plans: the complex pipeline on N/2 pairs
   const bool big_real = !cplx && pow2 && find_fwd_plan((int)N / 2) == nullptr && N > 65536 && N <= 524288;
   // every other size the reference accepts (rustfft plans any N) goes
-  // through the mixed-radix passes: at least 16 and at most 2^20, real N
-  // even (N/2 sample pairs; when the hop or the overlap is odd the blocks
-  // are framed in samples), and the transform length without a prime
-  // factor above 31
+  // through the mixed-radix passes: at least 16 and at most 2^20, and the
+  // transform length without a prime factor above 31.  Even real N runs as
+  // N/2 sample pairs (framed in samples when the hop or the overlap is
+  // odd), odd real N as an N-point complex transform of the samples.
   const bool planned = pow2 && (cplx ? (N >= 1024 && N <= 262144) : (big_real || find_fwd_plan((int)N / 2) != nullptr));
+  const bool real_odd = !cplx && N % 2 != 0;
   if (!planned) {
-    const int Nc = cplx ? (int)N : (int)N / 2;
-    if ((!cplx && N % 2 != 0) || N < 16 || N > (1u << 20) || gfft_radices(Nc).empty()) {
+    const int Nc = cplx || real_odd ? (int)N : (int)N / 2;
+    if (N < 16 || N > (1u << 20) || gfft_radices(Nc).empty()) {
       delete c;
       return SPK_ERR_UNSUPPORTED;
     }
@@ -1703,7 +1706,8 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   c->conv_kind = fi->conv;
   c->f32_in = !cplx && fi->canon_bps == 4;
   c->two_step = cplx || big_real || c->gfft;
-  c->Nc = cplx ? c->N : c->N / 2;
+  c->real_odd = real_odd;
+  c->Nc = cplx || real_odd ? c->N : c->N / 2;
   // the row transforms convert s16 pairs as they load them
   if (cfg->input_format == SPK_FMT_CS16LE) c->s16_pairs = true;
   if (big_real || (c->gfft && !cplx)) {
@@ -1748,7 +1752,8 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
     return cleanup(SPK_ERR_DEVICE);
   c->ncu = ncu;
   c->noalloc_scratch = scratch_address(dev);
-  c->spec_stride = ((long long)c->K + 63) / 64 * 64;
+  // (rows hold the K bins, or the whole transform where it is longer: odd real N)
+  c->spec_stride = ((long long)std::max(c->K, c->Nc) + 63) / 64 * 64;
   c->welch_out_cap = c->max_blocks / c->A + 1;
   if (c->chan_counters.alloc(kXcdQueues * sizeof(int)) != hipSuccess ||
       hipMemset(c->chan_counters.p, 0, c->chan_counters.bytes) != hipSuccess ||

@@ -107,9 +107,9 @@ def _largest_prime(n: int) -> int:
 def test_fft_size_acceptance_rule(cplx):
     """spk_create accepts (and then, without a GPU, fails at the first HIP
     call with SPK_ERR_DEVICE) exactly the sizes it documents: powers of two
-    in the planned ranges, and every other size from 16 to 2^20 (real: even)
-    whose transform length (N/2 real, N complex) has no prime factor above
-    31; every other reference-valid size is SPK_ERR_UNSUPPORTED."""
+    in the planned ranges, and every other size from 16 to 2^20 whose
+    transform length (N/2 for even real N, N otherwise) has no prime factor
+    above 31; every other reference-valid size is SPK_ERR_UNSUPPORTED."""
     lib = _lib.load()
     out = ctypes.c_void_p()
     fmt = _lib.FMT_CF32LE if cplx else _lib.FMT_S16LE
@@ -117,10 +117,10 @@ def test_fft_size_acceptance_rule(cplx):
         48000, 61440, 96000, 100000, 65536 * 3, 2 ** 19, 2 ** 20, 2 ** 20 + 8, 2 ** 20 - 1, 2 ** 21, 8 * 37 * 41,
         8 * 31 * 31 * 31, 3 * 5 * 7 * 11 * 13, 2 * 3 ** 9]
     for N in sizes:
-        Nc = N if cplx else N // 2
+        Nc = N if cplx or N % 2 else N // 2
         pow2 = N & (N - 1) == 0
         planned = pow2 and ((1024 <= N <= 262144) if cplx else (1024 <= N <= 524288))
-        mixed = (cplx or N % 2 == 0) and 16 <= N <= 2 ** 20 and _largest_prime(Nc) <= 31
+        mixed = 16 <= N <= 2 ** 20 and _largest_prime(Nc) <= 31
         want = 3 if (planned or mixed) else 2
         cfg = _lib.SpkConfig(N, fmt, 0.0, 8, 16, 0)
         assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == want, (N, cplx)

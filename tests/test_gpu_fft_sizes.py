@@ -47,6 +47,9 @@ CASES = [
     ("u8", 4620),         # real, Nc 2310: 2 3 5 7 11
     ("s16le", 2310),      # real N = 2 mod 4, odd Nc 1155: 3 5 7 11
     ("f32le", 1050),      # real N = 2 mod 4, odd Nc 525: 3 5 5 7
+    # odd real N: an N-point complex transform of the samples, no untangle
+    ("s16le", 1155),      # 3 5 7 11
+    ("u8", 1001),         # 7 11 13
     # complex sizes that are not multiples of 8
     ("cf32le", 1020),     # 4 3 5 17
     ("cf32le", 1014),     # 2 3 13 13
