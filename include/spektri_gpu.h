@@ -61,9 +61,9 @@ typedef struct spk_config {
   uint32_t fft_size;          /* N: powers of two 1024..524288 (real input) and 1024..262144
                                  (complex input) run in their planned kernels; any other N
                                  from 16 to 2^20 whose transform length (N/2 for even real
-                                 N, N otherwise) has no prime factor above 31 runs in
-                                 mixed-radix passes; other sizes return
-                                 SPK_ERR_UNSUPPORTED */
+                                 N, N otherwise) has no prime factor above 31, or splits
+                                 into two factors of at most 480, runs in mixed-radix
+                                 passes; other sizes return SPK_ERR_UNSUPPORTED */
   uint32_t input_format;      /* SPK_FMT_*: real or complex input follows the format (main.rs:35) */
   float scaling;              /* input scaling; 0 = input_format_scaling(fmt) (inputformats.rs:47-62) */
   uint32_t ffts_per_buf;      /* F: blocks per spk_process call */

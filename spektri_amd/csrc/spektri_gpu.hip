@@ -1678,15 +1678,23 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
   // real N beyond the forward kernel's plans: the complex pipeline on N/2 pairs
   const bool big_real = !cplx && pow2 && find_fwd_plan((int)N / 2) == nullptr && N > 65536 && N <= 524288;
   // every other size the reference accepts (rustfft plans any N) goes
-  // through the mixed-radix passes: at least 16 and at most 2^20, and the
-  // transform length without a prime factor above 31.  Even real N runs as
-  // N/2 sample pairs (framed in samples when the hop or the overlap is
-  // odd), odd real N as an N-point complex transform of the samples.
+  // through the mixed-radix passes: at least 16 and at most 2^20, and a
+  // transform length without a prime factor above 31 (per-radix passes) or
+  // one that splits into two factors of at most 480 (the four-step form,
+  // whose lines sum any radix directly).  Even real N runs as N/2 sample
+  // pairs (framed in samples when the hop or the overlap is odd), odd real
+  // N as an N-point complex transform of the samples.
   const bool planned = pow2 && (cplx ? (N >= 1024 && N <= 262144) : (big_real || find_fwd_plan((int)N / 2) != nullptr));
   const bool real_odd = !cplx && N % 2 != 0;
   if (!planned) {
     const int Nc = cplx || real_odd ? (int)N : (int)N / 2;
-    if (N < 16 || N > (1u << 20) || gfft_radices(Nc).empty()) {
+    bool ok = N >= 16 && N <= (1u << 20) && !gfft_radices(Nc).empty();
+    if (!ok && N >= 16 && N <= (1u << 20)) {
+      int a = 0, b = 0;
+      ok = gfft4_split(Nc, &a, &b) && (int)generic_radices(a).size() <= spk::kG4MaxStages &&
+           (int)generic_radices(b).size() <= spk::kG4MaxStages;
+    }
+    if (!ok) {
       delete c;
       return SPK_ERR_UNSUPPORTED;
     }

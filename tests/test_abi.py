@@ -1,6 +1,7 @@
 """CPU checks of the C ABI: the library loads, exports every symbol the
 header declares, and its host-only helpers match the oracle."""
 import ctypes
+import math
 import os
 import re
 
@@ -36,11 +37,11 @@ def test_null_and_invalid_arguments_return_codes():
     cfg = _lib.SpkConfig(0, _lib.FMT_S16LE, 0.0, 8, 16, 0)  # no FFT size
     assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 1
     # sizes rustfft plans but this library does not: a transform length
-    # with a prime factor above 31 (real N = 1004: 502 = 2 * 251 pairs,
-    # complex 1004 = 4 * 251; real N = 296: 148 = 4 * 37 pairs; complex
-    # N = 8 * 37), N above 2^20
-    for N, fmt in [(1004, _lib.FMT_S16LE), (1000 + 4, _lib.FMT_CF32LE), (296, _lib.FMT_S16LE),
-                   (8 * 37, _lib.FMT_CF32LE), (1 << 21, _lib.FMT_U8), (1 << 21, _lib.FMT_CF32LE)]:
+    # with a prime factor above 31 that does not split into two factors of
+    # at most 480 (real N = 2012: 1006 = 2 * 503 pairs, complex 1006; real
+    # N = 1994: 997 pairs, a prime), N above 2^20
+    for N, fmt in [(2012, _lib.FMT_S16LE), (1006, _lib.FMT_CF32LE), (1994, _lib.FMT_S16LE),
+                   (2 * 997, _lib.FMT_CF32LE), (1 << 21, _lib.FMT_U8), (1 << 21, _lib.FMT_CF32LE)]:
         cfg = _lib.SpkConfig(N, fmt, 0.0, 8, 16, 0)
         assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == 2, (N, fmt)
     cfg = _lib.SpkConfig(8192, 0x99, 0.0, 8, 16, 0)  # not a DataFormat code
@@ -103,13 +104,39 @@ def _largest_prime(n: int) -> int:
     return max(big, n) if n > 1 else big
 
 
+def _line_stages(L):
+    """Number of stages of a four-step line of L points: fours, a two, then
+    every odd prime factor (the library's radix order)."""
+    n = 0
+    while L % 4 == 0:
+        n, L = n + 1, L // 4
+    if L % 2 == 0:
+        n, L = n + 1, L // 2
+    p = 3
+    while L > 1:
+        while L % p == 0:
+            n, L = n + 1, L // p
+        p += 2
+    return n
+
+
+def _four_step(Nc):
+    """The four-step split Nc = N1 N2 (largest N1 <= sqrt(Nc)) with both
+    lines of at most 480 points and 12 stages."""
+    for d in range(math.isqrt(Nc), 1, -1):
+        if Nc % d == 0 and d <= 480 and Nc // d <= 480:
+            return _line_stages(d) <= 12 and _line_stages(Nc // d) <= 12
+    return False
+
+
 @pytest.mark.parametrize("cplx", [False, True])
 def test_fft_size_acceptance_rule(cplx):
     """spk_create accepts (and then, without a GPU, fails at the first HIP
     call with SPK_ERR_DEVICE) exactly the sizes it documents: powers of two
     in the planned ranges, and every other size from 16 to 2^20 whose
     transform length (N/2 for even real N, N otherwise) has no prime factor
-    above 31; every other reference-valid size is SPK_ERR_UNSUPPORTED."""
+    above 31 or splits into two factors of at most 480 (lines of at most 12
+    stages); every other reference-valid size is SPK_ERR_UNSUPPORTED."""
     lib = _lib.load()
     out = ctypes.c_void_p()
     fmt = _lib.FMT_CF32LE if cplx else _lib.FMT_S16LE
@@ -120,7 +147,7 @@ def test_fft_size_acceptance_rule(cplx):
         Nc = N if cplx or N % 2 else N // 2
         pow2 = N & (N - 1) == 0
         planned = pow2 and ((1024 <= N <= 262144) if cplx else (1024 <= N <= 524288))
-        mixed = 16 <= N <= 2 ** 20 and _largest_prime(Nc) <= 31
+        mixed = 16 <= N <= 2 ** 20 and (_largest_prime(Nc) <= 31 or _four_step(Nc))
         want = 3 if (planned or mixed) else 2
         cfg = _lib.SpkConfig(N, fmt, 0.0, 8, 16, 0)
         assert lib.spk_create(ctypes.byref(cfg), ctypes.byref(out)) == want, (N, cplx)

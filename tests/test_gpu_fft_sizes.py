@@ -50,6 +50,10 @@ CASES = [
     # odd real N: an N-point complex transform of the samples, no untangle
     ("s16le", 1155),      # 3 5 7 11
     ("u8", 1001),         # 7 11 13
+    # prime factors above 31: the four-step form with a line of that length
+    ("cf32le", 1004),     # 4 x 251
+    ("s16le", 296),       # real, Nc 148: 4 x 37
+    ("u8", 999),          # odd real: 27 x 37
     # complex sizes that are not multiples of 8
     ("cf32le", 1020),     # 4 3 5 17
     ("cf32le", 1014),     # 2 3 13 13
@@ -128,7 +132,7 @@ def test_quantized_records_mixed_radix():
 
 
 def test_unsupported_sizes_rejected():
-    for N, fmt in [(1004, _lib.FMT_S16LE), (8 * 37, _lib.FMT_CF32LE)]:
+    for N, fmt in [(2012, _lib.FMT_S16LE), (1006, _lib.FMT_CF32LE)]:
         with pytest.raises(Exception):
             DspState.init(DspParams(complex=fmt == _lib.FMT_CF32LE, fs_in=1.0, fft_size=N, input_format=fmt))
 
