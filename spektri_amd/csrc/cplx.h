@@ -223,6 +223,77 @@ __global__ void untangle_kernel(UntangleParams P) {
   else if (2 * k != P.M) Z[km] = Xm;  // (k = M/2 is its own mirror)
 }
 
+// Real input through the row and column kernels (N = 2^17..2^19): the
+// column transforms and the untangle in one launch.  Workgroup w of a block
+// takes columns k1 = 32 w + i (slots i < 32) and their mirrors N1 - k1
+// (slots 32 + i; slot 32 of workgroup 0 takes column N1/2, whose mirror is
+// itself, as column 0's is), so both Z[k] and Z[M - k] of every bin it owns
+// are in its LDS: bin k = k1 + N1 m has its mirror in column N1 - k1 at
+// m' = 63 - m (column 0: m' = -m mod 64).  Each column's 64-point DFT runs
+// as in cplx_col_kernel; then every thread forms X[k] = E + W_N^k O for 16
+// bins (untangle_kernel's formulas; W_N^k = W_N^k1 W_128^m from the table
+// of W_N^j, j <= N/4) and writes the spectrum row, X[M] included.
+// Grid: nblocks * N1 / 64 workgroups.
+__global__ __launch_bounds__(kChanThreads) void cplx_col_untangle_kernel(CplxColParams P, const cpx* __restrict__ tu) {
+  __shared__ cpx lds[64 * 16 * 4 + 64];  // [c][q][s], then [c][m] (stride 65)
+  const int c = threadIdx.x & 63, s = threadIdx.x >> 6;
+  const int wpb = P.N1 / 64;  // workgroups per block
+  const int g = (int)blockIdx.x / wpb, w = (int)blockIdx.x % wpb;
+  auto col_of = [&](int slot) {
+    if (slot < 32) return 32 * w + slot;
+    return (w == 0 && slot == 32) ? P.N1 / 2 : P.N1 - 32 * w - (slot - 32);
+  };
+  const int k1 = col_of(c);
+  const cpx* y = P.Y + (long long)g * 64 * P.N1 + k1;
+  cpx z[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) z[j] = y[(long long)(s + 4 * j) * P.N1];
+  dft<16, -1>(z);
+#pragma unroll
+  for (int q = 1; q < 16; ++q) z[q] = cmul(z[q], P.tw64[(s * q) & 63]);
+  auto la = [](int cc, int q, int ss) { return cc * 65 + q * 4 + ss; };
+#pragma unroll
+  for (int q = 0; q < 16; ++q) lds[la(c, q, s)] = z[q];
+  __syncthreads();
+  cpx zb[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int q = 4 * s + i;
+#pragma unroll
+    for (int ss = 0; ss < 4; ++ss) zb[i][ss] = lds[la(c, q, ss)];
+    dft<4, -1>(zb[i]);
+  }
+  __syncthreads();  // every exchange value has been read
+  // Z[k1 + N1 m] of slot c at lds[c * 65 + m]
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int p = 0; p < 4; ++p) lds[c * 65 + 4 * s + i + 16 * p] = zb[i][p];
+  __syncthreads();
+  // untangle: this thread's slot c, bins m = s + 4 j
+  const int M = 64 * P.N1;
+  const cpx wk1 = tu[k1];  // W_N^k1 (k1 < N1 <= N/4)
+  const bool self0 = k1 == 0, selfh = k1 == P.N1 / 2;
+  const int c2 = (self0 || selfh) ? c : (c ^ 32);
+  cpx* out = P.spec + (long long)g * P.spec_stride + k1;
+#pragma unroll 4
+  for (int j = 0; j < 16; ++j) {
+    const int m = s + 4 * j;
+    const int m2 = self0 ? ((64 - m) & 63) : 63 - m;
+    const cpx A = lds[c * 65 + m], B = lds[c2 * 65 + m2];
+    // W_128^m = W_N^(N1 m): the table holds N1 m' for m' <= 32; beyond,
+    // W_128^m = -conj(W_128^(64 - m))
+    cpx w128 = tu[(long long)P.N1 * (m <= 32 ? m : 64 - m)];
+    if (m > 32) w128 = cpx{-w128.x, w128.y};
+    const cpx W = cmul(wk1, w128);
+    const cpx E = {0.5f * (A.x + B.x), 0.5f * (A.y - B.y)};
+    const cpx O = {0.5f * (A.y + B.y), -0.5f * (A.x - B.x)};
+    const cpx WO = cmul(W, O);
+    out[(long long)P.N1 * m] = cpx{E.x + WO.x, E.y + WO.y};
+    if (self0 && m == 0) P.spec[(long long)g * P.spec_stride + M] = cpx{E.x - WO.x, WO.y - E.y};  // Nyquist bin
+  }
+}
+
 // Welch records from the kept spectra (spectrum.rs:95-116) for real input,
 // K = N/2 + 1 bins, and for complex input of any N (K = N: get_bin wraps
 // without a mirror): one thread per four bins k0..k0+3 (two 16-byte loads

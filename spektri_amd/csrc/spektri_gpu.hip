@@ -1334,12 +1334,18 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
   Q.spec = c->spec.as<cpx>();
   Q.spec_stride = c->spec_stride;
   Q.tw64 = c->ctw64.as<cpx>();
-  // 64 (block, column) pairs per workgroup
-  hipLaunchKernelGGL(spk::cplx_col_kernel, dim3((unsigned)(((long long)nb * N1 + 63) / 64)), dim3(spk::kChanThreads),
-                     0, s, Q);
+  if (c->complex_in) {
+    // 64 (block, column) pairs per workgroup
+    hipLaunchKernelGGL(spk::cplx_col_kernel, dim3((unsigned)(((long long)nb * N1 + 63) / 64)),
+                       dim3(spk::kChanThreads), 0, s, Q);
+  } else {
+    // real input (N1 >= 1024): columns with their mirror columns, untangled in the same launch
+    hipLaunchKernelGGL(spk::cplx_col_untangle_kernel, dim3((unsigned)((long long)nb * (N1 / 64))),
+                       dim3(spk::kChanThreads), 0, s, Q, c->ctu.as<cpx>());
+  }
   HIP_TRY(c, hipGetLastError());
   }
-  if (!c->complex_in && !c->real_odd) {
+  if (!c->complex_in && !c->real_odd && c->gfft) {
     spk::UntangleParams U{};
     U.spec = c->spec.as<cpx>();
     U.spec_stride = c->spec_stride;
