@@ -167,11 +167,11 @@ def make_input(wl, dev):
 def fwd_kernel_name(wl, all_fused):
     pow2 = (wl.fft_size & (wl.fft_size - 1)) == 0
     if not pow2:
-        return "gfft passes" + ("" if wl.complex else " + untangle_kernel")
+        return "gfft passes" + ("" if wl.complex or wl.fft_size % 2 else " + untangle_kernel")
     if wl.complex:
         return "cplx_row_kernel + cplx_col_kernel"
     if wl.fft_size > 65536:
-        return "cplx_row_kernel + cplx_col_kernel + untangle_kernel"
+        return "cplx_row_kernel + cplx_col_untangle_kernel"
     return f"fwd_real_kernel<M={wl.fft_size // 2}>" + (" (channel stage fused)" if all_fused else "")
 
 
