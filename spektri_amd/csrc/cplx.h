@@ -100,6 +100,164 @@ __global__ __launch_bounds__(WGT) void cplx_row_kernel(CplxRowParams P) {
   }
 }
 
+// Row stride (points) of the persistent form's staged rows: the padded row,
+// rounded up to 32 / TPW mod 32, so that the staging stores of a wave (TPW
+// rows, 64 / TPW consecutive n1 each) fall in distinct 8-byte bank pairs.
+template <int N1, int TPW>
+constexpr int cplx_row_stride() {
+  constexpr int base = lds_padded_size(N1), want = 32 / TPW;
+  return base + (((want - base) % 32) + 32) % 32;
+}
+
+// Persistent form of cplx_row_kernel for long rows (N1 = 2048 and 4096,
+// whose staged rows leave room for one workgroup per CU, so a workgroup's
+// loads cannot overlap another's transform).  Workgroup w serves XCD
+// x = w mod 8 and keeps one tile of TPW rows for all its blocks
+// g = x + 8 i, i = par, par + npar, ... (the TILES * npar workgroups of an
+// XCD take its blocks npar at a time, every tile of a block together).
+// With the rows fixed, every twiddle is fixed: the pass-2 and pass-3 table
+// entries are loaded into registers once, and the output factor
+// W_N^{n2 k1} = ta[n2][k1 mod 64] tb[n2][k1 / 64] is one per-thread ta
+// entry (k1 mod 64 = t mod 64 for every point of a thread) times tb entries
+// that are uniform over the wave.  The only vector loads inside the loop are
+// then the next block's input, issued as soon as the current block's input
+// is staged in LDS, so they are in flight for the whole transform.  Same
+// arithmetic as cplx_row_kernel, so the rows are bit-identical.
+// Grid: 8 * TILES * npar.
+template <int N1, int R, int R1, int R2, int R3, bool S16, int WGT>
+__global__ __launch_bounds__(WGT) void cplx_row_pf_kernel(CplxRowParams P) {
+  constexpr int T = N1 / R;
+  static_assert(R1 == R && R2 == R && R1 * R2 * R3 == N1, "one butterfly per thread in passes 1 and 2");
+  static_assert(R3 == R || R3 < 8, "pass 3: one radix-R butterfly, or R / R3 small ones");
+  constexpr int TPW = WGT / T, TILES = 64 / TPW, P3 = R1 * R2, PL = N1 / R3;
+  static_assert(T % 64 == 0 && PL % 64 == 0 && TPW >= 2 && WGT % T == 0, "lane layout");
+  constexpr int LP = cplx_row_stride<N1, TPW>();
+  __shared__ float lds_all[TPW * LP * 2];
+  using In = std::conditional_t<S16, int, cpx>;
+
+  const int x = (int)blockIdx.x % 8, wj = (int)blockIdx.x / 8;
+  const int tile = wj % TILES, par = wj / TILES, npar = (int)gridDim.x / (8 * TILES);
+  const int tl = threadIdx.x / T, t = threadIdx.x % T;
+  const int n20 = tile * TPW;
+  const int n2 = __builtin_amdgcn_readfirstlane(n20 + tl);  // T is a multiple of the wave
+  float* lds = lds_all + tl * LP * 2;
+  cpx* rows = reinterpret_cast<cpx*>(lds_all);
+
+  // Twiddles in LDS, copied once: the pass-table rows a butterfly reads
+  // (compact_rows) and the tile's tb rows.
+  constexpr int C2 = compact_rows<R2>() * R1, C3 = compact_rows<R3>() * P3, CB = N1 / 64;
+  __shared__ cpx ltw[C2 + C3 + TPW * CB];
+  cpx* ltw2 = ltw;
+  cpx* ltw3 = ltw + C2;
+  cpx* ltb = ltw + C2 + C3;
+  for (int i = threadIdx.x; i < C2 + C3 + TPW * CB; i += WGT) {
+    if (i < C2)
+      ltw[i] = P.tw2[compact_row<R2>(i / R1) * R1 + i % R1];
+    else if (i < C2 + C3)
+      ltw[i] = P.tw3[compact_row<R3>((i - C2) / P3) * P3 + (i - C2) % P3];
+    else
+      ltw[i] = P.tb[(n20 + (i - C2 - C3) / CB) * CB + (i - C2 - C3) % CB];
+  }
+  const cpx ta = P.ta[n2 * 64 + (t & 63)];
+  const cpx* tbl = ltb + tl * CB + (t >> 6);
+
+  // Input of block g: blocks with g H >= O read only the stream, through a
+  // buffer resource at the tile's first sample (one lane offset register,
+  // the point offsets as constants); the others (before the loop only: the
+  // host takes this form when 8 H >= O, so every prefetched block qualifies)
+  // select the history or the stream per sample.
+  In pf[R];
+  const unsigned voff = (unsigned)(64 * (threadIdx.x / TPW) + threadIdx.x % TPW) * (unsigned)sizeof(In);
+  // (no block: an empty resource at the stream start, whose loads read
+  // nothing and return zero)
+  auto load_stream = [&](int g, bool any) {
+    const char* blk = static_cast<const char*>(P.src) +
+                      (any ? ((long long)g * P.H - P.O + n20) * (long long)sizeof(In) : 0LL);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<char*>(blk), (short)0, any ? 64 * N1 * (int)sizeof(In) : 0, kBufferRsrcWord3);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int so = k * 64 * T * (int)sizeof(In);
+      if constexpr (S16)
+        pf[k] = (int)__builtin_amdgcn_raw_buffer_load_b32(rs, voff, so, 0);
+      else
+        pf[k] = __builtin_bit_cast(cpx, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, so, 0));
+    }
+  };
+
+  int g = x + 8 * par;
+  if (g >= P.nblocks) return;  // whole workgroup
+  if ((long long)g * P.H >= P.O) {
+    load_stream(g, true);
+  } else {
+    const In* src = static_cast<const In*>(P.src);
+    const In* hist = static_cast<const In*>(P.hist);
+    const long long s0 = (long long)g * P.H - P.O;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int f = k * WGT + (int)threadIdx.x;
+      const long long si = s0 + 64LL * (f / TPW) + n20 + f % TPW;
+      pf[k] = si < 0 ? hist[P.O + si] : src[si];
+    }
+  }
+  // Staging address of point k: the lane's base plus a constant (k T is a
+  // multiple of 32, so lds_pad(k T + a) = lds_pad(a) + k (T + T / 32)).
+  // The staging closes the loop body, after the output stores, so that the
+  // waits for the prefetched words count the same stores on every path (a
+  // staging at the top would be reached with and without them and wait for
+  // the stores as well).
+  const int stage0 = (int)(threadIdx.x % TPW) * LP + lds_pad((int)threadIdx.x / TPW);
+  auto stage = [&]() {
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      cpx a;
+      if constexpr (S16) {
+        a = {(float)(int16_t)pf[k], (float)(pf[k] >> 16)};
+      } else {
+        a = pf[k];
+      }
+      rows[stage0 + k * (T + T / 32)] = cscale(a, P.wscale);
+    }
+  };
+  __syncthreads();  // the twiddle copies are done
+  stage();
+  for (;;) {
+    const int gn = g + 8 * npar;
+    load_stream(gn, gn < P.nblocks);
+    __syncthreads();
+    cpx v[R];
+#pragma unroll
+    for (int e = 0; e < R; ++e) v[e] = reinterpret_cast<const cpx*>(lds)[lds_pad(st_in_index<N1, R, T, R1>(t, e))];
+    st_butterflies<N1, R, T, R1, 1, -1>(v, t, nullptr);
+    tile_exchange<T>(
+        v, lds, [&](int e) { return st_out_addr<N1, R, T, R1, 1>(t, e); },
+        [&](int e) { return st_in_addr<N1, R, T, R2>(t, e); });
+    apply_compact_twiddles<R, R1>(v, ltw2, t & (R1 - 1));
+    dft<R, -1>(v);
+    tile_exchange<T>(
+        v, lds, [&](int e) { return st_out_addr<N1, R, T, R2, R1>(t, e); },
+        [&](int e) { return st_in_addr<N1, R, T, R3>(t, e); });
+#pragma unroll
+    for (int u = 0; u < R / R3; ++u) {
+      cpx b[R3];
+#pragma unroll
+      for (int j = 0; j < R3; ++j) b[j] = v[u * R3 + j];
+      apply_compact_twiddles<R3, P3>(b, ltw3, (t + T * u) & (P3 - 1));
+      dft<R3, -1>(b);
+#pragma unroll
+      for (int q = 0; q < R3; ++q) v[u * R3 + q] = b[q];
+    }
+    cpx* y = P.Y + ((long long)g * 64 + n2) * N1;
+#pragma unroll
+    for (int e = 0; e < R; ++e)
+      y[t + T * (e / R3) + (e % R3) * PL] = cmul(v[e], cmul(ta, tbl[(T / 64) * (e / R3) + (PL / 64) * (e % R3)]));
+    if (gn >= P.nblocks) break;
+    __syncthreads();  // the transform has read its rows
+    stage();
+    g = gn;
+  }
+}
+
 struct CplxColParams {
   const cpx* Y;  // [block][64][N1]
   int N1;

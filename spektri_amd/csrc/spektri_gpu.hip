@@ -1121,13 +1121,26 @@ spk_status launch_batched(spk_ctx* c, int nb, hipStream_t s) {
 // Kernels only (no memset or copy nodes), so it can be captured into a
 // graph; every per-call quantity the kernels need is in device memory.
 template <int N1, int R, int R1, int R2, int R3>
-void launch_cplx_rows(const spk::CplxRowParams& P, int nb, bool s16, hipStream_t s) {
+void launch_cplx_rows(const spk::CplxRowParams& P, int nb, bool s16, int ncu, hipStream_t s) {
   // at least four rows per workgroup (rows of 4096 points: 1024 threads),
   // eight for rows of 2048 points (512 threads, 135 KB of LDS: a cache line
   // of s16 pairs then serves four workgroups instead of eight; rx888_256k
   // 0.605 -> 0.569 ms per step)
   constexpr int WGT = N1 == 2048 ? 8 * (N1 / R) : (N1 / R >= spk::kChanThreads ? 4 * (N1 / R) : spk::kChanThreads);
   constexpr int TPW = WGT / (N1 / R);
+  if constexpr (N1 >= 2048) {
+    if (8LL * P.H >= P.O) {
+      // one workgroup per CU: the persistent form, npar workgroups per
+      // (XCD, row tile); its prefetched blocks (g >= 8) lie past the history
+      constexpr int TILES = 64 / TPW;
+      const int grid = 8 * TILES * std::max(1, ncu / (8 * TILES));
+      if (s16)
+        hipLaunchKernelGGL((spk::cplx_row_pf_kernel<N1, R, R1, R2, R3, true, WGT>), dim3(grid), dim3(WGT), 0, s, P);
+      else
+        hipLaunchKernelGGL((spk::cplx_row_pf_kernel<N1, R, R1, R2, R3, false, WGT>), dim3(grid), dim3(WGT), 0, s, P);
+      return;
+    }
+  }
   const int grid = 8 * ((nb + 7) / 8) * (64 / TPW);
   if (s16)
     hipLaunchKernelGGL((spk::cplx_row_kernel<N1, R, R1, R2, R3, true, WGT>), dim3(grid), dim3(WGT), 0, s, P);
@@ -1135,17 +1148,17 @@ void launch_cplx_rows(const spk::CplxRowParams& P, int nb, bool s16, hipStream_t
     hipLaunchKernelGGL((spk::cplx_row_kernel<N1, R, R1, R2, R3, false, WGT>), dim3(grid), dim3(WGT), 0, s, P);
 }
 
-hipError_t dispatch_cplx_rows(int N1, const spk::CplxRowParams& P, int nb, bool s16, hipStream_t s) {
+hipError_t dispatch_cplx_rows(int N1, const spk::CplxRowParams& P, int nb, bool s16, int ncu, hipStream_t s) {
   switch (N1) {
-    case 16: launch_cplx_rows<16, 4, 4, 4, 1>(P, nb, s16, s); break;
-    case 32: launch_cplx_rows<32, 8, 8, 4, 1>(P, nb, s16, s); break;
-    case 64: launch_cplx_rows<64, 8, 8, 8, 1>(P, nb, s16, s); break;
-    case 128: launch_cplx_rows<128, 16, 16, 8, 1>(P, nb, s16, s); break;
-    case 256: launch_cplx_rows<256, 16, 16, 16, 1>(P, nb, s16, s); break;
-    case 512: launch_cplx_rows<512, 32, 32, 16, 1>(P, nb, s16, s); break;
-    case 1024: launch_cplx_rows<1024, 32, 32, 32, 1>(P, nb, s16, s); break;
-    case 2048: launch_cplx_rows<2048, 32, 32, 32, 2>(P, nb, s16, s); break;
-    case 4096: launch_cplx_rows<4096, 16, 16, 16, 16>(P, nb, s16, s); break;
+    case 16: launch_cplx_rows<16, 4, 4, 4, 1>(P, nb, s16, ncu, s); break;
+    case 32: launch_cplx_rows<32, 8, 8, 4, 1>(P, nb, s16, ncu, s); break;
+    case 64: launch_cplx_rows<64, 8, 8, 8, 1>(P, nb, s16, ncu, s); break;
+    case 128: launch_cplx_rows<128, 16, 16, 8, 1>(P, nb, s16, ncu, s); break;
+    case 256: launch_cplx_rows<256, 16, 16, 16, 1>(P, nb, s16, ncu, s); break;
+    case 512: launch_cplx_rows<512, 32, 32, 16, 1>(P, nb, s16, ncu, s); break;
+    case 1024: launch_cplx_rows<1024, 32, 32, 32, 1>(P, nb, s16, ncu, s); break;
+    case 2048: launch_cplx_rows<2048, 32, 32, 32, 2>(P, nb, s16, ncu, s); break;
+    case 4096: launch_cplx_rows<4096, 16, 16, 16, 16>(P, nb, s16, ncu, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1326,7 +1339,7 @@ spk_status enqueue_complex(spk_ctx* c, const void* src, int nb, hipStream_t s, h
   P.tw3 = c->ctw3.as<cpx>();
   P.ta = c->cta.as<cpx>();
   P.tb = c->ctb.as<cpx>();
-  HIP_TRY(c, dispatch_cplx_rows(N1, P, nb, c->s16_pairs, s));
+  HIP_TRY(c, dispatch_cplx_rows(N1, P, nb, c->s16_pairs, c->ncu, s));
   spk::CplxColParams Q{};
   Q.Y = c->cY.as<cpx>();
   Q.N1 = N1;

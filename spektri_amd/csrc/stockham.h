@@ -75,6 +75,40 @@ __device__ __forceinline__ void apply_pass_twiddles(cpx (&v)[r], const cpx* __re
   }
 }
 
+// The rows of a pass table that apply_pass_twiddles reads (all r - 1 for
+// r < 8; j = 1, 2, 3 and 4a otherwise), stored one after the other: a
+// compact copy, for tables kept in LDS.  compact_row(s) is the table row
+// (j - 1) of compact slot s.
+template <int r>
+__host__ __device__ constexpr int compact_rows() {
+  return r < 8 ? r - 1 : r / 4 + 2;
+}
+template <int r>
+__host__ __device__ constexpr int compact_row(int s) {
+  return (r < 8 || s < 3) ? s : 4 * (s - 2) - 1;
+}
+
+// apply_pass_twiddles over a compact copy of the table (same products).
+template <int r, int p>
+__device__ __forceinline__ void apply_compact_twiddles(cpx (&v)[r], const cpx* tw, int k) {
+  if constexpr (r < 8) {
+#pragma unroll
+    for (int j = 1; j < r; ++j) v[j] = cmul(v[j], tw[(j - 1) * p + k]);
+  } else {
+    cpx lo[4], hi[r / 4];
+#pragma unroll
+    for (int b = 1; b < 4; ++b) lo[b] = tw[(b - 1) * p + k];
+#pragma unroll
+    for (int a = 1; a < r / 4; ++a) hi[a] = tw[(2 + a) * p + k];
+#pragma unroll
+    for (int j = 1; j < r; ++j) {
+      const int a = j >> 2, b = j & 3;
+      const cpx w = (b == 0) ? hi[a] : (a == 0 ? lo[b] : cmul(lo[b], hi[a]));
+      v[j] = cmul(v[j], w);
+    }
+  }
+}
+
 // The table twiddles of one butterfly of pass (r, p) (r >= 8), loaded ahead
 // of their use: rows j = 1, 2, 3 and 4a as in apply_pass_twiddles.
 template <int r>
