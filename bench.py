@@ -364,7 +364,7 @@ def measure_design(wl, steps, warmup):
     bank = ChannelBank(wl.fft_size, wl.fs, [(f, L) for f, L, _ in wl.channels], x, ffts_per_buf=wl.ffts_per_buf)
     loss_fn = D.DesignLoss(wl, bank.nblocks, dev, torch.float32)
     w = D.initial_weights(wl, dev).requires_grad_(True)
-    opt = torch.optim.Adam([w], lr=1e-3)
+    opt = torch.optim.Adam([w], lr=1e-3, fused=True)  # one kernel per update
 
     def step():
         opt.zero_grad()

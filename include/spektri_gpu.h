@@ -214,7 +214,10 @@ spk_status spk_channel_records_packed(spk_ctx* ctx, uint64_t seq, uint64_t secs,
  * Valid only for contexts created with SPK_FLAG_KEEP_SPECTRA. */
 spk_status spk_spectra_device(spk_ctx* ctx, const float** dptr, uint64_t* stride);
 
-/* Clear history, Welch accumulators and counters (as a fresh DspState). */
+/* Clear history, Welch accumulators and counters (as a fresh DspState).
+ * Asynchronous: the device state is cleared on the context's stream after
+ * every call already enqueued there; the previous call's pending records
+ * are dropped. */
 spk_status spk_reset(spk_ctx* ctx);
 /* Run on the caller's HIP stream (hipStream_t passed as void*); NULL
  * returns to the context's own stream. */
@@ -310,8 +313,11 @@ spk_status spk_graph_info(spk_ctx* ctx, uint32_t* live, uint32_t* instantiated, 
  * last process call).  All pointers are device pointers; the calls are
  * asynchronous on the context's stream unless stated. */
 
-/* Set every channel's weights from a packed device array (synchronous: the
- * host copies are refreshed too). */
+/* Set every channel's weights from a packed device array: a copy on the
+ * context's stream (asynchronous, so weights must stay valid until the
+ * stream reaches it).  The host copies of the weights are read back only
+ * when a channel is added, removed or given host weights, or the channel
+ * set is laid out again (a wait on the stream then). */
 spk_status spk_set_weights_device(spk_ctx* ctx, const float* weights);
 /* Copy the last call's channel outputs into a packed device array of cap floats. */
 spk_status spk_channels_output_packed(spk_ctx* ctx, float* dst, size_t cap);

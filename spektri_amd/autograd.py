@@ -34,15 +34,19 @@ class _StageFn(torch.autograd.Function):
         bank = ctx.bank
         g = gout.contiguous().to(torch.float32)
         wbar = torch.empty(bank.weight_count, dtype=torch.float32, device=g.device)
-        # the library runs on its own stream: torch's work producing g must be done
-        torch.cuda.synchronize(g.device)
+        cur = bank._enter()
         bank.state.channels_adjoint(g.data_ptr(), wbar.data_ptr(), 0)
-        bank.state.synchronize()
+        bank._leave(cur, g, wbar)
         return wbar, None
 
 
 class ChannelBank:
-    """A fixed input stream and a channel set whose weights are trainable."""
+    """A fixed input stream and a channel set whose weights are trainable.
+
+    The library runs on a stream of its own, ordered against the caller's
+    current stream by stream waits (no host synchronisation): its work starts
+    after everything the caller queued before, and the caller's later work
+    starts after it."""
 
     def __init__(self, fft_size: int, fs: float, channels, samples: torch.Tensor, ffts_per_buf: int = 8,
                  averages: int | None = None):
@@ -56,6 +60,8 @@ class ChannelBank:
         self.state, _ = DspState.init(DspParams(complex=False, fs_in=fs, fft_size=fft_size,
                                                 ffts_per_buf=ffts_per_buf, spectrum_averages=averages or nblocks,
                                                 max_blocks=nblocks, keep_spectra=True))
+        self.stream = torch.cuda.Stream(self.samples.device)
+        self.state.set_stream(self.stream.cuda_stream)
         self.layout = []
         for first, L in channels:
             self.state.add_channel(int(first), int(L))
@@ -64,19 +70,32 @@ class ChannelBank:
         self.nblocks = nblocks
         self.output_floats = 2 * nblocks * sum(6 * (L // 8) for L in self.layout)
 
+    def _enter(self):
+        """The library's stream waits for the caller's current stream."""
+        cur = torch.cuda.current_stream(self.samples.device)
+        self.stream.wait_stream(cur)
+        return cur
+
+    def _leave(self, cur, *tensors):
+        """The caller's stream waits for the library's; the tensors the
+        library read or wrote are not reused before the library is done."""
+        cur.wait_stream(self.stream)
+        for t in tensors:
+            t.record_stream(self.stream)
+
     def _run(self, weights: torch.Tensor, out: torch.Tensor):
         w = weights.detach().contiguous().to(torch.float32)
         if w.numel() != self.weight_count or not w.is_cuda:
             raise ValueError(f"weights must be a device tensor of {self.weight_count} floats")
         st = self.state
-        torch.cuda.synchronize(w.device)
+        cur = self._enter()
         st.set_weights_device(w.data_ptr())
         st.reset()
         st.process_device(self.samples.data_ptr(), self.samples.numel())
         if st.last_blocks() != self.nblocks:
             raise _lib.SpektriError(_lib.SPK_OK, "unexpected block count")
         st.channels_output_packed(out.data_ptr(), self.output_floats)
-        st.synchronize()
+        self._leave(cur, w, out)
 
     def apply(self, weights: torch.Tensor) -> torch.Tensor:
         return _StageFn.apply(weights, self)

@@ -440,6 +440,9 @@ struct spk_ctx {
   std::vector<int> chan_w_off;          // float offsets of the live channels' weights
   std::map<int, ChanGroup> groups;
   bool chans_dirty = true;
+  // spk_set_weights_device wrote the weights on the device only: the
+  // channels' host copies (w) are read back before they are used or changed
+  bool w_device_newer = false;
 
   long long blocks_done = 0;  // stream blocks consumed so far
   uint32_t last_blocks = 0;
@@ -628,10 +631,16 @@ spk_status fuse_general(spk_ctx* c) {
 // Every captured graph and batch plan refers to the old buffers: the plans
 // are freed and the graphs marked stale (run_blocks updates a stale graph
 // before it is launched again).
+spk_status pull_weights(spk_ctx* c);
+
 spk_status rebuild_channels(spk_ctx* c) {
   // Tables are uploaded with null-stream copies, which do not wait for the
   // context stream: no launch of an earlier call may still read them.
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  {
+    spk_status st = pull_weights(c);
+    if (st != SPK_OK) return st;
+  }
   for (auto& kv : c->graphs) kv.second.stale = true;
   c->batches.clear();
   std::vector<float> wall;
@@ -1959,6 +1968,10 @@ spk_status spk_add_channel(spk_ctx* c, int64_t first_bin, uint32_t bins, const f
   if ((int)bins > c->N || (!find_chan_plan((int)bins) && (int)bins > spk::kGenBigMaxL))
     return fail(c, SPK_ERR_UNSUPPORTED,
                 "IFFT size must be <= N and at most 20480 (powers of two: 8192, or 16384)");
+  {
+    spk_status st = pull_weights(c);
+    if (st != SPK_OK) return st;
+  }
   Channel ch;
   ch.first = first_bin;
   ch.L = (int)bins;
@@ -1976,6 +1989,10 @@ spk_status spk_remove_channel(spk_ctx* c, uint32_t id) {
   try {
   if (!c) return SPK_ERR_INVALID;
   if (id >= c->chans.size() || c->chans[id].removed) return fail(c, SPK_ERR_INVALID, "no such channel");
+  {
+    spk_status st = pull_weights(c);
+    if (st != SPK_OK) return st;
+  }
   c->chans[id].removed = true;
   c->chans[id].w.clear();
   c->chans_dirty = true;
@@ -1989,6 +2006,10 @@ spk_status spk_set_channel_weights(spk_ctx* c, uint32_t id, const float* weights
   try {
   if (!c || !weights) return SPK_ERR_INVALID;
   if (id >= c->chans.size() || c->chans[id].removed) return fail(c, SPK_ERR_INVALID, "no such channel");
+  {
+    spk_status st = pull_weights(c);
+    if (st != SPK_OK) return st;
+  }
   Channel& ch = c->chans[id];
   ch.w.assign(weights, weights + ch.L);
   c->chans_dirty = true;
@@ -2318,8 +2339,8 @@ spk_status spk_spectra_device(spk_ctx* c, const float** dptr, uint64_t* stride) 
 spk_status spk_reset(spk_ctx* c) {
   try {
   if (!c) return SPK_ERR_INVALID;
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
-  // On the context's stream: it does not wait for the null stream.
+  // On the context's stream, after every call already enqueued there (no
+  // host wait): it does not wait for the null stream.
   HIP_TRY(c, hipMemsetAsync(c->hist.p, 0, c->hist.bytes, c->stream));
   HIP_TRY(c, hipMemsetAsync(c->carry.p, 0, c->carry.bytes, c->stream));
   HIP_TRY(c, hipMemsetAsync(c->pos.p, 0, c->pos.bytes, c->stream));
@@ -2393,17 +2414,12 @@ size_t packed_weight_floats(const spk_ctx* c) {
   return n;
 }
 
-spk_status spk_set_weights_device(spk_ctx* c, const float* w) {
-  try {
-  if (!c || !w) return SPK_ERR_INVALID;
-  if (c->chans.empty()) return fail(c, SPK_ERR_STATE, "no channels");
-  if (c->chans_dirty) {
-    spk_status st = rebuild_channels(c);
-    if (st != SPK_OK) return st;
-  }
+namespace {
+// The device weights (packed in channel id order, as uploaded by the last
+// rebuild) into the channels' host copies, once after spk_set_weights_device.
+spk_status pull_weights(spk_ctx* c) {
+  if (!c->w_device_newer) return SPK_OK;
   const size_t n = packed_weight_floats(c);
-  HIP_TRY(c, hipMemcpyAsync(c->weights.p, w, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-  // keep the host copies current so a later rebuild uploads the same weights
   std::vector<float> hw(n);
   HIP_TRY(c, hipMemcpyAsync(hw.data(), c->weights.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -2413,6 +2429,23 @@ spk_status spk_set_weights_device(spk_ctx* c, const float* w) {
     ch.w.assign(hw.begin() + off, hw.begin() + off + ch.L);
     off += (size_t)ch.L;
   }
+  c->w_device_newer = false;
+  return SPK_OK;
+}
+}  // namespace
+
+spk_status spk_set_weights_device(spk_ctx* c, const float* w) {
+  try {
+  if (!c || !w) return SPK_ERR_INVALID;
+  if (c->chans.empty()) return fail(c, SPK_ERR_STATE, "no channels");
+  if (c->chans_dirty) {
+    spk_status st = rebuild_channels(c);
+    if (st != SPK_OK) return st;
+  }
+  const size_t n = packed_weight_floats(c);
+  // stream-ordered, no wait: the host copies are read back when needed
+  HIP_TRY(c, hipMemcpyAsync(c->weights.p, w, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  c->w_device_newer = true;
   return SPK_OK;
   } catch (...) {
     return entry_exception(c);

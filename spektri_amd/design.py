@@ -48,17 +48,19 @@ def tone_terms(wl):
 class DesignLoss:
     """loss(ys) for per-channel outputs ys[c] of shape [nblocks * 3L/4, 2],
     or for the packed output tensor (every channel's output in channel order,
-    as ChannelBank.apply returns it), from which the rows are gathered with
-    one index per output length."""
+    as ChannelBank.apply returns it).
+
+    A row's tone has nu = m / L cycles per sample (m = j - L/2), so its
+    projection is L-periodic in n: p = DFT_L(fold)[m mod L] / n_out, where
+    fold[r] = sum_q y[r + q L] (the output zero-padded to whole periods).
+    One fold and one L-point FFT per channel give every row of that channel,
+    with a single pass over the outputs (no per-row phase tables)."""
 
     def __init__(self, wl, nblocks: int, device, dtype):
         self.rows = tone_terms(wl)
         if not self.rows:
             raise ValueError("no loss terms")
         self.device, self.dtype = device, dtype
-        # Rows of one output length are projected together, as a few
-        # whole-tensor operations over [rows, samples] (a loop over the rows
-        # would launch thousands of small kernels per training step).
         offs = np.cumsum([0] + [nblocks * (3 * L // 4) for _, L, _ in wl.channels])
         by_len = {}
         for k, (c, t, j, kind, a) in enumerate(self.rows):
@@ -68,50 +70,56 @@ class DesignLoss:
         order = []
         for L, ks in by_len.items():
             n_out = nblocks * (3 * L // 4)
-            n = np.arange(n_out, dtype=np.int64)
-            js = np.array([self.rows[k][2] for k in ks], dtype=np.int64)
-            # theta / 2 pi = (j - L/2) n / L, reduced exactly in integers
-            ph = np.mod(np.outer(js - L // 2, n), L).astype(np.float64) / L
-            chans = [self.rows[k][0] for k in ks]
-            uniq = sorted(set(chans))
+            uniq = sorted({self.rows[k][0] for k in ks})
             pos = {c: i for i, c in enumerate(uniq)}
-            pidx = torch.tensor(offs[chans][:, None] + n[None, :], device=device)
-            self.groups.append((uniq, torch.tensor([pos[c] for c in chans], device=device), pidx,
-                                torch.tensor(np.cos(2 * np.pi * ph), dtype=dtype, device=device),
-                                torch.tensor(np.sin(2 * np.pi * ph), dtype=dtype, device=device)))
+            starts = offs[uniq]
+            if len(uniq) == 1 or np.all(np.diff(starts) == n_out):
+                src = int(starts[0])  # the channels' outputs are one contiguous run
+            else:
+                src = torch.tensor(starts[:, None] + np.arange(n_out)[None, :], device=device)
+            # row k reads bin (j - L/2) mod L of its channel's folded transform
+            sel = torch.tensor([pos[self.rows[k][0]] * L + (self.rows[k][2] - L // 2) % L for k in ks],
+                               device=device)
+            self.groups.append((L, n_out, uniq, src, sel))
             order += ks
         # position of row k in the concatenated group results
         self.inv = torch.tensor(np.argsort(np.array(order)), device=device)
-        self.kind = torch.tensor([r[3] for r in self.rows], device=device)
+        kinds = np.array([r[3] for r in self.rows])
+        # index tensors, not boolean masks: a mask index waits for the device
+        self.stop_idx = torch.tensor(np.flatnonzero(kinds == 0), device=device)
+        self.pass_idx = torch.tensor(np.flatnonzero(kinds == 1), device=device)
         self.amp = torch.tensor([r[4] for r in self.rows], dtype=dtype, device=device)
 
     def projections(self, ys):
         pr, pi = [], []
         packed = isinstance(ys, torch.Tensor)
-        for uniq, idx, pidx, cs, sn in self.groups:
-            if packed:
-                # (per-channel views of a packed tensor would each give a
-                # full-size gradient in the backward pass)
-                # gather (whose backward is a scatter-add, not a sort)
-                y = torch.gather(ys, 0, pidx.reshape(-1, 1).expand(-1, 2)).view(*pidx.shape, 2).to(self.dtype)
+        for L, n_out, uniq, src, sel in self.groups:
+            nch = len(uniq)
+            if not packed:
+                y = torch.stack([ys[c] for c in uniq])
+            elif isinstance(src, int):
+                y = ys[src:src + nch * n_out].view(nch, n_out, 2)
             else:
-                y = torch.stack([ys[c].to(self.dtype) for c in uniq])[idx]
-            yr, yi = y[..., 0], y[..., 1]
-            # y e^{-i theta} = (yr cos + yi sin) + i (yi cos - yr sin)
-            pr.append((yr * cs + yi * sn).mean(1))
-            pi.append((yi * cs - yr * sn).mean(1))
+                # (gather, whose backward is a scatter-add, not a sort)
+                y = torch.gather(ys, 0, src.reshape(-1, 1).expand(-1, 2)).view(nch, n_out, 2)
+            y = y.to(self.dtype)
+            q = -(-n_out // L)
+            if q * L != n_out:
+                y = torch.nn.functional.pad(y, (0, 0, 0, q * L - n_out))
+            fold = y.reshape(nch, q, L, 2).sum(1)
+            p = torch.fft.fft(torch.complex(fold[..., 0], fold[..., 1]), dim=-1).reshape(-1)[sel] / n_out
+            pr.append(p.real)
+            pi.append(p.imag)
         return torch.cat(pr)[self.inv], torch.cat(pi)[self.inv]
 
     def __call__(self, ys):
         pr, pi = self.projections(ys)
         g2 = (pr * pr + pi * pi) / (self.amp * self.amp)
-        stop = g2[self.kind == 0]
-        pas = torch.sqrt(g2[self.kind == 1])
         terms = []
-        if stop.numel():
-            terms.append(stop.mean())
-        if pas.numel():
-            terms.append(((pas - 1.0) ** 2).mean())
+        if len(self.stop_idx):
+            terms.append(g2[self.stop_idx].mean())
+        if len(self.pass_idx):
+            terms.append(((torch.sqrt(g2[self.pass_idx]) - 1.0) ** 2).mean())
         return sum(terms)
 
 
