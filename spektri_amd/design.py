@@ -54,43 +54,49 @@ class DesignLoss:
     projection is L-periodic in n: p = DFT_L(fold)[m mod L] / n_out, where
     fold[r] = sum_q y[r + q L] (the output zero-padded to whole periods).
     One fold and one L-point FFT per channel give every row of that channel,
-    with a single pass over the outputs (no per-row phase tables)."""
+    with a single pass over the outputs (no per-row phase tables); channels
+    of one size between the first and the last with rows are folded as one
+    view of the packed outputs when they are contiguous there."""
 
     def __init__(self, wl, nblocks: int, device, dtype):
         self.rows = tone_terms(wl)
         if not self.rows:
             raise ValueError("no loss terms")
         self.device, self.dtype = device, dtype
-        offs = np.cumsum([0] + [nblocks * (3 * L // 4) for _, L, _ in wl.channels])
+        sizes = [L for _, L, _ in wl.channels]
+        offs = np.cumsum([0] + [nblocks * (3 * L // 4) for L in sizes])
         by_len = {}
         for k, (c, t, j, kind, a) in enumerate(self.rows):
-            L = wl.channels[c][1]
-            by_len.setdefault(L, []).append(k)
+            by_len.setdefault(sizes[c], []).append(k)
         self.groups = []
         order = []
         for L, ks in by_len.items():
             n_out = nblocks * (3 * L // 4)
-            uniq = sorted({self.rows[k][0] for k in ks})
-            pos = {c: i for i, c in enumerate(uniq)}
-            starts = offs[uniq]
-            if len(uniq) == 1 or np.all(np.diff(starts) == n_out):
-                src = int(starts[0])  # the channels' outputs are one contiguous run
+            chans = [self.rows[k][0] for k in ks]
+            lo, hi = min(chans), max(chans)
+            if all(sizes[c] == L for c in range(lo, hi + 1)):
+                # channels lo..hi are one contiguous run of equal outputs
+                uniq = list(range(lo, hi + 1))
+                src = int(offs[lo])
             else:
-                src = torch.tensor(starts[:, None] + np.arange(n_out)[None, :], device=device)
+                uniq = sorted(set(chans))
+                src = torch.tensor(offs[uniq][:, None] + np.arange(n_out)[None, :], device=device)
+            pos = {c: i for i, c in enumerate(uniq)}
             # row k reads bin (j - L/2) mod L of its channel's folded transform
             sel = torch.tensor([pos[self.rows[k][0]] * L + (self.rows[k][2] - L // 2) % L for k in ks],
                                device=device)
             self.groups.append((L, n_out, uniq, src, sel))
             order += ks
-        # position of row k in the concatenated group results
+        # rows in group order: position of row k, and the terms' rows
         self.inv = torch.tensor(np.argsort(np.array(order)), device=device)
-        kinds = np.array([r[3] for r in self.rows])
-        # index tensors, not boolean masks: a mask index waits for the device
-        self.stop_idx = torch.tensor(np.flatnonzero(kinds == 0), device=device)
-        self.pass_idx = torch.tensor(np.flatnonzero(kinds == 1), device=device)
-        self.amp = torch.tensor([r[4] for r in self.rows], dtype=dtype, device=device)
+        kinds = np.array([self.rows[k][3] for k in order])
+        # index tensors (a boolean mask index waits for the device)
+        self.stop_pos = torch.tensor(np.flatnonzero(kinds == 0), device=device)
+        self.pass_pos = torch.tensor(np.flatnonzero(kinds == 1), device=device)
+        self.amp = torch.tensor([self.rows[k][4] for k in order], dtype=dtype, device=device)
 
-    def projections(self, ys):
+    def _grouped(self, ys):
+        """Projections (re, im) of the rows in group order."""
         pr, pi = [], []
         packed = isinstance(ys, torch.Tensor)
         for L, n_out, uniq, src, sel in self.groups:
@@ -107,19 +113,27 @@ class DesignLoss:
             if q * L != n_out:
                 y = torch.nn.functional.pad(y, (0, 0, 0, q * L - n_out))
             fold = y.reshape(nch, q, L, 2).sum(1)
-            p = torch.fft.fft(torch.complex(fold[..., 0], fold[..., 1]), dim=-1).reshape(-1)[sel] / n_out
+            # (index_select: its backward adds into place, with no sort)
+            p = torch.fft.fft(torch.view_as_complex(fold), dim=-1).reshape(-1).index_select(0, sel) / n_out
             pr.append(p.real)
             pi.append(p.imag)
-        return torch.cat(pr)[self.inv], torch.cat(pi)[self.inv]
+        if len(pr) == 1:
+            return pr[0], pi[0]
+        return torch.cat(pr), torch.cat(pi)
+
+    def projections(self, ys):
+        """Projections (re, im) of every row, in row order."""
+        pr, pi = self._grouped(ys)
+        return pr.index_select(0, self.inv), pi.index_select(0, self.inv)
 
     def __call__(self, ys):
-        pr, pi = self.projections(ys)
+        pr, pi = self._grouped(ys)
         g2 = (pr * pr + pi * pi) / (self.amp * self.amp)
         terms = []
-        if len(self.stop_idx):
-            terms.append(g2[self.stop_idx].mean())
-        if len(self.pass_idx):
-            terms.append(((torch.sqrt(g2[self.pass_idx]) - 1.0) ** 2).mean())
+        if len(self.stop_pos):
+            terms.append(g2.index_select(0, self.stop_pos).mean())
+        if len(self.pass_pos):
+            terms.append(((torch.sqrt(g2.index_select(0, self.pass_pos)) - 1.0) ** 2).mean())
         return sum(terms)
 
 
