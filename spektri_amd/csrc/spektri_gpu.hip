@@ -433,6 +433,7 @@ struct spk_ctx {
   size_t weights_live = 0;   // floats of live channels' weights (then 256 zeros)
   int grounds = 0, fscat_nyq2 = 0;
   bool keep_spec = false;
+  bool keep_flag = false;  // created with SPK_FLAG_KEEP_SPECTRA (the adjoint's condition)
   bool fused = false;        // channels run inside the forward kernel
 
   std::vector<Channel> chans;
@@ -443,6 +444,10 @@ struct spk_ctx {
   // spk_set_weights_device wrote the weights on the device only: the
   // channels' host copies (w) are read back before they are used or changed
   bool w_device_newer = false;
+  // weights laid out one copy per channel from now on (spk_set_weights_device
+  // addresses them so); w_shared: the current layout shares copies
+  bool w_packed = false;
+  bool w_shared = false;
 
   long long blocks_done = 0;  // stream blocks consumed so far
   uint32_t last_blocks = 0;
@@ -648,14 +653,36 @@ spk_status rebuild_channels(spk_ctx* c) {
   c->chan_w_off.assign(c->chans.size(), -1);
   long long off = 0;
   std::vector<int>& w_off = c->chan_w_off;
+  // Channels of one IFFT size with bit-identical weights share one copy: the
+  // channel kernels re-read the weights every block, and a large set (2000
+  // wideband channels: 18 MB, four distinct vectors) would not stay in L2.
+  // The packed layout (one copy per channel, id order) is kept when the
+  // weights are addressed per channel: device uploads and the adjoint
+  // (contexts created with SPK_FLAG_KEEP_SPECTRA).
+  const bool share = !c->keep_flag && !c->w_packed;
+  std::map<std::vector<uint32_t>, int> seen;
+  size_t packed = 0;
   for (size_t i = 0; i < c->chans.size(); ++i) {
     const Channel& ch = c->chans[i];
     if (ch.removed) continue;
-    w_off[i] = (int)wall.size();
-    wall.insert(wall.end(), ch.w.begin(), ch.w.end());
+    packed += (size_t)ch.L;
     c->chan_out_off[i] = off;
     off += (long long)c->max_blocks * chan_out_len(ch.L);
+    if (share) {
+      std::vector<uint32_t> key(ch.w.size() + 1);
+      key[0] = (uint32_t)ch.L;
+      std::memcpy(key.data() + 1, ch.w.data(), ch.w.size() * sizeof(float));
+      auto it = seen.find(key);
+      if (it != seen.end()) {
+        w_off[i] = it->second;
+        continue;
+      }
+      seen.emplace(std::move(key), (int)wall.size());
+    }
+    w_off[i] = (int)wall.size();
+    wall.insert(wall.end(), ch.w.begin(), ch.w.end());
   }
+  c->w_shared = wall.size() < packed;
   {
     std::vector<spk::ChanRecDesc> recs;
     long long prefix = 0;
@@ -1783,7 +1810,8 @@ spk_status spk_create(const spk_config* cfg, spk_ctx** out) {
 
   // the two-step pipeline always keeps the spectra (Welch and the channel
   // stage read them)
-  c->keep_spec = c->two_step || (cfg->flags & SPK_FLAG_KEEP_SPECTRA) != 0;
+  c->keep_flag = (cfg->flags & SPK_FLAG_KEEP_SPECTRA) != 0;
+  c->keep_spec = c->two_step || c->keep_flag;
   // Persistent forward grid: resident workgroups per CU times CUs.
   int dev = 0, ncu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
@@ -2438,6 +2466,10 @@ spk_status spk_set_weights_device(spk_ctx* c, const float* w) {
   try {
   if (!c || !w) return SPK_ERR_INVALID;
   if (c->chans.empty()) return fail(c, SPK_ERR_STATE, "no channels");
+  if (!c->w_packed) {
+    c->w_packed = true;
+    if (c->w_shared) c->chans_dirty = true;  // lay the weights out per channel
+  }
   if (c->chans_dirty) {
     spk_status st = rebuild_channels(c);
     if (st != SPK_OK) return st;
@@ -2590,7 +2622,7 @@ spk_status channels_adjoint(spk_ctx* c, const float* ybar, float* wbar, float* x
                             size_t part_floats) {
   if (!c || !ybar || !wbar) return SPK_ERR_INVALID;
   if (!c->have_output || c->chans_dirty) return fail(c, SPK_ERR_STATE, "no output yet");
-  if (!c->keep_spec) return fail(c, SPK_ERR_STATE, "the adjoint needs a context created with SPK_FLAG_KEEP_SPECTRA");
+  if (!c->keep_flag) return fail(c, SPK_ERR_STATE, "the adjoint needs a context created with SPK_FLAG_KEEP_SPECTRA");
   const int nb = (int)c->last_blocks;
   size_t need = 0;
   for (const auto& kv : c->groups) {
