@@ -351,7 +351,10 @@ def measure_design(wl, steps, warmup):
     """Differentiable weight design (design_16k): a step is one training step
     -- forward path with the current weights, packed outputs, the design
     loss, the hand-written adjoint and an Adam update -- on the GPU.  The
-    adjoint launch is also timed alone (HIP events) with its algorithmic
+    step is timed launched eagerly and, after those steps, captured once
+    into a CUDA graph and replayed (the same work, without the host
+    launching each of its kernels); the replayed step is ``ms_per_step``.
+    The adjoint launch is also timed alone (HIP events) with its algorithmic
     bytes (gathered bins and output gradients read, weight gradient
     written)."""
     import torch
@@ -364,15 +367,19 @@ def measure_design(wl, steps, warmup):
     bank = ChannelBank(wl.fft_size, wl.fs, [(f, L) for f, L, _ in wl.channels], x, ffts_per_buf=wl.ffts_per_buf)
     loss_fn = D.DesignLoss(wl, bank.nblocks, dev, torch.float32)
     w = D.initial_weights(wl, dev).requires_grad_(True)
-    opt = torch.optim.Adam([w], lr=1e-3, fused=True)  # one kernel per update
+    # one kernel per update, and replayable from a graph
+    opt = torch.optim.Adam([w], lr=1e-3, fused=True, capturable=True)
 
-    def step():
-        opt.zero_grad()
+    def body():
         out = bank.apply(w)
         loss = loss_fn(out)
         loss.backward()
         opt.step()
         return loss
+
+    def step():
+        opt.zero_grad()
+        return body()
 
     for _ in range(warmup):
         step()
@@ -380,6 +387,24 @@ def measure_design(wl, steps, warmup):
     t0 = time.perf_counter()
     for _ in range(steps):
         loss = step()
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t0) * 1e3 / steps
+    # capture one step (gradients set to None first, so that the captured
+    # backward writes them rather than adding to them), then replay it; the
+    # last eager loss is dropped first: it keeps that step's autograd graph,
+    # and with it the weights' gradient accumulator of the eager stream,
+    # which the captured backward must not reuse
+    del loss
+    graph = torch.cuda.CUDAGraph()
+    opt.zero_grad(set_to_none=True)
+    with torch.cuda.graph(graph):
+        loss = body()
+    for _ in range(warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        graph.replay()
     torch.cuda.synchronize()
     ms_step = (time.perf_counter() - t0) * 1e3 / steps
     # the adjoint alone, on the library's stream
@@ -405,7 +430,9 @@ def measure_design(wl, steps, warmup):
     adj_bytes = nb * sum(L * 8 + (6 * (L // 8)) * 8 for _, L, _ in wl.channels) + nw * 4
     bank.close()
     return {"value": round(nb * wl.hop / (ms_step * 1e-3) / 1e6, 1), "ms_per_step": round(ms_step, 4),
-            "step": "training step: forward path, packed outputs, design loss, adjoint, Adam (host-timed)",
+            "eager_ms_per_step": round(eager_ms, 4),
+            "step": "training step: forward path, packed outputs, design loss, adjoint, Adam (host-timed; "
+                    "captured into a CUDA graph and replayed; eager_ms_per_step launched one kernel at a time)",
             "final_loss": float(loss.detach()),
             "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBPS,
                          "kernel": "chan_adjoint_kernel<256> + adjoint_reduce_kernel",

@@ -46,7 +46,9 @@ class ChannelBank:
     The library runs on a stream of its own, ordered against the caller's
     current stream by stream waits (no host synchronisation): its work starts
     after everything the caller queued before, and the caller's later work
-    starts after it."""
+    starts after it.  A training step can therefore be captured into a CUDA
+    graph (torch.cuda.graph) once the bank has run eagerly (its buffers are
+    allocated on first use) and replayed."""
 
     def __init__(self, fft_size: int, fs: float, channels, samples: torch.Tensor, ffts_per_buf: int = 8,
                  averages: int | None = None):
@@ -78,10 +80,12 @@ class ChannelBank:
 
     def _leave(self, cur, *tensors):
         """The caller's stream waits for the library's; the tensors the
-        library read or wrote are not reused before the library is done."""
+        library read or wrote are not reused before the library is done
+        (under graph capture the graph's memory pool keeps them)."""
         cur.wait_stream(self.stream)
-        for t in tensors:
-            t.record_stream(self.stream)
+        if not torch.cuda.is_current_stream_capturing():
+            for t in tensors:
+                t.record_stream(self.stream)
 
     def _run(self, weights: torch.Tensor, out: torch.Tensor):
         w = weights.detach().contiguous().to(torch.float32)

@@ -382,6 +382,25 @@ __device__ __forceinline__ void store_out(cpx* p, cpx v) {
   __builtin_nontemporal_store(v.y, &p->y);
 }
 
+// Zero up to four device buffers (32-bit words, then the 1-3 tail bytes;
+// allocations start 256-byte aligned).  A kernel rather than
+// hipMemsetAsync, so that a caller may capture the call into a graph.
+struct ZeroParams {
+  unsigned char* p[4];
+  size_t bytes[4];
+};
+__global__ void zero_buffers_kernel(ZeroParams P) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int b = 0; b < 4; ++b) {
+    unsigned char* p = P.p[b];
+    if (!p) continue;
+    const size_t nw = P.bytes[b] / 4;
+    for (size_t i = i0; i < nw; i += stride) reinterpret_cast<uint32_t*>(p)[i] = 0u;
+    for (size_t i = 4 * nw + i0; i < P.bytes[b]; i += stride) p[i] = 0;
+  }
+}
+
 // Symmetric in the two neighbours.
 __device__ __forceinline__ float welch_term(cpx c0, cpx cm, cpx cp) {
   const float cr = c0.x - 0.5f * (cm.x + cp.x);

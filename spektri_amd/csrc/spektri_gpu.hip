@@ -2368,10 +2368,19 @@ spk_status spk_reset(spk_ctx* c) {
   try {
   if (!c) return SPK_ERR_INVALID;
   // On the context's stream, after every call already enqueued there (no
-  // host wait): it does not wait for the null stream.
-  HIP_TRY(c, hipMemsetAsync(c->hist.p, 0, c->hist.bytes, c->stream));
-  HIP_TRY(c, hipMemsetAsync(c->carry.p, 0, c->carry.bytes, c->stream));
-  HIP_TRY(c, hipMemsetAsync(c->pos.p, 0, c->pos.bytes, c->stream));
+  // host wait): it does not wait for the null stream.  A kernel, not
+  // memsets, so that the call can be captured into a graph.
+  spk::ZeroParams Z{};
+  const DevBuf* bufs[3] = {&c->hist, &c->carry, &c->pos};
+  size_t most = 0;
+  for (int i = 0; i < 3; ++i) {
+    Z.p[i] = static_cast<unsigned char*>(bufs[i]->p);
+    Z.bytes[i] = bufs[i]->bytes;
+    most = std::max(most, bufs[i]->bytes);
+  }
+  const int grid = (int)std::min<size_t>((most / 4 + 255) / 256 + 1, (size_t)c->ncu * 4);
+  hipLaunchKernelGGL(spk::zero_buffers_kernel, dim3(grid), dim3(256), 0, c->stream, Z);
+  HIP_TRY(c, hipGetLastError());
   c->parity = 0;
   c->pend_nb = 0;  // dropped with the rest of the stream state
   c->blocks_done = 0;
