@@ -215,9 +215,10 @@ spk_status spk_channel_records_packed(spk_ctx* ctx, uint64_t seq, uint64_t secs,
 spk_status spk_spectra_device(spk_ctx* ctx, const float** dptr, uint64_t* stride);
 
 /* Clear history, Welch accumulators and counters (as a fresh DspState).
- * Asynchronous: the device state is cleared on the context's stream after
- * every call already enqueued there; the previous call's pending records
- * are dropped. */
+ * Asynchronous: the device state is cleared by a kernel on the context's
+ * stream after every call already enqueued there (no memset, so the call
+ * can be captured into a graph); the previous call's pending records are
+ * dropped. */
 spk_status spk_reset(spk_ctx* ctx);
 /* Run on the caller's HIP stream (hipStream_t passed as void*); NULL
  * returns to the context's own stream. */
