@@ -1,5 +1,5 @@
 """Channels added, removed and reweighted between replays of captured HIP
-graphs (``-m gpu``; SURVEY's planned subscription-driven channels,
+graphs (``-m gpu``; the reference's planned subscription-driven channels,
 spektri README.md:50-53, fcfb.rs:41-64,96-99): the executable graphs are
 updated in place (hipGraphExecUpdate) instead of being dropped and
 captured anew, and every call's channel outputs match the FP64 oracle for
