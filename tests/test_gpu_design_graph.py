@@ -53,7 +53,7 @@ def test_captured_design_step_replays_match_eager_steps():
         opt_e.zero_grad()
         le = float(body_e().detach())
         graph.replay()
-        lg = float(loss_g)
+        lg = float(loss_g.detach())
         we, wg = w_e.detach().clone(), w_g.detach().clone()
         torch.cuda.synchronize()
         dw = float((we - wg).abs().max() / we.abs().max())
