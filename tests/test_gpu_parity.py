@@ -438,7 +438,7 @@ def run_complex(x_c64: np.ndarray, N, F, A, channels, batched=False, graphs=Fals
     return st, ids
 
 
-@pytest.mark.parametrize("N", [1024, 2048, 4096, 16384, 65536, 131072, 262144])
+@pytest.mark.parametrize("N", [1024, 2048, 4096, 16384, 65536, 131072, 262144, 524288])
 def test_complex_input_every_size(N):
     """cf32 input: spectra, Welch records (A straddling the call) and channels
     of every IFFT size, including channels wrapping around the spectrum edge."""
